@@ -1,0 +1,506 @@
+"""MI355X-native multi-right-hand-side sparse solves (host mirror of BlockPowerFlow.jl).
+
+The entry points keep the reference's names, argument meaning and error
+behaviour; Julia's mutating ``f!`` becomes ``f_`` here:
+
+    block_gmres(A, B[, X0]; memory, M, N, ldiv, restart, reorthogonalization,
+                atol, rtol, itmax, timemax, verbose, history)   src/block_gmres.jl:127-154
+    block_gmres_(solver, A, B[, X0]; ...)                        src/block_gmres.jl:156-173
+    BlockGMRESSolver(A, B; memory)                               src/block_gmres.jl:62-68
+    block_bicgstab(A, B; C, N, atol, rtol, itmax, timemax, verbose, history)
+                                                                 src/block_bicgstab.jl:182-190
+    ilu0(A[, nrhs]), ic0(A[, nrhs]), ldiv_(y, P, x)              src/preconditioners_*.jl
+    CUSOLVERRF.*                                                 src/cusolverRF.jl
+
+Every computation runs in libbpf_amd.so (hand-written gfx950 kernels); this
+module only moves arguments across the C ABI.  ``HipMatrix`` (device dense,
+column-major) and ``HipSparseMatrixCSR`` (device CSR, int32 indices) play the
+roles of CuMatrix / CuSparseMatrixCSR (or ROCMatrix / ROCSparseMatrixCSR).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import BpfError, SingularException, check
+
+__all__ = [
+    "Context", "default_context", "HipMatrix", "HipSparseMatrixCSR",
+    "ilu0", "ic0", "ldiv_", "mul_", "block_gmres", "block_gmres_", "BlockGMRESSolver",
+    "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
+    "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
+]
+
+L = _lib.load()
+
+
+class Context:
+    """A device + stream (the implicit task-local device of CUDA.jl/AMDGPU.jl)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(L.bpf_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        check(L.bpf_ctx_synchronize(self.handle))
+
+    def timer_start(self):
+        check(L.bpf_timer_start(self.handle))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        check(L.bpf_timer_stop(self.handle, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            L.bpf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def device_count() -> int:
+    c = C.c_int()
+    check(L.bpf_device_count(C.byref(c)))
+    return c.value
+
+
+class HipMatrix:
+    """Dense device matrix, column-major (Fortran order), float64 or float32."""
+
+    def __init__(self, shape, dtype=np.float64, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        if isinstance(shape, int):
+            shape = (shape,)
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        nbytes = max(1, int(np.prod(self.shape))) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(L.bpf_malloc(self.ctx.handle, C.byref(p), nbytes))
+        self.ptr = p
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_numpy(cls, a, ctx: Context | None = None, dtype=None):
+        a = np.asarray(a)
+        dt = np.dtype(dtype) if dtype is not None else (a.dtype if a.dtype in (np.float32, np.float64) else np.float64)
+        a = np.asfortranarray(a, dtype=dt)
+        m = cls(a.shape, dt, ctx)
+        if a.size:
+            check(L.bpf_memcpy_h2d(m.ctx.handle, m.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return m
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        if out.size:
+            check(L.bpf_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def copy_(self, other: "HipMatrix"):
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(L.bpf_memcpy_d2d(self.ctx.handle, self.ptr, other.ptr, self.nbytes))
+        return self
+
+    def copy(self) -> "HipMatrix":
+        m = HipMatrix(self.shape, self.dtype, self.ctx)
+        return m.copy_(self)
+
+    @property
+    def nrows(self):
+        return self.shape[0]
+
+    @property
+    def ncols(self):
+        return self.shape[1] if len(self.shape) > 1 else 1
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            L.bpf_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self.shape[0]
+
+
+Array = HipMatrix.to_numpy
+
+
+class HipSparseMatrixCSR:
+    """Square device CSR matrix (int32 indices, float64 values)."""
+
+    def __init__(self, indptr, indices, data, n: int | None = None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+        indices = np.ascontiguousarray(indices, dtype=np.int32)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        n = len(indptr) - 1 if n is None else n
+        h = C.c_void_p()
+        check(L.bpf_csr_create(self.ctx.handle, n, int(indptr[-1]) if len(indptr) else 0,
+                               indptr.ctypes.data_as(_lib.ip), indices.ctypes.data_as(_lib.ip),
+                               data.ctypes.data_as(_lib.dp), C.byref(h)))
+        self.handle = h
+        self.n = n
+        self.indptr = indptr
+        self.indices = indices
+        self._data = data.copy()
+
+    @classmethod
+    def from_scipy(cls, A, ctx=None):
+        A = A.tocsr()
+        A.sort_indices()
+        return cls(A.indptr, A.indices, A.data, A.shape[0], ctx)
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    def nnz(self):
+        return int(self.indptr[-1])
+
+    @property
+    def nzVal(self) -> np.ndarray:
+        """Host copy of the values (A.nzVal)."""
+        return self._data.copy()
+
+    def set_values(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        if data.shape != (self.nnz(),):
+            raise ValueError("values must have nnz entries")
+        check(L.bpf_csr_set_values(self.handle, data.ctypes.data_as(_lib.dp)))
+        self._data = data.copy()
+
+    def norm_inf(self) -> float:
+        v = C.c_double()
+        check(L.bpf_csr_info(self.handle, None, None, C.byref(v)))
+        return v.value
+
+    def __matmul__(self, X: HipMatrix) -> HipMatrix:
+        Y = HipMatrix(X.shape, X.dtype, self.ctx)
+        mul_(Y, self, X)
+        return Y
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                L.bpf_csr_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def _dtype_code(dt):
+    return _lib.BPF_F32 if np.dtype(dt) == np.float32 else _lib.BPF_F64
+
+
+def mul_(Y: HipMatrix, A, X: HipMatrix, alpha: float = 1.0, beta: float = 0.0):
+    """mul!(Y, A, X[, alpha, beta]) for a CSR A (SpMM) or a preconditioner
+    (applies its inverse, the meaning block_bicgstab's N has)."""
+    if isinstance(A, _Preconditioner):
+        return ldiv_(Y, A, X)
+    if X.dtype != Y.dtype:
+        raise TypeError("X and Y must have the same element type")
+    p = X.ncols
+    check(L.bpf_spmm(A.ctx.handle, A.handle, _dtype_code(X.dtype), _lib.BPF_COL_MAJOR, p, X.ptr,
+                     X.nrows, Y.ptr, Y.nrows, alpha, beta))
+    return Y
+
+
+spmm = mul_
+
+
+# ---------------------------------------------------------------- preconditioners
+class _Preconditioner:
+    kind = _lib.BPF_PREC_ILU0
+
+    def __init__(self, A: HipSparseMatrixCSR, nrhs: int | None = None):
+        self.ctx = A.ctx
+        self.n = A.n
+        self.nrhs = nrhs
+        h = C.c_void_p()
+        check(L.bpf_prec_create(self.ctx.handle, A.handle, self.kind, C.byref(h)))
+        self.handle = h
+        self.pattern = (A.indptr, A.indices)
+
+    def factor_(self, A: HipSparseMatrixCSR):
+        """Numeric refactorization with A's current values (same pattern)."""
+        check(L.bpf_prec_factor(self.handle, A.handle))
+        return self
+
+    def factor_values(self) -> np.ndarray:
+        out = np.empty(int(self.pattern[0][-1]), dtype=np.float64)
+        check(L.bpf_prec_get_factor(self.handle, out.ctypes.data_as(_lib.dp)))
+        return out
+
+    @property
+    def P(self):
+        """The combined factor as (indptr, indices, values) on the host."""
+        return self.pattern[0], self.pattern[1], self.factor_values()
+
+    def levels(self):
+        a, b = C.c_int(), C.c_int()
+        check(L.bpf_prec_levels(self.handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                L.bpf_prec_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class ILU0(_Preconditioner):
+    """ILU(0) (NVIDIA_ILU0 / AMD_ILU0, src/preconditioners_nvidia.jl:37-47,
+    src/preconditioners_amd.jl:110-117)."""
+    kind = _lib.BPF_PREC_ILU0
+
+
+class IC0(_Preconditioner):
+    """IC(0) (NVIDIA_IC0 / AMD_IC0, src/preconditioners_nvidia.jl:1-11,
+    src/preconditioners_amd.jl:72-80)."""
+    kind = _lib.BPF_PREC_IC0
+
+
+def ilu0(A: HipSparseMatrixCSR, nrhs: int | None = None) -> ILU0:
+    """ilu0(A, nrhs) / ilu0(A): ILU(0) on A's pattern."""
+    return ILU0(A, nrhs)
+
+
+def ic0(A: HipSparseMatrixCSR, nrhs: int | None = None) -> IC0:
+    """ic0(A, nrhs) / ic0(A): IC(0) on A's lower pattern."""
+    return IC0(A, nrhs)
+
+
+def ldiv_(y, P, x=None):
+    """ldiv!(y, P, x): y = P^{-1} x for an ILU0/IC0 preconditioner or a
+    CUSOLVERRF factorization (src/preconditioners_nvidia.jl:49-59,
+    src/cusolverRF.jl:317-331).  ``ldiv_(rf, x)`` solves in place."""
+    if isinstance(y, CUSOLVERRF.CusolverRfLU) and x is None:
+        return CUSOLVERRF._ldiv_inplace(y, P)
+    if isinstance(P, CUSOLVERRF._RfBase):
+        return CUSOLVERRF._ldiv(y, P, x)
+    if y.dtype != x.dtype:
+        raise TypeError("x and y must have the same element type")
+    if np.dtype(x.dtype) != np.float64:
+        raise TypeError("column-major preconditioner apply takes float64 blocks")
+    check(L.bpf_prec_apply(P.handle, _lib.BPF_F64, _lib.BPF_COL_MAJOR, x.ncols, x.ptr, x.nrows,
+                           y.ptr, y.nrows))
+    return y
+
+
+# ---------------------------------------------------------------- stats
+class BlockGMRESStats:
+    """BlockGMRESStats (src/block_utils.jl:361-367) plus backward-error fields."""
+
+    def __init__(self):
+        self.niter = 0
+        self.solved = False
+        self.residuals: list[float] = []
+        self.timer = 0.0
+        self.status = "unknown"
+        self.refine_steps = 0
+        self.max_berr = math.nan
+
+    def __repr__(self):
+        return (f"{type(self).__name__}(niter={self.niter}, solved={self.solved}, "
+                f"status={self.status!r}, timer={self.timer:.3g}s)")
+
+
+class SimpleStats(BlockGMRESStats):
+    """SimpleStats (src/block_bicgstab.jl:15-21)."""
+
+
+_STATUS = {
+    _lib.BPF_STATUS_SOLVED: "solution good enough given atol and rtol",
+    _lib.BPF_STATUS_TIRED: "maximum number of iterations exceeded",
+    _lib.BPF_STATUS_OVERTIMED: "time limit exceeded",
+    _lib.BPF_STATUS_UNKNOWN: "unknown",
+}
+
+
+def _fill_stats(stats, st: _lib.SolverStats, hist, history):
+    stats.niter = st.niter
+    stats.solved = bool(st.solved)
+    stats.status = _STATUS.get(st.status, "unknown")
+    stats.timer = st.timer
+    stats.refine_steps = st.refine_steps
+    stats.max_berr = st.max_berr
+    stats.residuals = list(hist[: min(st.nresiduals, len(hist))]) if history else []
+
+
+def _params(memory=20, restart=False, reorthogonalization=False, atol=None, rtol=None, itmax=0,
+            timemax=math.inf, verbose=0, history=False, berr_tol=0.0, precision="fp64",
+            block_width=0, max_refine=10, inner_rtol=1e-6):
+    prm = _lib.SolverParams()
+    L.bpf_solver_params_default(C.byref(prm))
+    eps = math.sqrt(np.finfo(np.float64).eps)
+    prm.memory = int(memory)
+    prm.restart = int(bool(restart))
+    prm.reorthogonalization = int(bool(reorthogonalization))
+    prm.atol = eps if atol is None else float(atol)
+    prm.rtol = eps if rtol is None else float(rtol)
+    prm.itmax = int(itmax)
+    prm.timemax = 0.0 if (timemax is None or math.isinf(timemax)) else float(timemax)
+    prm.verbose = int(verbose)
+    prm.history = int(bool(history))
+    prm.berr_tol = float(berr_tol)
+    prm.precision = _lib.BPF_PRECISION_MIXED if precision in ("mixed", 1) else _lib.BPF_PRECISION_FP64
+    prm.block_width = int(block_width)
+    prm.max_refine = int(max_refine)
+    prm.inner_rtol = float(inner_rtol)
+    return prm
+
+
+# ---------------------------------------------------------------- block GMRES
+class BlockGMRESSolver:
+    """Workspace of block_gmres! (src/block_gmres.jl:21-68).  ``memory`` is
+    clamped to div(n, p) as in the reference; X aliases the returned solution."""
+
+    def __init__(self, A, B, memory: int = 5):
+        n, p = A.n, B.ncols
+        self.m, self.n, self.p = n, n, p
+        self.memory = max(1, min(n // max(p, 1), memory))
+        self.X = HipMatrix((n, p), np.float64, A.ctx)
+        self.stats = BlockGMRESStats()
+        self.warm_start = False
+        self._X0 = None
+
+    def warm_start_(self, X0: HipMatrix):
+        """warm_start!(solver, X0) (src/block_gmres.jl:70-79; the reference's
+        field typo Δx/ΔX is not reproduced)."""
+        if X0.shape != self.X.shape:
+            raise ValueError(f"X0 should have size ({self.n}, {self.p})")
+        self._X0 = X0
+        self.warm_start = True
+        return self
+
+
+def _check_system(A, B):
+    if A.shape[0] != A.shape[1]:
+        raise ValueError("System must be square")
+    if A.n != B.nrows:
+        raise ValueError("Inconsistent problem size")
+    if B.dtype != np.float64:
+        warnings.warn(f"eltype(B) = {B.dtype} differs from eltype(A) = float64; converting.")
+
+
+def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, X0=None, *, M=None,
+                 N=None, ldiv=False, restart=False, reorthogonalization=False, atol=None, rtol=None,
+                 itmax=0, timemax=math.inf, verbose=0, history=False, berr_tol=0.0,
+                 precision="fp64", block_width=0, max_refine=10, inner_rtol=1e-6):
+    """block_gmres!(solver, A, B[, X0]; kwargs...) -> solver (src/block_gmres.jl:156-411)."""
+    _check_system(A, B)
+    if M is not None:
+        raise NotImplementedError("left preconditioning (M) is not part of the multi-RHS path")
+    if N is not None and not ldiv:
+        raise TypeError("N must be applied with ldiv=true (an ILU0/IC0 object defines ldiv! only)")
+    if X0 is not None:
+        solver.warm_start_(X0)
+    prm = _params(solver.memory, restart, reorthogonalization, atol, rtol, itmax, timemax, verbose,
+                  history, berr_tol, precision, block_width, max_refine, inner_rtol)
+    st = _lib.SolverStats()
+    cap = 1 + (itmax if itmax > 0 else 2 * (A.n // max(1, B.ncols))) + 16
+    hist = np.zeros(cap)
+    x0 = solver._X0 if solver.warm_start else None
+    Nh = N.handle if N is not None else None
+    check(L.bpf_block_gmres_x0(A.ctx.handle, A.handle, Nh, B.ncols, B.ptr, B.nrows,
+                               x0.ptr if x0 is not None else None, x0.nrows if x0 is not None else 0,
+                               solver.X.ptr, solver.X.nrows, C.byref(prm), C.byref(st),
+                               hist.ctypes.data_as(_lib.dp), cap))
+    solver.warm_start = False
+    solver._X0 = None
+    _fill_stats(solver.stats, st, hist, history)
+    return solver
+
+
+def block_gmres(A: HipSparseMatrixCSR, B: HipMatrix, X0: HipMatrix | None = None, *, memory=20,
+                **kw):
+    """(X, stats) = block_gmres(A, B[, X0]; memory, M, N, ldiv, restart,
+    reorthogonalization, atol, rtol, itmax, timemax, verbose, history)
+    (src/block_gmres.jl:127-154).  Extra keywords: berr_tol, precision
+    ("fp64" | "mixed"), block_width, max_refine, inner_rtol."""
+    solver = BlockGMRESSolver(A, B, memory=memory)
+    block_gmres_(solver, A, B, X0, **kw)
+    return solver.X, solver.stats
+
+
+# ---------------------------------------------------------------- block BiCGSTAB
+class BlockBicgstabSolver:
+    """Workspace of block_bicgstab! (src/block_bicgstab.jl:25-58)."""
+
+    def __init__(self, A, B):
+        self.X = HipMatrix((A.n, B.ncols), np.float64, A.ctx)
+        self.stats = SimpleStats()
+
+
+def block_bicgstab_(solver, A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0,
+                    timemax=math.inf, verbose=0, history=False):
+    """block_bicgstab!(solver, A, B; C=B, N, ...) -> (X, stats)
+    (src/block_bicgstab.jl:187-299).  N (an ILU0/IC0 object) is applied as
+    mul!(Y, N, P), i.e. as its inverse."""
+    _check_system(A, B)
+    prm = _params(atol=atol, rtol=rtol, itmax=itmax, timemax=timemax, verbose=verbose, history=history)
+    st = _lib.SolverStats()
+    cap = 1 + (itmax if itmax > 0 else 2 * A.n) + 16
+    cap = min(cap, 1 << 20)
+    hist = np.zeros(cap)
+    Ch = C.ptr if C is not None else None
+    check(L.bpf_block_bicgstab(A.ctx.handle, A.handle, N.handle if N is not None else None, B.ncols,
+                               B.ptr, B.nrows, Ch, C.nrows if C is not None else 0, solver.X.ptr,
+                               solver.X.nrows, _ctypes_byref(prm), _ctypes_byref(st),
+                               hist.ctypes.data_as(_lib.dp), cap))
+    _fill_stats(solver.stats, st, hist, history)
+    return solver.X, solver.stats
+
+
+def _ctypes_byref(x):
+    return C.byref(x)
+
+
+def block_bicgstab(A, B, **kw):
+    """(X, stats) = block_bicgstab(A, B; C=B, N, atol, rtol, itmax, timemax,
+    verbose, history) (src/block_bicgstab.jl:182-185)."""
+    solver = BlockBicgstabSolver(A, B)
+    return block_bicgstab_(solver, A, B, **kw)
+
+
+def backward_error(A: HipSparseMatrixCSR, X: HipMatrix, B: HipMatrix) -> np.ndarray:
+    """Per-RHS FP64 backward error ||Ax-b||inf / (||A||inf ||x||inf + ||b||inf)."""
+    out = np.zeros(B.ncols)
+    check(L.bpf_backward_error(A.ctx.handle, A.handle, B.ncols, X.ptr, X.nrows, B.ptr, B.nrows,
+                               out.ctypes.data_as(_lib.dp)))
+    return out
+
+
+from .cusolverrf import CUSOLVERRF  # noqa: E402
+from .batched import BatchedSystems  # noqa: E402

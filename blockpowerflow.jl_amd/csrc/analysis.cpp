@@ -1,0 +1,121 @@
+// Host-side analysis: level sets of triangular operators (Anderson & Saad
+// 1989 level scheduling) and their level-ordered storage.
+#include <algorithm>
+#include <numeric>
+
+#include "bpf_internal.h"
+
+namespace bpf {
+
+std::vector<Seg> make_plan(const TriOp &op, int tail_rows) {
+  std::vector<Seg> plan;
+  int l = 0;
+  while (l < op.nlevels) {
+    const bool wide = (op.h_lvl[l + 1] - op.h_lvl[l]) > tail_rows;
+    int e = l + 1;
+    while (e < op.nlevels && ((op.h_lvl[e + 1] - op.h_lvl[e]) > tail_rows) == wide) ++e;
+    plan.push_back({l, e, wide});
+    l = e;
+  }
+  return plan;
+}
+
+void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
+                 const std::vector<int> &colidx, int kind, hipStream_t s) {
+  op.n = n;
+  op.unit = (kind == 0);
+  op.lower = (kind == 0 || kind == 2);
+  // Per-row entry lists (positions into the combined array) and diagonals.
+  std::vector<int> dpos(n, -1);
+  std::vector<int> cnt(n, 0);
+  for (int i = 0; i < n; ++i) {
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int j = colidx[k];
+      if (j == i) dpos[i] = k;
+      if (kind == 0 || kind == 2) { if (j < i) ++cnt[i]; }
+      else if (kind == 1) { if (j > i) ++cnt[i]; }
+      else { if (j < i) ++cnt[j]; }   // transpose: entry (i,j) -> row j
+    }
+  }
+  if (kind != 0)
+    for (int i = 0; i < n; ++i)
+      if (dpos[i] < 0) BPF_THROW(BPF_E_SINGULAR, "structurally zero diagonal in row %d", i);
+  // Row lists in original numbering (CSR of the operator).
+  std::vector<int> optr(n + 1, 0);
+  for (int i = 0; i < n; ++i) optr[i + 1] = optr[i] + cnt[i];
+  std::vector<int> ocol(optr[n]), opos(optr[n]);
+  std::vector<int> fill(optr.begin(), optr.end() - 1);
+  for (int i = 0; i < n; ++i) {
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int j = colidx[k];
+      if (kind == 0 || kind == 2) {
+        if (j < i) { ocol[fill[i]] = j; opos[fill[i]++] = k; }
+      } else if (kind == 1) {
+        if (j > i) { ocol[fill[i]] = j; opos[fill[i]++] = k; }
+      } else if (j < i) {
+        ocol[fill[j]] = i; opos[fill[j]++] = k;
+      }
+    }
+  }
+  // Levels.
+  std::vector<int> level(n, 0);
+  int nlev = 0;
+  if (op.lower) {
+    for (int i = 0; i < n; ++i) {
+      int lv = 0;
+      for (int k = optr[i]; k < optr[i + 1]; ++k) lv = std::max(lv, level[ocol[k]] + 1);
+      level[i] = lv;
+      nlev = std::max(nlev, lv + 1);
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int lv = 0;
+      for (int k = optr[i]; k < optr[i + 1]; ++k) lv = std::max(lv, level[ocol[k]] + 1);
+      level[i] = lv;
+      nlev = std::max(nlev, lv + 1);
+    }
+  }
+  if (n == 0) nlev = 0;
+  op.nlevels = nlev;
+  op.h_lvl.assign(nlev + 1, 0);
+  for (int i = 0; i < n; ++i) op.h_lvl[level[i] + 1]++;
+  for (int l = 0; l < nlev; ++l) op.h_lvl[l + 1] += op.h_lvl[l];
+  op.h_order.assign(n, 0);
+  {
+    std::vector<int> f(op.h_lvl.begin(), op.h_lvl.end() - 1);
+    for (int i = 0; i < n; ++i) op.h_order[f[level[i]]++] = i;
+  }
+  op.h_ptr.assign(n + 1, 0);
+  op.max_row_nnz = 0;
+  for (int t = 0; t < n; ++t) {
+    const int i = op.h_order[t];
+    const int len = optr[i + 1] - optr[i];
+    op.h_ptr[t + 1] = op.h_ptr[t] + len;
+    op.max_row_nnz = std::max(op.max_row_nnz, len);
+  }
+  op.h_col.resize(op.h_ptr[n]);
+  op.h_gather.resize(op.h_ptr[n]);
+  op.h_dgather.assign(n, 0);
+  for (int t = 0; t < n; ++t) {
+    const int i = op.h_order[t];
+    int o = op.h_ptr[t];
+    for (int k = optr[i]; k < optr[i + 1]; ++k, ++o) {
+      op.h_col[o] = ocol[k];
+      op.h_gather[o] = opos[k];
+    }
+    op.h_dgather[t] = dpos[i] < 0 ? 0 : dpos[i];
+  }
+  upload(op.order, op.h_order, s);
+  upload(op.ptr, op.h_ptr, s);
+  upload(op.col, op.h_col, s);
+  upload(op.gather, op.h_gather, s);
+  upload(op.dgather, op.h_dgather, s);
+  upload(op.lvl, op.h_lvl, s);
+  const size_t m = std::max<size_t>(1, op.h_col.size());
+  op.val.ensure(m * sizeof(double));
+  op.val32.ensure(m * sizeof(float));
+  op.diag.ensure(std::max(1, n) * sizeof(double));
+  op.diag32.ensure(std::max(1, n) * sizeof(float));
+}
+
+}  // namespace bpf

@@ -1,0 +1,197 @@
+// Internal declarations of libbpf_amd: handle structures, error plumbing,
+// device buffers, triangular-solve schedules and kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bpf.h"
+
+namespace bpf {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+
+struct Error {
+  int code;
+};
+
+#define BPF_THROW(code, ...)          \
+  do {                                \
+    ::bpf::set_error(__VA_ARGS__);    \
+    throw ::bpf::Error{code};         \
+  } while (0)
+
+#define BPF_CHECK_HIP(expr)                                                    \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      BPF_THROW(BPF_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                __FILE__, __LINE__);                                           \
+  } while (0)
+
+#define BPF_REQUIRE(cond, ...)                 \
+  do {                                         \
+    if (!(cond)) BPF_THROW(BPF_E_INVALID, __VA_ARGS__); \
+  } while (0)
+
+// Runs f() and converts exceptions into status codes.
+template <typename F>
+int guard(F &&f) {
+  try {
+    f();
+    return BPF_OK;
+  } catch (const Error &e) {
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return BPF_E_NOMEM;
+  } catch (...) {
+    set_error("unknown exception");
+    return BPF_E_INVALID;
+  }
+}
+
+// ---------------------------------------------------------------- buffers
+// Owning device buffer (hipMalloc); grows, never shrinks.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release();
+  void ensure(size_t nbytes);              // contents not preserved
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+void check_hip(hipError_t e, const char *what);
+
+// Synchronous upload of a host vector into a (grown) device buffer.
+template <typename T>
+void upload(DevBuf &b, const std::vector<T> &h, hipStream_t s) {
+  b.ensure((h.empty() ? 1 : h.size()) * sizeof(T));
+  if (!h.empty())
+    check_hip(hipMemcpyAsync(b.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(upload)");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(upload)");
+}
+
+}  // namespace bpf
+
+// ---------------------------------------------------------------- handles
+struct bpf_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bpf::DevBuf scratch;        // generic reduction scratch
+  bpf::DevBuf pinned_dummy;
+  double *h_small = nullptr;  // pinned host staging for small D2H reads
+  size_t h_small_cap = 0;
+  void ensure_small(size_t ndouble);
+};
+
+struct bpf_csr_s {
+  bpf_ctx ctx = nullptr;
+  int n = 0, nnz = 0;
+  std::vector<int> h_rowptr, h_colidx;
+  bpf::DevBuf rowptr, colidx, val, val32;
+  bpf::DevBuf rowlen_order;    // rows sorted for SpMM (unused: natural order)
+  double norm_inf = 0.0;       // ||A||_inf of the current values
+  bool val32_valid = false;
+  void refresh_derived();      // after a value change: norm, FP32 copy
+};
+
+namespace bpf {
+
+// A triangular operator in level-scheduled form.  Rows are stored in level
+// order; row t of the stored CSR is original row `order[t]`.  Off-diagonal
+// entries only; the diagonal (if not unit) is stored separately.  `gather`
+// maps stored entry -> position in the source value array (combined factor),
+// `dgather` maps stored row -> position of its diagonal.
+struct TriOp {
+  int n = 0;
+  bool unit = true;            // implicit unit diagonal
+  bool lower = true;           // dependencies on smaller (true) / larger indices
+  int nlevels = 0;
+  std::vector<int> h_lvl;      // level start offsets into order (nlevels+1)
+  std::vector<int> h_order;
+  std::vector<int> h_ptr, h_col, h_gather, h_dgather;
+  DevBuf order, ptr, col, gather, dgather, lvl;
+  DevBuf val, val32, diag, diag32;   // refreshed from the factor values
+  int max_row_nnz = 0;
+};
+
+// Launch plan: consecutive levels run either one launch per level ("wide")
+// or many levels in one single-workgroup launch ("narrow").  A level is wide
+// when it has more than `tail_rows` rows.
+struct Seg { int l0, l1; bool wide; };
+std::vector<Seg> make_plan(const TriOp &op, int tail_rows);
+
+// Builds a TriOp from a CSR pattern.  kind: 0 = strict lower part (unit diag),
+// 1 = upper part (diag separate), 2 = lower part with diag separate,
+// 3 = transpose of the lower part (upper operator, diag separate).
+void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
+                 const std::vector<int> &colidx, int kind, hipStream_t s);
+// Copies values (FP64 and FP32) from a combined factor array on the device.
+void triop_refresh(TriOp &op, const double *d_fac, hipStream_t s);
+
+}  // namespace bpf
+
+struct bpf_prec_s {
+  bpf_ctx ctx = nullptr;
+  int kind = BPF_PREC_ILU0;
+  int n = 0, nnz = 0;
+  bpf::DevBuf rowptr, colidx, diagpos, fac;   // combined factor on A's pattern
+  std::vector<int> h_rowptr, h_colidx, h_diagpos;
+  bpf::TriOp lo, up;                          // solve operators
+  bpf::DevBuf tmp;                            // n x p scratch for apply
+  int max_row_nnz = 0;
+};
+
+struct bpf_rf_s;
+struct bpf_batch_s;
+
+namespace bpf {
+
+// ---------------------------------------------------------------- launchers
+// Row-major blocks: element (i, c) at X[i * ld + c].
+template <typename TV>
+void launch_spmm(hipStream_t s, int n, const int *rowptr, const int *colidx,
+                 const void *val, bool val32, int p, const TV *X, int ldx, TV *Y, int ldy,
+                 double alpha, double beta);
+void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *colidx,
+                          const void *val, bool val32, int p, const void *X, int ldx,
+                          void *Y, int ldy, double alpha, double beta, bool f32);
+
+// y = op^{-1} x for p columns (row-major).  x may alias y.
+template <typename TV>
+void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy);
+
+// Numeric IKJ factorization on a combined pattern (ILU0 when the pattern is
+// A's, exact LU when it is the full fill pattern).  The values must already
+// hold the (permuted) matrix entries.  Uses the lower TriOp's levels.
+void launch_ilu_factor(hipStream_t s, const TriOp &lower, int n, const int *rowptr,
+                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
+                       int *d_status);
+void launch_ic_factor(hipStream_t s, const TriOp &lower, int n, const int *rowptr,
+                      const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
+                      int *d_status);
+
+// Layout conversions (column-major <-> row-major) with type conversion.
+void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst,
+                     int ldd, bool dst32);
+void launch_rm_to_cm(hipStream_t s, int n, int p, const void *src, int lds, bool src32,
+                     double *dst, int ldd);
+
+}  // namespace bpf

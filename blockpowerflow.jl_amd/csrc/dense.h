@@ -1,0 +1,136 @@
+// Dense block kernels of the block-Krylov solvers: argument structures and
+// launchers (see dense_kernels.hip for the kernels).
+#pragma once
+
+#include "bpf_internal.h"
+
+namespace bpf {
+
+// One independent Krylov group: columns [c0, c0 + w) of the n x p blocks.
+struct GroupDesc {
+  int c0;
+  int w;
+};
+
+// Psi_{i,g} = V_{i0+i}[:, g]^T W[:, g] for i < nb.
+struct ProjArgs {
+  const void *const *Vp;  // device table of block pointers (block j = Vp[j]),
+  const void *V1;         // or a single block when Vp is null (nb = 1)
+  int i0, nb, ldv;
+  const void *W;
+  int ldw;
+  int n;
+  const GroupDesc *groups;
+  int ng;
+  int chunk_rows;      // set by the launcher
+  double *part;        // set by the launcher
+};
+
+// Y[:, g] = beta*Y[:, g] + alpha * sum_{i<nb} V_{i0+i}[:, g] * Psi_{i,g}
+struct GemmArgs {
+  const void *const *Vp;
+  const void *V1;
+  int i0, nb, ldv;
+  const double *psi;   // Psi_{i,g} at psi + i*psi_istride + g*psi_gstride (S x S, ld ldpsi)
+  size_t psi_istride, psi_gstride;
+  int ldpsi;
+  void *Y;
+  int ldy;
+  int n;
+  double alpha, beta;
+  const GroupDesc *groups;
+};
+
+struct CholArgs {
+  const double *G;     // per group at G + g*gstride, S x S
+  double *R, *Rinv;
+  size_t gstride;
+  int S;
+  int n;               // rows of the block (for the shift)
+  const GroupDesc *groups;
+  int *flags;          // bit 0: a shift was needed
+};
+
+// Hessenberg arena: block column k is a separate region cols[k] holding, per
+// group g, k+2 blocks of S x S (Psi_0..Psi_k, C; after the QR update the
+// rotated R blocks and the compact reflector of column k), then the tau
+// vectors (ng x S) and the final right-hand-side blocks Z_k (ng x S x S).
+//   block (k, g, i) = cols[k] + (g*(k+2) + i)*S*S
+//   tau   (k, g)    = cols[k] + ng*(k+2)*S*S + g*S
+//   zfin  (k, g)    = cols[k] + ng*(k+2)*S*S + ng*S + g*S*S
+__host__ __device__ inline size_t hcol_elems(int k, int ng, int S) {
+  return (size_t)ng * (k + 2) * S * S + (size_t)ng * S + (size_t)ng * S * S;
+}
+
+struct HessArgs {
+  double *const *cols;
+  int k;               // current column
+  int S;
+  int ng;
+  const GroupDesc *groups;
+};
+
+// QR update of column k; E (per group 2S x S at E + g*2*S*S) holds [Z_k; 0]
+// on entry.  On exit zfin(k) = rotated Z_k, E = [Lambda; 0] (the next Z),
+// rnorm[c0 + c] = ||Lambda[:, c]||_2.
+struct HessQrArgs {
+  double *const *cols;
+  int k;
+  int S;
+  int ng;
+  double *E;
+  const GroupDesc *groups;
+  double *rnorm;
+};
+
+struct BacksubArgs {
+  double *const *cols;
+  int K;               // number of block columns
+  int S;
+  int ng;
+  double *Y;           // Y_i of group g at Y + (i*ng + g)*S*S
+  const GroupDesc *groups;
+};
+
+int pad_width(int w);
+void fill_int(hipStream_t s, int *p, int v, int n);
+void fill_double(hipStream_t s, double *p, double v, size_t n);
+void fill_float(hipStream_t s, float *p, float v, size_t n);
+
+// out block for (i, g) at out + i*o_istride + g*o_gstride (S x S).
+template <typename TVV, typename TW>
+void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride, size_t o_gstride,
+                 bool accumulate, DevBuf &part);
+template <typename TVV, typename TY>
+void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng);
+void launch_chol(hipStream_t s, CholArgs a, int ng);
+// C_g = A_g B_g for upper-triangular S x S blocks (A_g, B_g at g*stride,
+// C_g at g*cstride).
+void launch_trmm_upper(hipStream_t s, const double *A, const double *B, double *C, int S,
+                       size_t stride, size_t cstride, const GroupDesc *groups, int ng);
+void launch_hess_apply(hipStream_t s, HessArgs a, int ng);
+void launch_hess_qr(hipStream_t s, HessQrArgs a, int ng);
+void launch_backsub(hipStream_t s, BacksubArgs a, int ng);
+
+// Y_i[:, b] *= scale[c0 + b] for the K x ng blocks of a back-substitution result.
+void launch_scale_ycols(hipStream_t s, double *Y, int K, int ng, int S, const GroupDesc *groups,
+                        const double *scale);
+// X[:, c] *= scale[c] (row-major n x p), or /= when invert.
+void launch_scale_cols(hipStream_t s, int n, int p, double *X, int ld, const double *scale, bool invert);
+// Small dense LU with partial pivoting (one workgroup per group): A (w x w,
+// ld S) overwritten by L\U, piv (S ints); solve applies it to B (w x w).
+void launch_lu_small(hipStream_t s, double *A, int *piv, int S, const GroupDesc *groups, int ng);
+void launch_lu_small_solve(hipStream_t s, const double *A, const int *piv, double *B, int S,
+                           const GroupDesc *groups, int ng);
+// out = sum over the n x p row-major block of x .* y (FP64), device scalar.
+template <typename T>
+void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out);
+
+void colmax_colmajor(hipStream_t s, int n, int p, const double *X, int ld, double *out);
+template <typename T>
+void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out);
+void axpby(hipStream_t s, size_t m, double alpha, const double *x, double beta, double *y);
+template <typename TS, typename TD>
+void copy_cast(hipStream_t s, int n, int p, const TS *src, int lds, TD *dst, int ldd, double scale);
+
+}  // namespace bpf

@@ -1,0 +1,271 @@
+// Block-Arnoldi engine: grouped block GMRES iterations on the device.
+//
+// Follows the structure of block_gmres! (src/block_gmres.jl:251-393):
+//   P = N^{-1} V_k                      (:297-298)
+//   W = A P                             (:299)
+//   block Gram-Schmidt against V_0..V_k (:301-313; classical block form,
+//                                        optional second pass = the
+//                                        reference's `reorthogonalization`)
+//   V_{k+1} C = W  (thin QR)            (:316; CholQR2 in place of geqrf/orgqr)
+//   apply the stored reflectors and form the new one on [Psi_kk; C],
+//   rotate the right-hand side          (:318-337)
+//   residual estimate ||Lambda||        (:341-342, per column here)
+//   block back substitution, X += Z Y   (:366-385)
+#include "krylov.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace bpf {
+
+void *BlockArena::block(int i) {
+  while (i >= (int)ptrs.size()) {
+    DevBuf b;
+    b.ensure(block_elems * esize * chunk);
+    for (int q = 0; q < chunk; ++q) ptrs.push_back(static_cast<char *>(b.ptr) + q * block_elems * esize);
+    chunks.push_back(std::move(b));
+  }
+  return ptrs[i];
+}
+
+PtrTable::~PtrTable() {
+  if (h) (void)hipHostFree(h);
+}
+
+void PtrTable::set(int i, void *ptr, hipStream_t s) {
+  if (i >= cap) {
+    int nc = std::max(64, cap * 2);
+    while (nc <= i) nc *= 2;
+    void **nh = nullptr;
+    BPF_CHECK_HIP(hipStreamSynchronize(s));
+    BPF_CHECK_HIP(hipHostMalloc((void **)&nh, nc * sizeof(void *), hipHostMallocDefault));
+    std::memset(nh, 0, nc * sizeof(void *));
+    if (h) {
+      std::memcpy(nh, h, cap * sizeof(void *));
+      (void)hipHostFree(h);
+    }
+    h = nh;
+    DevBuf nd;
+    nd.ensure(nc * sizeof(void *));
+    if (cap > 0)
+      BPF_CHECK_HIP(hipMemcpyAsync(nd.ptr, h, cap * sizeof(void *), hipMemcpyHostToDevice, s));
+    d = std::move(nd);
+    cap = nc;
+  }
+  if (h[i] == ptr) return;
+  h[i] = ptr;
+  BPF_CHECK_HIP(hipMemcpyAsync(d.as<void *>() + i, h + i, sizeof(void *), hipMemcpyHostToDevice, s));
+}
+
+BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConfig &cfg)
+    : ctx_(ctx), A_(A), N_(N), cfg_(cfg), st_(ctx->stream) {
+  BPF_REQUIRE(cfg.p > 0 && cfg.n > 0, "empty block");
+  int s = cfg.s > 0 ? std::min(cfg.s, cfg.p) : cfg.p;
+  if (s > 128) {
+    // Split into near-equal groups of at most 128 columns.
+    const int ng = (cfg.p + 127) / 128;
+    s = (cfg.p + ng - 1) / ng;
+  }
+  S_ = pad_width(s);
+  BPF_REQUIRE(S_ > 0, "block width %d unsupported", s);
+  for (int c0 = 0; c0 < cfg.p; c0 += s) h_groups_.push_back({c0, std::min(s, cfg.p - c0)});
+  ng_ = (int)h_groups_.size();
+  upload(d_groups_, h_groups_, st_);
+  const size_t es = cfg.f32 ? sizeof(float) : sizeof(double);
+  const size_t blk = (size_t)cfg.n * cfg.p;
+  V_.init(blk, es, 4);
+  if (cfg.flexible && N) Z_.init(blk, es, 4);
+  W_.ensure(blk * es);
+  P_.ensure(blk * es);
+  const size_t ss = (size_t)ng_ * S_ * S_ * sizeof(double);
+  G_.ensure(ss); R1_.ensure(ss); R1i_.ensure(ss); R2_.ensure(ss); R2i_.ensure(ss);
+  E_.ensure(2 * ss);
+  rnorm_.ensure(std::max(1, cfg.p) * sizeof(double));
+  flags_.ensure(16);
+  ctx->ensure_small(std::max(cfg.p, 16));
+}
+
+BlockArnoldi::~BlockArnoldi() = default;
+
+double *BlockArnoldi::col_region(int k) {
+  while ((int)cols_.size() <= k) {
+    const int kk = (int)cols_.size();
+    DevBuf b;
+    b.ensure(hcol_elems(kk, ng_, S_) * sizeof(double));
+    coltab_.set(kk, b.ptr, st_);
+    cols_.push_back(std::move(b));
+  }
+  return cols_[k].as<double>();
+}
+
+template <typename TV>
+void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
+  // Two CholQR passes; C = R2 R1 (upper), Vout = W R1^{-1} R2^{-1}.
+  const GroupDesc *dg = d_groups_.as<GroupDesc>();
+  const int n = cfg_.n, p = cfg_.p;
+  const size_t SS = (size_t)S_ * S_;
+  void *src = W;
+  DevBuf *Rs[2] = {&R1_, &R2_};
+  DevBuf *Ris[2] = {&R1i_, &R2i_};
+  for (int pass = 0; pass < 2; ++pass) {
+    ProjArgs pa{nullptr, src, 0, 1, p, src, p, n, dg, ng_, 0, nullptr};
+    launch_proj<TV, TV>(st_, pa, S_, G_.as<double>(), 0, SS, false, part_);
+    CholArgs ca{G_.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S_, n, dg,
+                flags_.as<int>()};
+    launch_chol(st_, ca, ng_);
+    void *dst = pass == 1 ? Vout : W;
+    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S_, dst, p, n, 1.0, 0.0, dg};
+    launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    src = dst;
+  }
+  launch_trmm_upper(st_, R2_.as<double>(), R1_.as<double>(), Cout, S_, SS, cstride, dg, ng_);
+}
+
+template <typename TV>
+void BlockArnoldi::form(double *X, double alpha) {
+  // Y = R^{-1} Z (block back substitution), then X += alpha * sum_i Z_i Y_i
+  // (flexible) or X += alpha * N^{-1} sum_i V_i Y_i.
+  const int K = k_;
+  if (K <= 0) return;
+  const GroupDesc *dg = d_groups_.as<GroupDesc>();
+  const int n = cfg_.n, p = cfg_.p;
+  const size_t SS = (size_t)S_ * S_;
+  Y_.ensure((size_t)K * ng_ * SS * sizeof(double));
+  BacksubArgs ba{coltab_.dev<double>(), K, S_, ng_, Y_.as<double>(), dg};
+  launch_backsub(st_, ba, ng_);
+  if (scale_) launch_scale_ycols(st_, Y_.as<double>(), K, ng_, S_, dg, scale_);
+  const bool flex = cfg_.flexible && N_;
+  const void *const *tab = flex ? ztab_.dev<const void>() : vtab_.dev<const void>();
+  if (flex || !N_) {
+    GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
+                S_, X, p, n, alpha, 1.0, dg};
+    launch_block_gemm<TV, double>(st_, ga, S_, ng_);
+  } else {
+    // P = sum_i V_i Y_i (TV), then X += alpha * N^{-1} P.
+    GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
+                S_, P_.ptr, p, n, 1.0, 0.0, dg};
+    launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    TV *P = P_.as<TV>();
+    trisolve<TV>(st_, N_->lo, p, P, p, P, p);
+    trisolve<TV>(st_, N_->up, p, P, p, P, p);
+    Xtmp_.ensure((size_t)n * p * sizeof(double));
+    copy_cast<TV, double>(st_, n, p, P, p, Xtmp_.as<double>(), p, 1.0);
+    axpby(st_, (size_t)n * p, alpha, Xtmp_.as<double>(), 1.0, X);
+  }
+}
+
+void BlockArnoldi::peek(const double *Xbase, double *Xout) {
+  BPF_CHECK_HIP(hipMemcpyAsync(Xout, Xbase, (size_t)cfg_.n * cfg_.p * sizeof(double),
+                               hipMemcpyDeviceToDevice, st_));
+  if (cfg_.f32) form<float>(Xout, 1.0);
+  else form<double>(Xout, 1.0);
+}
+
+int BlockArnoldi::run(const double *R0, double *X, int maxit, const StopFn &stop, double *rnorm_out) {
+  return cfg_.f32 ? run_t<float>(R0, X, maxit, stop, rnorm_out)
+                  : run_t<double>(R0, X, maxit, stop, rnorm_out);
+}
+
+template <typename TV>
+int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &stop,
+                        double *rnorm_out) {
+  const GroupDesc *dg = d_groups_.as<GroupDesc>();
+  const int n = cfg_.n, p = cfg_.p;
+  const size_t SS = (size_t)S_ * S_;
+  const bool flex = cfg_.flexible && N_;
+  const bool f32 = std::is_same<TV, float>::value;
+  k_ = 0;
+  shifted_ = false;
+  fill_int(st_, flags_.as<int>(), 0, 1);
+  // V_0 Gamma = R0 (CholQR2); E = [Gamma; 0].
+  copy_cast<double, TV>(st_, n, p, R0, p, W_.as<TV>(), p, 1.0);
+  fill_double(st_, E_.as<double>(), 0.0, 2 * (size_t)ng_ * SS);
+  vtab_.set(0, V_.block(0), st_);
+  // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
+  cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
+  double *h_rn = ctx_->h_small;
+  int it = 0;
+  while (it < maxit) {
+    const int k = it;
+    // P = N^{-1} V_k
+    TV *Vk = static_cast<TV *>(V_.block(k));
+    TV *Pk = Vk;
+    if (N_) {
+      Pk = flex ? static_cast<TV *>(Z_.block(k)) : P_.as<TV>();
+      if (flex) ztab_.set(k, Pk, st_);
+      trisolve<TV>(st_, N_->lo, p, Vk, p, Pk, p);
+      trisolve<TV>(st_, N_->up, p, Pk, p, Pk, p);
+    }
+    // W = A P
+    launch_spmm<TV>(st_, n, A_->rowptr.as<int>(), A_->colidx.as<int>(),
+                    f32 ? A_->val32.ptr : A_->val.ptr, f32, p, Pk, p, W_.as<TV>(), p, 1.0, 0.0);
+    // Block Gram-Schmidt against V_0..V_k.
+    double *col = col_region(k);
+    const size_t gstr = (size_t)(k + 2) * SS;
+    const void *const *vt = vtab_.dev<const void>();
+    ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
+    launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
+    GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
+    launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    if (cfg_.reorth) {
+      tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
+      double *t = tmpcol_.as<double>();
+      launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
+      GemmArgs gb = ga;
+      gb.psi = t;
+      launch_block_gemm<TV, TV>(st_, gb, S_, ng_);
+      // col += t over blocks 0..k of each group (block k+1 of t is unused).
+      for (int g = 0; g < ng_; ++g)
+        axpby(st_, (size_t)(k + 1) * SS, 1.0, t + g * gstr, 1.0, col + g * gstr);
+    }
+    // V_{k+1} C = W; C into block k+1 of each group's column.
+    cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr);
+    vtab_.set(k + 1, V_.block(k + 1), st_);
+    // QR update of the block Hessenberg matrix.
+    HessArgs ha{coltab_.dev<double>(), k, S_, ng_, dg};
+    launch_hess_apply(st_, ha, ng_);
+    HessQrArgs qa{coltab_.dev<double>(), k, S_, ng_, E_.as<double>(), dg, rnorm_.as<double>()};
+    launch_hess_qr(st_, qa, ng_);
+    BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)p * sizeof(double), hipMemcpyDeviceToHost, st_));
+    BPF_CHECK_HIP(hipStreamSynchronize(st_));
+    ++it;
+    k_ = it;
+    if (stop(h_rn, it)) break;
+  }
+  if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)p * sizeof(double));
+  int fl = 0;
+  BPF_CHECK_HIP(hipMemcpy(&fl, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  shifted_ = fl != 0;
+  form<TV>(X, 1.0);
+  return it;
+}
+
+template int BlockArnoldi::run_t<double>(const double *, double *, int, const StopFn &, double *);
+template int BlockArnoldi::run_t<float>(const double *, double *, int, const StopFn &, double *);
+
+// ------------------------------------------------------------- utilities
+void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double *X, double *R) {
+  BPF_CHECK_HIP(hipMemcpyAsync(R, B, (size_t)A->n * p * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_spmm<double>(s, A->n, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, X, p,
+                      R, p, -1.0, 1.0);
+}
+
+void backward_error_rm(bpf_ctx ctx, bpf_csr A, int p, const double *R, const double *X,
+                       const double *B, double *h_berr, DevBuf &stat) {
+  stat.ensure(3 * (size_t)p * sizeof(double));
+  double *d = stat.as<double>();
+  colstat_rowmajor<double>(ctx->stream, A->n, p, R, p, 0, d);
+  colstat_rowmajor<double>(ctx->stream, A->n, p, X, p, 0, d + p);
+  colstat_rowmajor<double>(ctx->stream, A->n, p, B, p, 0, d + 2 * p);
+  std::vector<double> h(3 * (size_t)p);
+  BPF_CHECK_HIP(hipMemcpyAsync(h.data(), d, 3 * (size_t)p * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  BPF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < p; ++c) {
+    const double den = A->norm_inf * h[p + c] + h[2 * p + c];
+    h_berr[c] = den > 0.0 ? h[c] / den : (h[c] > 0.0 ? INFINITY : 0.0);
+  }
+}
+
+}  // namespace bpf

@@ -1,0 +1,116 @@
+// Block-Krylov building blocks shared by the single-system and batched
+// solvers: a growable block arena, the block-Arnoldi engine (grouped block
+// GMRES with CholQR2 block orthonormalization and an incremental Householder
+// QR of the block Hessenberg matrix), and small device utilities.
+#pragma once
+
+#include <functional>
+#include <memory>
+
+#include "dense.h"
+
+namespace bpf {
+
+// n x p blocks of element type T, allocated in chunks; a device table holds
+// the block pointers for kernels that walk several blocks.
+struct BlockArena {
+  size_t block_elems = 0;
+  size_t esize = 0;
+  int chunk = 8;
+  std::vector<DevBuf> chunks;
+  std::vector<void *> ptrs;
+  void init(size_t elems, size_t es, int chunk_blocks = 8) {
+    block_elems = elems;
+    esize = es;
+    chunk = chunk_blocks;
+  }
+  void *block(int i);   // grows as needed
+  int capacity() const { return (int)ptrs.size(); }
+};
+
+// Device table of pointers with a pinned host mirror; entries are appended
+// with small asynchronous copies (an entry never changes once written).
+struct PtrTable {
+  void **h = nullptr;
+  DevBuf d;
+  int cap = 0;
+  PtrTable() = default;
+  PtrTable(const PtrTable &) = delete;
+  PtrTable &operator=(const PtrTable &) = delete;
+  ~PtrTable();
+  void set(int i, void *ptr, hipStream_t s);
+  template <typename T> T *const *dev() const { return d.as<T *>(); }
+};
+
+// Host-visible description of one Krylov run.
+struct ArnoldiConfig {
+  int n = 0, p = 0;            // rows, columns of the RHS block
+  int s = 0;                   // group width
+  bool f32 = false;            // basis / operator precision
+  bool flexible = false;       // store Z_k = N^{-1} V_k
+  bool reorth = false;         // second block Gram-Schmidt pass
+};
+
+// Stopping decision callback: given per-column residual 2-norm estimates of
+// the (scaled) problem and the iteration count, return true to stop.
+using StopFn = std::function<bool(const double *rnorm, int iter)>;
+
+class BlockArnoldi {
+ public:
+  BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConfig &cfg);
+  ~BlockArnoldi();
+  // Runs up to maxit block iterations for A D = R0 (R0: FP64 row-major n x p,
+  // columns pre-scaled by the caller), then adds alpha * D to X (FP64
+  // row-major).  Returns the number of iterations; rnorm_out (host, p) gets
+  // the final residual estimates.
+  int run(const double *R0, double *X, int maxit, const StopFn &stop, double *rnorm_out);
+  // Forms X + D for the current state without ending the run (used to
+  // estimate ||x|| mid-run).
+  void peek(const double *Xbase, double *Xout);
+  // Column scaling of the right-hand side handed to run(): the correction
+  // added to X is multiplied column-wise by d_scale (device, p entries).
+  void set_scale(const double *d_scale) { scale_ = d_scale; }
+  int iterations() const { return k_; }
+  bool shifted() const { return shifted_; }
+
+ private:
+  template <typename TV> int run_t(const double *R0, double *X, int maxit, const StopFn &stop,
+                                   double *rnorm_out);
+  template <typename TV> void form(double *X, double alpha);
+  template <typename TV> void cholqr2(void *W, void *Vout, double *Cout, size_t cstride);
+  double *col_region(int k);
+
+  const double *scale_ = nullptr;
+  bpf_ctx ctx_;
+  bpf_csr A_;
+  bpf_prec N_;
+  ArnoldiConfig cfg_;
+  int S_ = 0, ng_ = 0;
+  int k_ = 0;                  // completed block columns
+  bool shifted_ = false;
+  hipStream_t st_;
+  std::vector<GroupDesc> h_groups_;
+  DevBuf d_groups_;
+  BlockArena V_, Z_;           // basis and (flexible) preconditioned basis
+  DevBuf W_, P_;               // work blocks (TV), P: preconditioned block
+  // Hessenberg arena: one region per block column k (pointer table).
+  std::vector<DevBuf> cols_;
+  PtrTable coltab_, vtab_, ztab_;
+  DevBuf tmpcol_;              // second-pass coefficients
+  DevBuf G_, R1_, R1i_, R2_, R2i_;
+  DevBuf E_;                   // current [Z_k; Lambda] per group
+  DevBuf rnorm_;
+  DevBuf Y_;                   // back substitution output
+  DevBuf part_;                // projection partial sums
+  DevBuf flags_;
+  DevBuf Xtmp_;                // FP64 n x p scratch (non-flexible solution assembly)
+};
+
+// Row-major FP64 residual R = B - A X.
+void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double *X, double *R);
+// Per-column backward error from row-major FP64 R, X, B; host output.
+void backward_error_rm(bpf_ctx ctx, bpf_csr A, int p, const double *R, const double *X,
+                       const double *B, double *h_berr, DevBuf &stat);
+void scale_columns(hipStream_t s, int n, int p, double *X, int ld, const double *d_scale);
+
+}  // namespace bpf

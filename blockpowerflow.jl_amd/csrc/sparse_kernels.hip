@@ -1,0 +1,524 @@
+// Sparse kernels for gfx950: CSR SpMM over n x p row-major blocks, level
+// scheduled multi-RHS triangular solves, and IKJ ILU / IC factorization.
+//
+// Layout.  Inside the library every n x p block is row-major (element (i, c)
+// at X[i*ld + c]): a nonzero a_ij then multiplies the contiguous row X[j, :],
+// so a row gather is one coalesced load of p elements instead of p strided
+// loads.  A group of G lanes (G = power of two <= 64) owns one matrix row and
+// spreads the p columns over its lanes (CPL columns per lane).
+#include "bpf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace bpf {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int next_pow2(int v) {
+  int g = 1;
+  while (g < v) g <<= 1;
+  return g;
+}
+
+// Lanes per row and columns per lane for a p-wide block.
+inline void lane_shape(int p, int &glog, int &cpl) {
+  if (p <= 64) {
+    int g = next_pow2(std::max(p, 1));
+    glog = 0;
+    while ((1 << glog) < g) ++glog;
+    cpl = 1;
+  } else {
+    glog = 6;
+    cpl = p <= 128 ? 2 : 4;
+  }
+}
+
+template <typename T> __device__ __forceinline__ T ld_val(const void *v, int k);
+template <> __device__ __forceinline__ double ld_val<double>(const void *v, int k) {
+  return static_cast<const double *>(v)[k];
+}
+template <> __device__ __forceinline__ float ld_val<float>(const void *v, int k) {
+  return static_cast<const float *>(v)[k];
+}
+
+// ------------------------------------------------------------------ SpMM
+template <typename TA, typename TV, int CPL>
+__global__ __launch_bounds__(kBlock) void spmm_rm_kernel(
+    int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+    const TA *__restrict__ val, int p, const TV *__restrict__ X, int ldx,
+    TV *__restrict__ Y, int ldy, TV alpha, TV beta, int glog) {
+  const int G = 1 << glog;
+  const int rpb = kBlock >> glog;
+  const int row = blockIdx.x * rpb + (threadIdx.x >> glog);
+  const int lane = threadIdx.x & (G - 1);
+  if (row >= n) return;
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  for (int cbase = 0; cbase < p; cbase += G * CPL) {
+    TV acc[CPL];
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
+    for (int k = k0; k < k1; ++k) {
+      const int j = colidx[k];
+      const TV a = static_cast<TV>(val[k]);
+      const TV *xr = X + (size_t)j * ldx;
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = cbase + lane + m * G;
+        if (c < p) acc[m] = fma(a, xr[c], acc[m]);
+      }
+    }
+    TV *yr = Y + (size_t)row * ldy;
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+      const int c = cbase + lane + m * G;
+      if (c < p) yr[c] = beta == TV(0) ? alpha * acc[m] : fma(beta, yr[c], alpha * acc[m]);
+    }
+  }
+}
+
+// Column-major SpMM used at the library boundary: one thread per (row, col).
+template <typename TA, typename TV>
+__global__ __launch_bounds__(kBlock) void spmm_cm_kernel(
+    int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+    const TA *__restrict__ val, int p, const TV *__restrict__ X, int ldx,
+    TV *__restrict__ Y, int ldy, TV alpha, TV beta) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || c >= p) return;
+  const TV *x = X + (size_t)c * ldx;
+  TV acc = TV(0);
+  for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    acc = fma(static_cast<TV>(val[k]), x[colidx[k]], acc);
+  TV *y = Y + (size_t)c * ldy + i;
+  *y = beta == TV(0) ? alpha * acc : fma(beta, *y, alpha * acc);
+}
+
+// ------------------------------------------------------ triangular solve
+struct TriDev {
+  const int *order, *ptr, *col, *lvl;
+  const void *val, *diag;
+};
+
+// Solves one stored row t of a level-scheduled triangular operator for all
+// p columns owned by this lane group.
+template <typename TF, typename TV, bool UNIT, int CPL>
+__device__ __forceinline__ void tri_row(const TriDev &a, int t, int lane, int G,
+                                        const TV *x, int ldx, TV *y, int ldy, int p) {
+  const int i = a.order[t];
+  const int k0 = a.ptr[t], k1 = a.ptr[t + 1];
+  const TF *val = static_cast<const TF *>(a.val);
+  TV dinv = TV(1);
+  if (!UNIT) dinv = TV(1) / static_cast<TV>(static_cast<const TF *>(a.diag)[t]);
+  for (int cbase = 0; cbase < p; cbase += G * CPL) {
+    TV acc[CPL];
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+      const int c = cbase + lane + m * G;
+      acc[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
+    }
+    for (int k = k0; k < k1; ++k) {
+      const int j = a.col[k];
+      const TV v = static_cast<TV>(val[k]);
+      const TV *yr = y + (size_t)j * ldy;
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = cbase + lane + m * G;
+        if (c < p) acc[m] = fma(-v, yr[c], acc[m]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+      const int c = cbase + lane + m * G;
+      if (c < p) y[(size_t)i * ldy + c] = UNIT ? acc[m] : acc[m] * dinv;
+    }
+  }
+}
+
+// One launch per wide level: rows [t0, t1) of the level-ordered list.
+template <typename TF, typename TV, bool UNIT, int CPL>
+__global__ __launch_bounds__(kBlock) void tri_level_kernel(TriDev a, int t0, int t1,
+                                                           const TV *x, int ldx, TV *y,
+                                                           int ldy, int p, int glog) {
+  const int G = 1 << glog;
+  const int rpb = kBlock >> glog;
+  const int t = t0 + blockIdx.x * rpb + (threadIdx.x >> glog);
+  if (t >= t1) return;
+  tri_row<TF, TV, UNIT, CPL>(a, t, threadIdx.x & (G - 1), G, x, ldx, y, ldy, p);
+}
+
+// One single-workgroup launch walks many narrow levels [l0, l1), with a
+// workgroup barrier between levels.
+constexpr int kTailBlock = 1024;
+template <typename TF, typename TV, bool UNIT, int CPL>
+__global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, int l1,
+                                                              const TV *x, int ldx, TV *y,
+                                                              int ldy, int p, int glog) {
+  const int G = 1 << glog;
+  const int ngroups = kTailBlock >> glog;
+  const int gid = threadIdx.x >> glog;
+  const int lane = threadIdx.x & (G - 1);
+  for (int l = l0; l < l1; ++l) {
+    const int t1 = a.lvl[l + 1];
+    for (int t = a.lvl[l] + gid; t < t1; t += ngroups)
+      tri_row<TF, TV, UNIT, CPL>(a, t, lane, G, x, ldx, y, ldy, p);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------- factorization
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lds_find(const int *s_col, int lo, int hi, int j) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = s_col[mid];
+    if (c == j) return mid;
+    if (c < j) lo = mid + 1; else hi = mid;
+  }
+  return -1;
+}
+
+struct FacDev {
+  const int *rowptr, *colidx, *diagpos, *order, *lvl;
+  double *fac;
+  int *status;
+  int cap;  // LDS entries per wave
+};
+
+// IKJ elimination of row i (one wave).  Row i's pattern and values are staged
+// in this wave's LDS slice; rows k < i referenced by it are final.
+__device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_val) {
+  const int base = f.rowptr[i];
+  const int len = f.rowptr[i + 1] - base;
+  const int dloc = f.diagpos[i] - base;
+  for (int q = lane; q < len; q += 64) {
+    s_col[q] = f.colidx[base + q];
+    s_val[q] = f.fac[base + q];
+  }
+  wave_sync();
+  for (int kk = 0; kk < dloc; ++kk) {
+    const int k = s_col[kk];
+    const double ukk = f.fac[f.diagpos[k]];
+    const double l = s_val[kk] / ukk;
+    const int j0 = f.diagpos[k] + 1, j1 = f.rowptr[k + 1];
+    for (int jj = j0 + lane; jj < j1; jj += 64) {
+      const int pos = lds_find(s_col, kk + 1, len, f.colidx[jj]);
+      if (pos >= 0) s_val[pos] -= l * f.fac[jj];
+    }
+    wave_sync();
+    if (lane == 0) {
+      s_val[kk] = l;
+      if (ukk == 0.0) atomicMin(f.status, k);
+    }
+    wave_sync();
+  }
+  for (int q = lane; q < len; q += 64) f.fac[base + q] = s_val[q];
+  if (lane == 0 && s_val[dloc] == 0.0) atomicMin(f.status, i);
+  wave_sync();
+}
+
+// IC(0), row-oriented: l_ik = (a_ik - sum_{j<k} l_ij l_kj) / l_kk on the
+// lower pattern, l_ii = sqrt(a_ii - sum_j l_ij^2).
+__device__ void ic_row(const FacDev &f, int i, int lane, int *s_col, double *s_val) {
+  const int base = f.rowptr[i];
+  const int dloc = f.diagpos[i] - base;
+  for (int q = lane; q <= dloc; q += 64) {
+    s_col[q] = f.colidx[base + q];
+    s_val[q] = f.fac[base + q];
+  }
+  wave_sync();
+  for (int kk = 0; kk < dloc; ++kk) {
+    const int k = s_col[kk];
+    const int kb = f.rowptr[k], kd = f.diagpos[k];
+    double part = 0.0;
+    for (int jj = kb + lane; jj < kd; jj += 64) {
+      const int pos = lds_find(s_col, 0, kk, f.colidx[jj]);
+      if (pos >= 0) part += s_val[pos] * f.fac[jj];
+    }
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    const double lkk = f.fac[kd];
+    wave_sync();
+    if (lane == 0) s_val[kk] = (s_val[kk] - part) / lkk;
+    wave_sync();
+  }
+  double sq = 0.0;
+  for (int q = lane; q < dloc; q += 64) sq += s_val[q] * s_val[q];
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+  wave_sync();
+  if (lane == 0) {
+    const double d = s_val[dloc] - sq;
+    if (!(d > 0.0)) { atomicMin(f.status, i); s_val[dloc] = 1.0; }
+    else s_val[dloc] = sqrt(d);
+  }
+  wave_sync();
+  for (int q = lane; q <= dloc; q += 64) f.fac[base + q] = s_val[q];
+  wave_sync();
+}
+
+template <bool IC>
+__global__ void fac_level_kernel(FacDev f, int t0, int t1) {
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = t0 + blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= t1) return;
+  int *s_col = reinterpret_cast<int *>(smem) + (size_t)wave * f.cap;
+  double *s_val = reinterpret_cast<double *>(smem + (size_t)(blockDim.x >> 6) * f.cap * sizeof(int)) +
+                  (size_t)wave * f.cap;
+  if (IC) ic_row(f, f.order[t], lane, s_col, s_val);
+  else ilu_row(f, f.order[t], lane, s_col, s_val);
+}
+
+template <bool IC>
+__global__ void fac_tail_kernel(FacDev f, int l0, int l1) {
+  extern __shared__ unsigned char smem[];
+  const int nw = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int *s_col = reinterpret_cast<int *>(smem) + (size_t)wave * f.cap;
+  double *s_val = reinterpret_cast<double *>(smem + (size_t)nw * f.cap * sizeof(int)) +
+                  (size_t)wave * f.cap;
+  for (int l = l0; l < l1; ++l) {
+    for (int t = f.lvl[l] + wave; t < f.lvl[l + 1]; t += nw) {
+      if (IC) ic_row(f, f.order[t], lane, s_col, s_val);
+      else ilu_row(f, f.order[t], lane, s_col, s_val);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ conversions
+template <typename TD>
+__global__ __launch_bounds__(kBlock) void cm_to_rm_kernel(int n, int p, const double *src,
+                                                         int lds, TD *dst, int ldd) {
+  // 32 x 32 tile transpose through LDS.
+  __shared__ double tile[32][33];
+  const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, c = c0 + r;
+    tile[r][tx] = (i < n && c < p) ? src[(size_t)c * lds + i] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, c = c0 + tx;
+    if (i < n && c < p) dst[(size_t)i * ldd + c] = static_cast<TD>(tile[tx][r]);
+  }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void rm_to_cm_kernel(int n, int p, const TS *src, int lds,
+                                                         double *dst, int ldd) {
+  __shared__ double tile[32][33];
+  const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, c = c0 + tx;
+    tile[r][tx] = (i < n && c < p) ? static_cast<double>(src[(size_t)i * lds + c]) : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, c = c0 + r;
+    if (i < n && c < p) dst[(size_t)c * ldd + i] = tile[tx][r];
+  }
+}
+
+// Gathers level-ordered triangular values (and diagonals) from the factor.
+__global__ __launch_bounds__(kBlock) void tri_gather_kernel(int m, const int *gather,
+                                                           const double *fac, double *v64,
+                                                           float *v32) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m) return;
+  const double v = fac[gather[k]];
+  v64[k] = v;
+  v32[k] = static_cast<float>(v);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ hosts
+template <typename TV>
+void launch_spmm(hipStream_t s, int n, const int *rowptr, const int *colidx, const void *val,
+                 bool val32, int p, const TV *X, int ldx, TV *Y, int ldy, double alpha,
+                 double beta) {
+  if (n <= 0 || p <= 0) return;
+  int glog, cpl;
+  lane_shape(p, glog, cpl);
+  const int rpb = kBlock >> glog;
+  dim3 grid((n + rpb - 1) / rpb);
+  const TV a = static_cast<TV>(alpha), b = static_cast<TV>(beta);
+#define BPF_SPMM(TA, C)                                                                 \
+  hipLaunchKernelGGL((spmm_rm_kernel<TA, TV, C>), grid, dim3(kBlock), 0, s, n, rowptr,  \
+                     colidx, static_cast<const TA *>(val), p, X, ldx, Y, ldy, a, b, glog)
+  if (val32) {
+    if (cpl == 1) BPF_SPMM(float, 1); else if (cpl == 2) BPF_SPMM(float, 2); else BPF_SPMM(float, 4);
+  } else {
+    if (cpl == 1) BPF_SPMM(double, 1); else if (cpl == 2) BPF_SPMM(double, 2); else BPF_SPMM(double, 4);
+  }
+#undef BPF_SPMM
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void launch_spmm<double>(hipStream_t, int, const int *, const int *, const void *, bool,
+                                  int, const double *, int, double *, int, double, double);
+template void launch_spmm<float>(hipStream_t, int, const int *, const int *, const void *, bool,
+                                 int, const float *, int, float *, int, double, double);
+
+void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *colidx,
+                          const void *val, bool val32, int p, const void *X, int ldx, void *Y,
+                          int ldy, double alpha, double beta, bool f32) {
+  if (n <= 0 || p <= 0) return;
+  dim3 grid((n + kBlock - 1) / kBlock, p);
+  if (f32) {
+    if (val32)
+      hipLaunchKernelGGL((spmm_cm_kernel<float, float>), grid, dim3(kBlock), 0, s, n, rowptr, colidx,
+                         static_cast<const float *>(val), p, static_cast<const float *>(X), ldx,
+                         static_cast<float *>(Y), ldy, (float)alpha, (float)beta);
+    else
+      hipLaunchKernelGGL((spmm_cm_kernel<double, float>), grid, dim3(kBlock), 0, s, n, rowptr, colidx,
+                         static_cast<const double *>(val), p, static_cast<const float *>(X), ldx,
+                         static_cast<float *>(Y), ldy, (float)alpha, (float)beta);
+  } else {
+    hipLaunchKernelGGL((spmm_cm_kernel<double, double>), grid, dim3(kBlock), 0, s, n, rowptr, colidx,
+                       static_cast<const double *>(val), p, static_cast<const double *>(X), ldx,
+                       static_cast<double *>(Y), ldy, alpha, beta);
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+template <typename TV>
+void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy) {
+  if (op.n <= 0 || p <= 0) return;
+  constexpr bool F32 = std::is_same<TV, float>::value;
+  TriDev a{op.order.as<int>(), op.ptr.as<int>(), op.col.as<int>(), op.lvl.as<int>(),
+           F32 ? op.val32.ptr : op.val.ptr, F32 ? op.diag32.ptr : op.diag.ptr};
+  int glog, cpl;
+  lane_shape(p, glog, cpl);
+  const int rpb = kBlock >> glog;
+  const auto plan = make_plan(op, 2 * (kTailBlock >> glog));
+  for (const auto &seg : plan) {
+    if (seg.wide) {
+      for (int l = seg.l0; l < seg.l1; ++l) {
+        const int t0 = op.h_lvl[l], t1 = op.h_lvl[l + 1];
+        dim3 grid((t1 - t0 + rpb - 1) / rpb);
+#define BPF_TRI(U, C)                                                                     \
+  hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C>), grid, dim3(kBlock), 0, s, a, t0, t1, \
+                     x, ldx, y, ldy, p, glog)
+        if (op.unit) {
+          if (cpl == 1) BPF_TRI(true, 1); else if (cpl == 2) BPF_TRI(true, 2); else BPF_TRI(true, 4);
+        } else {
+          if (cpl == 1) BPF_TRI(false, 1); else if (cpl == 2) BPF_TRI(false, 2); else BPF_TRI(false, 4);
+        }
+#undef BPF_TRI
+      }
+    } else {
+#define BPF_TAIL(U, C)                                                                       \
+  hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0, \
+                     seg.l1, x, ldx, y, ldy, p, glog)
+      if (op.unit) {
+        if (cpl == 1) BPF_TAIL(true, 1); else if (cpl == 2) BPF_TAIL(true, 2); else BPF_TAIL(true, 4);
+      } else {
+        if (cpl == 1) BPF_TAIL(false, 1); else if (cpl == 2) BPF_TAIL(false, 2); else BPF_TAIL(false, 4);
+      }
+#undef BPF_TAIL
+    }
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void trisolve<double>(hipStream_t, const TriOp &, int, const double *, int, double *, int);
+template void trisolve<float>(hipStream_t, const TriOp &, int, const float *, int, float *, int);
+
+static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int *rowptr,
+                          const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
+                          int *d_status) {
+  const int cap = std::max(64, max_row_nnz);
+  const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
+  if (per_wave > 160 * 1024)
+    BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
+  int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
+  auto allow = [](const void *fn, size_t shm) {
+    if (shm > 64 * 1024)
+      BPF_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  };
+  allow(reinterpret_cast<const void *>(&fac_level_kernel<true>), per_wave * wpb);
+  allow(reinterpret_cast<const void *>(&fac_level_kernel<false>), per_wave * wpb);
+  allow(reinterpret_cast<const void *>(&fac_tail_kernel<true>), per_wave);
+  allow(reinterpret_cast<const void *>(&fac_tail_kernel<false>), per_wave);
+  FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), fac, d_status, cap};
+  const auto plan = make_plan(lower, 32);
+  for (const auto &seg : plan) {
+    if (seg.wide) {
+      const size_t shm = per_wave * wpb;
+      for (int l = seg.l0; l < seg.l1; ++l) {
+        const int t0 = lower.h_lvl[l], t1 = lower.h_lvl[l + 1];
+        dim3 grid((t1 - t0 + wpb - 1) / wpb);
+        if (ic) hipLaunchKernelGGL((fac_level_kernel<true>), grid, dim3(64 * wpb), shm, s, f, t0, t1);
+        else hipLaunchKernelGGL((fac_level_kernel<false>), grid, dim3(64 * wpb), shm, s, f, t0, t1);
+      }
+    } else {
+      int tw = (int)std::min<size_t>(16, std::max<size_t>(1, (64 * 1024) / per_wave));
+      const size_t shm = per_wave * tw;
+      if (ic) hipLaunchKernelGGL((fac_tail_kernel<true>), dim3(1), dim3(64 * tw), shm, s, f, seg.l0, seg.l1);
+      else hipLaunchKernelGGL((fac_tail_kernel<false>), dim3(1), dim3(64 * tw), shm, s, f, seg.l0, seg.l1);
+    }
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void launch_ilu_factor(hipStream_t s, const TriOp &lower, int, const int *rowptr,
+                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
+                       int *d_status) {
+  launch_factor(false, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);
+}
+
+void launch_ic_factor(hipStream_t s, const TriOp &lower, int, const int *rowptr,
+                      const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
+                      int *d_status) {
+  launch_factor(true, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);
+}
+
+void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst, int ldd,
+                     bool dst32) {
+  if (n <= 0 || p <= 0) return;
+  dim3 grid((n + 31) / 32, (p + 31) / 32);
+  if (dst32)
+    hipLaunchKernelGGL((cm_to_rm_kernel<float>), grid, dim3(kBlock), 0, s, n, p, src, lds,
+                       static_cast<float *>(dst), ldd);
+  else
+    hipLaunchKernelGGL((cm_to_rm_kernel<double>), grid, dim3(kBlock), 0, s, n, p, src, lds,
+                       static_cast<double *>(dst), ldd);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void launch_rm_to_cm(hipStream_t s, int n, int p, const void *src, int lds, bool src32,
+                     double *dst, int ldd) {
+  if (n <= 0 || p <= 0) return;
+  dim3 grid((n + 31) / 32, (p + 31) / 32);
+  if (src32)
+    hipLaunchKernelGGL((rm_to_cm_kernel<float>), grid, dim3(kBlock), 0, s, n, p,
+                       static_cast<const float *>(src), lds, dst, ldd);
+  else
+    hipLaunchKernelGGL((rm_to_cm_kernel<double>), grid, dim3(kBlock), 0, s, n, p,
+                       static_cast<const double *>(src), lds, dst, ldd);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void triop_refresh(TriOp &op, const double *d_fac, hipStream_t s) {
+  const int m = (int)op.h_col.size();
+  if (m > 0) {
+    hipLaunchKernelGGL(tri_gather_kernel, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m,
+                       op.gather.as<int>(), d_fac, op.val.as<double>(), op.val32.as<float>());
+  }
+  if (!op.unit && op.n > 0) {
+    hipLaunchKernelGGL(tri_gather_kernel, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       op.n, op.dgather.as<int>(), d_fac, op.diag.as<double>(),
+                       op.diag32.as<float>());
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace bpf

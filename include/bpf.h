@@ -1,0 +1,234 @@
+/*
+ * bpf.h -- C ABI of the MI355X-native multi-right-hand-side solve library
+ * (libbpf_amd.so).  Hand-written gfx950 HIP kernels behind plain C entry
+ * points: opaque handles, plain pointers and sizes, int status (0 = OK,
+ * otherwise a BPF_E* code; bpf_last_error() gives the message of the last
+ * failure on the calling thread).
+ *
+ * Every entry point replaces a call site of exanauts/BlockPowerFlow.jl; the
+ * citation on each declaration names the reference interface it stands in
+ * for.  Dense blocks at this boundary are column-major n x p with a leading
+ * dimension, exactly as the reference's Julia matrices (CuMatrix / ROCMatrix);
+ * indices are 0-based int32 (the reference converts its 1-based Cint arrays
+ * with `.- 1` before handing them to the vendor library, src/cusolverRF.jl:251).
+ *
+ * Pointers named d_* are device pointers (allocated with bpf_malloc or any
+ * HIP allocator on the context's device); pointers named h_* are host memory.
+ * All work of a context runs in order on that context's stream.  A handle may
+ * be used by one host thread at a time.
+ */
+#ifndef BPF_H
+#define BPF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  BPF_OK = 0,
+  BPF_E_INVALID = 1,     /* bad argument (shape, size, null handle)        */
+  BPF_E_HIP = 2,         /* HIP runtime error                              */
+  BPF_E_SINGULAR = 3,    /* zero / non-positive pivot                      */
+  BPF_E_NOMEM = 4,       /* device or host allocation failed               */
+  BPF_E_UNSUPPORTED = 5, /* a size or option this build does not support   */
+  BPF_E_BREAKDOWN = 6    /* Krylov breakdown that could not be repaired    */
+};
+
+/* ---- element types and layouts ---------------------------------------- */
+enum { BPF_F64 = 0, BPF_F32 = 1 };
+enum { BPF_COL_MAJOR = 0, BPF_ROW_MAJOR = 1 };
+
+typedef struct bpf_ctx_s *bpf_ctx;
+typedef struct bpf_csr_s *bpf_csr;
+typedef struct bpf_prec_s *bpf_prec;
+typedef struct bpf_rf_s *bpf_rf;
+typedef struct bpf_batch_s *bpf_batch;
+
+const char *bpf_last_error(void);
+const char *bpf_version(void);
+
+/* ---- context, device memory ------------------------------------------ */
+/* Replaces the implicit CUDA.jl / AMDGPU.jl task-local device and default
+ * stream that every reference call runs on (src/BlockPowerFlow.jl:9-15). */
+int bpf_device_count(int *count);
+int bpf_ctx_create(int device, bpf_ctx *ctx);
+int bpf_ctx_destroy(bpf_ctx ctx);
+int bpf_ctx_synchronize(bpf_ctx ctx);
+/* Device allocation and copies; stand in for CuMatrix{T}(undef, n, p),
+ * copyto!, Array(x) (e.g. src/block_gmres.jl:45-56, test/cusolverRF.jl:23). */
+int bpf_malloc(bpf_ctx ctx, void **d_ptr, size_t bytes);
+int bpf_free(bpf_ctx ctx, void *d_ptr);
+int bpf_memcpy_h2d(bpf_ctx ctx, void *d_dst, const void *h_src, size_t bytes);
+int bpf_memcpy_d2h(bpf_ctx ctx, void *h_dst, const void *d_src, size_t bytes);
+int bpf_memcpy_d2d(bpf_ctx ctx, void *d_dst, const void *d_src, size_t bytes);
+/* Event timer over the context's stream (milliseconds between start/stop). */
+int bpf_timer_start(bpf_ctx ctx);
+int bpf_timer_stop(bpf_ctx ctx, double *ms);
+
+/* ---- CSR matrices ------------------------------------------------------ */
+/* CuSparseMatrixCSR{T,Cint}(J) / ROCSparseMatrixCSR(A): upload of a square
+ * CSR matrix with int32 indices and float64 values (benchmarks/benchmarks_amd.jl:43,
+ * test/cusolverRF.jl:8).  Column indices are sorted on upload if needed. */
+int bpf_csr_create(bpf_ctx ctx, int n, int nnz, const int32_t *h_rowptr,
+                   const int32_t *h_colidx, const double *h_val, bpf_csr *A);
+/* New values on the same pattern (gJ.nzVal .*= 2.0, test/cusolverRF.jl:24):
+ * host or device source array of nnz float64 in the pattern's order. */
+int bpf_csr_set_values(bpf_csr A, const double *h_val);
+int bpf_csr_set_values_device(bpf_csr A, const double *d_val);
+int bpf_csr_info(bpf_csr A, int *n, int *nnz, double *norm_inf);
+int bpf_csr_destroy(bpf_csr A);
+
+/* ---- SpMM ---------------------------------------------------------------- */
+/* Y = alpha*A*X + beta*Y with X, Y n x p blocks in the given layout and type.
+ * Replaces mul!(W, A, P) (src/block_gmres.jl:299, src/block_bicgstab.jl:252,264)
+ * which dispatches to cuSPARSE / rocSPARSE SpMM. */
+int bpf_spmm(bpf_ctx ctx, bpf_csr A, int dtype, int layout, int p,
+             const void *d_X, int ldx, void *d_Y, int ldy, double alpha, double beta);
+
+/* ---- ILU(0) / IC(0) preconditioners -------------------------------------- */
+enum { BPF_PREC_ILU0 = 0, BPF_PREC_IC0 = 1 };
+/* ilu0(A, nrhs) / ilu0(A) (src/preconditioners_nvidia.jl:42-47,
+ * src/preconditioners_amd.jl:43-46) and ic0(A, nrhs) / ic0(A)
+ * (src/preconditioners_nvidia.jl:6-11, src/preconditioners_amd.jl:5-9):
+ * level analysis of A's pattern (host, once) plus the numeric factorization
+ * on the device.  The factor keeps A's pattern (combined CSR: strict lower =
+ * L with unit diagonal, upper with diagonal = U; for IC0 the lower part with
+ * diagonal = L). */
+int bpf_prec_create(bpf_ctx ctx, bpf_csr A, int kind, bpf_prec *P);
+/* Numeric refactorization with A's current values (same pattern). */
+int bpf_prec_factor(bpf_prec P, bpf_csr A);
+/* y = P^{-1} x : ldiv!(y, ilu, x) (src/preconditioners_nvidia.jl:49-59,
+ * src/preconditioners_amd.jl:119-131); ldiv!(y, ic, x) (nvidia :13-23,
+ * amd :82-94).  Also what mul!(Y, N, P) means for block_bicgstab's N
+ * (src/block_bicgstab.jl:251,263).  d_Y may alias d_X. */
+int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p,
+                   const void *d_X, int ldx, void *d_Y, int ldy);
+/* Copy the combined factor values (nnz float64, A's pattern) to the host. */
+int bpf_prec_get_factor(bpf_prec P, double *h_val);
+/* Number of level sets of the lower / upper solve schedules. */
+int bpf_prec_levels(bpf_prec P, int *nlevels_lower, int *nlevels_upper);
+int bpf_prec_destroy(bpf_prec P);
+
+/* ---- block Krylov solvers ------------------------------------------------ */
+enum {
+  BPF_STATUS_UNKNOWN = 0,
+  BPF_STATUS_SOLVED = 1,    /* "solution good enough given atol and rtol" */
+  BPF_STATUS_TIRED = 2,     /* "maximum number of iterations exceeded"    */
+  BPF_STATUS_OVERTIMED = 3  /* "time limit exceeded"                      */
+};
+enum { BPF_PRECISION_FP64 = 0, BPF_PRECISION_MIXED = 1 };
+
+typedef struct {
+  int memory;          /* block-basis memory hint (reference default 20)       */
+  int restart;         /* 1: restarted block-GMRES(memory)                      */
+  int itmax;           /* 0: 2*div(n,p) for GMRES, 2n for BiCGSTAB              */
+  double atol, rtol;   /* block Frobenius criterion ||R|| <= atol + rtol||R0||  */
+  double timemax;      /* seconds; <= 0 means no limit                          */
+  int verbose;         /* print every `verbose` iterations (0 = silent)         */
+  int history;         /* record residual norms                                 */
+  /* Additions (no counterpart in the reference; defaults reproduce it): */
+  double berr_tol;     /* > 0: also stop when every RHS has backward error
+                          ||Ax-b||inf/(||A||inf||x||inf+||b||inf) <= berr_tol   */
+  int precision;       /* BPF_PRECISION_FP64 or BPF_PRECISION_MIXED             */
+  int block_width;     /* s: RHS per independent Krylov block (0 = p)           */
+  int max_refine;      /* mixed: maximum FP64 refinement steps                  */
+  double inner_rtol;   /* mixed: inner reduction factor per refinement step     */
+  int reorthogonalization; /* second block Gram-Schmidt pass (reference kwarg) */
+} bpf_solver_params;
+
+typedef struct {
+  int niter;           /* Krylov iterations (all refinement steps)              */
+  int solved;
+  int status;          /* BPF_STATUS_*                                          */
+  int refine_steps;    /* FP64 refinement steps taken (mixed)                   */
+  double timer;        /* seconds                                               */
+  int nresiduals;      /* entries written to the history buffer                 */
+  double max_berr;     /* final max backward error over RHS (if berr_tol > 0)   */
+} bpf_solver_stats;
+
+void bpf_solver_params_default(bpf_solver_params *prm);
+
+/* block_gmres(A, B; memory, N, ldiv=true, restart, atol, rtol, itmax, timemax,
+ * verbose, history) -> (X, stats)  (src/block_gmres.jl:127-154,170-411).
+ * N may be NULL (identity).  B, X: device, column-major float64 n x p.
+ * h_history (optional, capacity hist_cap) receives ||R_k||_F per iteration. */
+int bpf_block_gmres(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
+                    const double *d_B, int ldb, double *d_X, int ldx,
+                    const bpf_solver_params *prm, bpf_solver_stats *stats,
+                    double *h_history, int hist_cap);
+/* Warm start: block_gmres(A, B, X0; ...) (src/block_gmres.jl:127-140); d_X0
+ * holds the initial guess (may alias d_X). */
+int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
+                       const double *d_B, int ldb, const double *d_X0, int ldx0,
+                       double *d_X, int ldx, const bpf_solver_params *prm,
+                       bpf_solver_stats *stats, double *h_history, int hist_cap);
+/* block_bicgstab(A, B; C=B, N, atol, rtol, itmax, timemax, verbose, history)
+ * (src/block_bicgstab.jl:182-299).  d_C may be NULL (C = B). */
+int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
+                       const double *d_B, int ldb, const double *d_C, int ldc,
+                       double *d_X, int ldx, const bpf_solver_params *prm,
+                       bpf_solver_stats *stats, double *h_history, int hist_cap);
+/* Per-RHS FP64 backward error ||Ax-b||inf/(||A||inf||x||inf+||b||inf) into
+ * h_berr[p] (no reference counterpart: the reference prints norm(B - A*X),
+ * benchmarks/benchmarks_amd.jl:64). */
+int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx,
+                       const double *d_B, int ldb, double *h_berr);
+
+/* ---- batched systems ------------------------------------------------------- */
+/* Many independent systems of uneven size solved together (no reference
+ * counterpart; the reference solves one system per call).  Host CSR arrays
+ * are given per system; B/X per system are device column-major blocks. */
+int bpf_batch_create(bpf_ctx ctx, int nsys, const int32_t *n, const int32_t *nnz,
+                     const int32_t *const *h_rowptr, const int32_t *const *h_colidx,
+                     const double *const *h_val, const int32_t *p, bpf_batch *batch);
+int bpf_batch_solve(bpf_batch batch, const double *const *d_B, double *const *d_X,
+                    const bpf_solver_params *prm, bpf_solver_stats *stats,
+                    int32_t *h_solved_per_system);
+int bpf_batch_destroy(bpf_batch batch);
+
+/* ---- fixed-pattern LU refactorization (CUSOLVERRF replacement) -------------- */
+enum { BPF_ORDER_AMD = 0, BPF_ORDER_MDQ = 1, BPF_ORDER_METIS = 2, BPF_ORDER_RCM = 3, BPF_ORDER_NATURAL = 4 };
+/* CusolverRfLU(A; nrhs, ordering, tol, fast_mode) (src/cusolverRF.jl:222-301):
+ * host symbolic analysis = fill-reducing ordering + threshold-pivoted sparse
+ * LU (glu_analysis_host, :56-220), then device setup and refactorization.
+ * Raises BPF_E_SINGULAR when a pivot is below tol (the reference constructs
+ * but never throws SingularException, :164-166). nrhs is not limited. */
+int bpf_rf_create(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf *rf);
+/* update!(rflu, A) / lu!(rflu, A) (src/cusolverRF.jl:304-314,334): new values
+ * of A on the analysed pattern, same pivots, device refactorization. */
+int bpf_rf_refactor(bpf_rf rf, bpf_csr A);
+/* ldiv!(x, rflu, b) (src/cusolverRF.jl:317-324) for nrhs columns: direct solve
+ * with the current factors in FP64 (dtype BPF_F64) or with FP32 factors and
+ * FP32 triangular solves (BPF_F32, followed by nothing else). d_X may alias d_B. */
+int bpf_rf_solve(bpf_rf rf, int dtype, int nrhs, const double *d_B, int ldb,
+                 double *d_X, int ldx);
+/* FP32 triangular solves plus `refine` FP64 iterative-refinement steps with
+ * the true residual B - A X (no reference counterpart). */
+int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int ldb,
+                         double *d_X, int ldx, int refine, double *h_berr);
+/* Host copies of the analysis: permutations P, Q (n each, B = A[P,Q]) and the
+ * factor pattern / values (combined L\U CSR of B). */
+int bpf_rf_info(bpf_rf rf, int *n, int *nnz_factor, int *nlevels_lower, int *nlevels_upper);
+int bpf_rf_get_factor(bpf_rf rf, int32_t *h_P, int32_t *h_Q, int32_t *h_rowptr,
+                      int32_t *h_colidx, double *h_val);
+int bpf_rf_destroy(bpf_rf rf);   /* cudestroy!(rflu), src/cusolverRF.jl:54 */
+
+/* CusolverRfLUBatch(A, batchsize; ordering, tol, fast_mode)
+ * (src/cusolverRF.jl:336-419): one pattern, batchsize value sets. */
+int bpf_rf_batch_create(bpf_ctx ctx, bpf_csr A, int batchsize, int ordering,
+                        double tol, bpf_rf *rf);
+/* rf_batch_refactor!(rflu, Anzval) (src/cusolverRF.jl:472-481): d_nzval is
+ * nnz x batchsize column-major float64 (one value set per column). */
+int bpf_rf_batch_refactor(bpf_rf rf, const double *d_nzval, int ld);
+/* ldiv!(X, rflu, B) for the batch (src/cusolverRF.jl:451-460): column i of B
+ * is solved with factor i. */
+int bpf_rf_batch_solve(bpf_rf rf, const double *d_B, int ldb, double *d_X, int ldx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BPF_H */

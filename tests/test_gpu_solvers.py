@@ -1,0 +1,111 @@
+"""GPU block solvers against the FP64 sparse direct oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(bpf, gen, oracle, nbus, p, seed):
+    _, S = gen.make_system(nbus, p, seed=seed)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    return S, A, B, Xd
+
+
+def _rel(X, Xd):
+    return np.abs(X - Xd).max(axis=0) / np.abs(Xd).max(axis=0)
+
+
+@pytest.mark.parametrize("p", [1, 16, 32])
+def test_block_gmres_fp64_reference_criterion(bpf, gen, oracle, p):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, p, 31)
+    P = bpf.ilu0(A, p)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-10, history=True)
+    Xh = X.to_numpy()
+    assert st.solved, st
+    assert st.status == "solution good enough given atol and rtol"
+    r = np.linalg.norm(S.B - S.scipy() @ Xh)
+    assert r <= 1e-10 * np.linalg.norm(S.B) * 10
+    assert len(st.residuals) == st.niter + 1
+    assert np.all(_rel(Xh, Xd) < 1e-6)
+
+
+def test_block_gmres_unpreconditioned_finite_termination(bpf, gen, oracle):
+    # Without restart and preconditioner, block GMRES terminates in ceil(n/p) steps.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 9, 2, 32)
+    n = S.n
+    X, st = bpf.block_gmres(A, B, memory=100, atol=0.0, rtol=1e-12, itmax=(n + 1) // 2)
+    assert st.niter <= (n + 1) // 2
+    np.testing.assert_allclose(X.to_numpy(), Xd, rtol=1e-7, atol=1e-9 * np.abs(Xd).max())
+
+
+@pytest.mark.parametrize("nbus,p", [(300, 16), (2000, 64), (9000, 64)])
+def test_block_gmres_fp64_backward_error(bpf, gen, oracle, nbus, p):
+    S, A, B, Xd = _setup(bpf, gen, oracle, nbus, p, 33)
+    P = bpf.ilu0(A, p)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=30, berr_tol=1e-10, itmax=500)
+    Xh = X.to_numpy()
+    be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+    assert st.solved, st
+    assert be.max() <= 1e-10
+    assert np.all(_rel(Xh, Xd) < 1e-5)
+
+
+@pytest.mark.parametrize("nbus,p,s", [(300, 16, 0), (2000, 64, 32), (9000, 128, 0)])
+def test_block_gmres_mixed(bpf, gen, oracle, nbus, p, s):
+    S, A, B, Xd = _setup(bpf, gen, oracle, nbus, p, 34)
+    P = bpf.ilu0(A, p)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, precision="mixed", berr_tol=1e-10,
+                            block_width=s, itmax=1000)
+    Xh = X.to_numpy()
+    be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+    assert st.solved, (st, be.max())
+    assert be.max() <= 1e-10
+    assert st.refine_steps >= 1
+    assert np.all(_rel(Xh, Xd) < 1e-5)
+
+
+def test_block_gmres_restart_and_warm_start(bpf, gen, oracle):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 8, 35)
+    P = bpf.ilu0(A)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=10, restart=True, atol=0.0, rtol=1e-9,
+                            itmax=400)
+    assert st.solved
+    X0 = bpf.HipMatrix.from_numpy(Xd * (1 + 1e-3))
+    X2, st2 = bpf.block_gmres(A, B, X0, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-9)
+    assert st2.solved
+    assert st2.niter <= st.niter
+    np.testing.assert_allclose(X2.to_numpy(), Xd, rtol=1e-5, atol=1e-7 * np.abs(Xd).max())
+
+
+def test_block_gmres_itmax_status(bpf, gen, oracle):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 4, 36)
+    X, st = bpf.block_gmres(A, B, atol=0.0, rtol=1e-14, itmax=3)
+    assert not st.solved
+    assert st.niter == 3
+    assert st.status == "maximum number of iterations exceeded"
+
+
+@pytest.mark.parametrize("p", [1, 8, 32])
+def test_block_bicgstab(bpf, gen, oracle, p):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, p, 37)
+    P = bpf.ilu0(A)
+    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-10, itmax=2000, history=True)
+    Xh = X.to_numpy()
+    assert st.solved, st
+    assert np.all(_rel(Xh, Xd) < 1e-6)
+    assert abs(st.residuals[-1] - np.linalg.norm(S.B - S.scipy() @ Xh)) <= 1e-6 * np.linalg.norm(S.B)
+
+
+def test_block_bicgstab_matches_cpu_restatement(bpf, gen, oracle):
+    # p = 1, same preconditioner: the GPU recurrence and the CPU restatement of
+    # src/block_bicgstab.jl follow the same iterates to rounding.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 1, 38)
+    P = bpf.ilu0(A)
+    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-8, itmax=200, history=True)
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    Nf = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    Xc, stc = oracle.block_bicgstab(S.scipy(), S.B, N=Nf, atol=0.0, rtol=1e-8, itmax=200, history=True)
+    k = min(len(st.residuals), len(stc.residuals), 8)
+    np.testing.assert_allclose(st.residuals[:k], stc.residuals[:k], rtol=1e-6)
