@@ -23,6 +23,8 @@ namespace bpf {
 namespace {
 
 constexpr int kB = 256;
+// Dynamic LDS per workgroup used before falling back to global memory.
+constexpr size_t kLdsLimit = 64 * 1024;
 
 template <typename T> __device__ __forceinline__ double to_d(T v) { return static_cast<double>(v); }
 
@@ -195,12 +197,13 @@ __global__ __launch_bounds__(kB) void block_gemm_kernel(GemmArgs a) {
 // One workgroup per group: G (w x w, ld S) -> R upper (R^T R = G + shift I),
 // Rinv = R^{-1}.  If a pivot is not positive, restart with a shift
 // sigma = 11 (n w + w (w+1)) u ||G||_F (Fukaya et al. 2020) and flag it.
-__global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
+__global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
   extern __shared__ double sm[];
   const int g = blockIdx.x;
   const int w = a.groups[g].w;
   const int S = a.S;
-  double *G = sm;                   // w x w working copy (ld S)
+  // w x w working copy (ld S): in LDS when it fits, else in the R output.
+  double *G = use_lds ? sm : a.R + (size_t)g * a.gstride;
   __shared__ int s_fail;
   __shared__ double s_norm;
   const double *Gin = a.G + (size_t)g * a.gstride;
@@ -256,9 +259,11 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   // R = upper(G); Rinv by column-oriented back substitution, one column per thread.
   for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
     const int r = e / S, c = e % S;
-    R[r * S + c] = (r < w && c < w && c >= r) ? G[r * S + c] : (r == c && r >= w ? 1.0 : 0.0);
+    const double v = (r < w && c < w && c >= r) ? G[r * S + c] : (r == c && r >= w ? 1.0 : 0.0);
+    R[r * S + c] = v;
   }
   __syncthreads();
+  G = R;
   for (int c = threadIdx.x; c < S; c += blockDim.x) {
     // solve R x = e_c (upper) for x = column c of Rinv
     for (int r = S - 1; r >= 0; --r) {
@@ -303,26 +308,40 @@ __device__ __forceinline__ double *hzfin(double *const *cols, int k, int g, int 
 // Column b of group g: apply reflector blocks Omega_0..Omega_{k-1} (block i
 // is the compact 2S x S Householder factor stored in blocks i, i+1 of block
 // column i) to the new block column [Psi_0; ...; Psi_k], in place.
-__global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a) {
-  extern __shared__ double y[];      // (k+1)*S entries of column b
+// Column b of the block column is staged in LDS when (k+1)*S doubles fit,
+// otherwise updated in place in global memory (stride S).
+struct ColView {
+  double *base;
+  int S;
+  bool lds;
+  __device__ __forceinline__ double &operator[](int e) const {
+    return lds ? base[e] : base[(size_t)(e / S) * S * S + (size_t)(e % S) * S];
+  }
+};
+
+__global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a, int use_lds) {
+  extern __shared__ double ys[];      // (k+1)*S entries of column b
   const int b = blockIdx.x, g = blockIdx.y;
   const int S = a.S, k = a.k;
   const int w = a.groups[g].w;
   if (b >= w) return;
   double *hcol = hblock(a.cols, k, g, 0, S);
-  for (int e = threadIdx.x; e < (k + 1) * S; e += kB) y[e] = hcol[(size_t)(e / S) * S * S + (e % S) * S + b];
+  if (use_lds) {
+    for (int e = threadIdx.x; e < (k + 1) * S; e += kB) ys[e] = hcol[(size_t)(e / S) * S * S + (e % S) * S + b];
+  }
   __syncthreads();
+  const ColView yv{use_lds ? ys : hcol + b, S, use_lds != 0};
   __shared__ double red[kB / 64];
   for (int i = 0; i < k; ++i) {
     const double *H = hblock(a.cols, i, g, i, S);   // 2S x S row-major
     const double *tau = htau(a.cols, i, g, a.ng, S);
-    double *seg = y + i * S;
+    const int off = i * S;
     for (int j = 0; j < w; ++j) {
       // v = [0..0, 1, H[j+1..2S-1][j]]; w = tau_j * v^T seg
       double part = 0.0;
       for (int r = j + threadIdx.x; r < 2 * S; r += kB) {
         const double v = (r == j) ? 1.0 : H[(size_t)r * S + j];
-        part += v * seg[r];
+        part += v * yv[off + r];
       }
       for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
       if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
@@ -331,12 +350,13 @@ __global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a) {
       const double t = tau[j] * dot;
       for (int r = j + threadIdx.x; r < 2 * S; r += kB) {
         const double v = (r == j) ? 1.0 : H[(size_t)r * S + j];
-        seg[r] -= t * v;
+        yv[off + r] -= t * v;
       }
       __syncthreads();
     }
   }
-  for (int e = threadIdx.x; e < (k + 1) * S; e += kB) hcol[(size_t)(e / S) * S * S + (e % S) * S + b] = y[e];
+  if (use_lds)
+    for (int e = threadIdx.x; e < (k + 1) * S; e += kB) hcol[(size_t)(e / S) * S * S + (e % S) * S + b] = ys[e];
 }
 
 // One workgroup per group: D = [Psi_kk; C] (2S x S, row-major, in a.D) is
@@ -428,12 +448,22 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
 // ---------------------------------------------------- back substitution
 // Column b of group g: solve the block upper-triangular system with blocks
 // Rcol(j)[i] (block i of block column j, S x S row-major) for y = Zrhs(:, b).
-__global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a) {
-  extern __shared__ double y[];     // K*S
+struct YView {
+  double *base;   // LDS array, or &Y[g-block 0][0][b]
+  int S, ng;
+  bool lds;
+  __device__ __forceinline__ double &operator[](int e) const {
+    return lds ? base[e] : base[((size_t)(e / S) * ng) * S * S + (size_t)(e % S) * S];
+  }
+};
+
+__global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a, int use_lds) {
+  extern __shared__ double ys[];     // K*S
   const int b = blockIdx.x, g = blockIdx.y;
   const int S = a.S, K = a.K;
   const int w = a.groups[g].w;
   if (b >= w) return;
+  const YView y{use_lds ? ys : a.Y + (size_t)g * S * S + b, S, a.ng, use_lds != 0};
   for (int e = threadIdx.x; e < K * S; e += kB) {
     const int i = e / S, r = e % S;
     y[e] = hzfin(a.cols, i, g, a.ng, S)[(size_t)r * S + b];
@@ -461,9 +491,11 @@ __global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a) {
     }
     __syncthreads();
   }
+  __syncthreads();
   for (int e = threadIdx.x; e < K * S; e += kB) {
     const int i = e / S, r = e % S;
-    a.Y[((size_t)i * a.ng + g) * S * S + (size_t)r * S + b] = (r < w) ? y[e] : 0.0;
+    const double v = (r < w) ? y[e] : 0.0;
+    if (use_lds || r >= w) a.Y[((size_t)i * a.ng + g) * S * S + (size_t)r * S + b] = v;
   }
 }
 
@@ -603,18 +635,11 @@ template void launch_block_gemm<double, double>(hipStream_t, GemmArgs, int, int)
 template void launch_block_gemm<float, float>(hipStream_t, GemmArgs, int, int);
 template void launch_block_gemm<float, double>(hipStream_t, GemmArgs, int, int);
 
-// Kernels that stage more than 64 KiB of dynamic LDS must opt in (gfx950 has
-// 160 KiB per workgroup).
-static void allow_lds(const void *fn, size_t shm) {
-  if (shm > 64 * 1024)
-    BPF_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-}
 
 void launch_chol(hipStream_t s, CholArgs a, int ng) {
   const size_t shm = (size_t)a.S * a.S * sizeof(double);
-  if (shm > 160 * 1024) BPF_THROW(BPF_E_UNSUPPORTED, "block width %d too large for Cholesky", a.S);
-  allow_lds(reinterpret_cast<const void *>(&chol_kernel), shm);
-  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), shm, s, a);
+  const int lds = shm <= kLdsLimit ? 1 : 0;
+  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), lds ? shm : 0, s, a, lds);
   BPF_CHECK_HIP(hipGetLastError());
 }
 
@@ -628,9 +653,8 @@ void launch_trmm_upper(hipStream_t s, const double *A, const double *B, double *
 void launch_hess_apply(hipStream_t s, HessArgs a, int ng) {
   if (a.k <= 0) return;
   const size_t shm = (size_t)(a.k + 1) * a.S * sizeof(double);
-  if (shm > 160 * 1024) BPF_THROW(BPF_E_UNSUPPORTED, "Hessenberg column too tall (%d blocks)", a.k + 1);
-  allow_lds(reinterpret_cast<const void *>(&hess_apply_kernel), shm);
-  hipLaunchKernelGGL(hess_apply_kernel, dim3(a.S, ng), dim3(kB), shm, s, a);
+  const int lds = shm <= kLdsLimit ? 1 : 0;
+  hipLaunchKernelGGL(hess_apply_kernel, dim3(a.S, ng), dim3(kB), lds ? shm : 0, s, a, lds);
   BPF_CHECK_HIP(hipGetLastError());
 }
 
@@ -642,9 +666,8 @@ void launch_hess_qr(hipStream_t s, HessQrArgs a, int ng) {
 void launch_backsub(hipStream_t s, BacksubArgs a, int ng) {
   if (a.K <= 0) return;
   const size_t shm = (size_t)a.K * a.S * sizeof(double);
-  if (shm > 160 * 1024) BPF_THROW(BPF_E_UNSUPPORTED, "back substitution too tall (%d blocks)", a.K);
-  allow_lds(reinterpret_cast<const void *>(&backsub_kernel), shm);
-  hipLaunchKernelGGL(backsub_kernel, dim3(a.S, ng), dim3(kB), shm, s, a);
+  const int lds = shm <= kLdsLimit ? 1 : 0;
+  hipLaunchKernelGGL(backsub_kernel, dim3(a.S, ng), dim3(kB), lds ? shm : 0, s, a, lds);
   BPF_CHECK_HIP(hipGetLastError());
 }
 

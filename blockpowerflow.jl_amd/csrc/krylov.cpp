@@ -122,6 +122,42 @@ void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
   launch_trmm_upper(st_, R2_.as<double>(), R1_.as<double>(), Cout, S_, SS, cstride, dg, ng_);
 }
 
+// Psi_{i,k} = V_i^T W and W -= V_i Psi_{i,k}, i = 0..k, into block column k.
+// BMGS (the reference's loop, src/block_gmres.jl:301-304) projects against
+// one block at a time; BCGS projects against all blocks at once (one read of
+// the basis per pass); BCGS2 repeats the pass and accumulates (:307-313).
+template <typename TV>
+void BlockArnoldi::orthogonalize(int k, double *col) {
+  const GroupDesc *dg = d_groups_.as<GroupDesc>();
+  const int n = cfg_.n, p = cfg_.p;
+  const size_t SS = (size_t)S_ * S_;
+  const size_t gstr = (size_t)(k + 2) * SS;
+  const void *const *vt = vtab_.dev<const void>();
+  if (cfg_.orth == ORTH_BMGS) {
+    for (int i = 0; i <= k; ++i) {
+      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
+      launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, 0, gstr, false, part_);
+      GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
+      launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    }
+    return;
+  }
+  ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
+  launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
+  GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
+  launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+  if (cfg_.orth == ORTH_BCGS2) {
+    tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
+    double *t = tmpcol_.as<double>();
+    launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
+    GemmArgs gb = ga;
+    gb.psi = t;
+    launch_block_gemm<TV, TV>(st_, gb, S_, ng_);
+    for (int g = 0; g < ng_; ++g)
+      axpby(st_, (size_t)(k + 1) * SS, 1.0, t + g * gstr, 1.0, col + g * gstr);
+  }
+}
+
 template <typename TV>
 void BlockArnoldi::form(double *X, double alpha) {
   // Y = R^{-1} Z (block back substitution), then X += alpha * sum_i Z_i Y_i
@@ -202,23 +238,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
                     f32 ? A_->val32.ptr : A_->val.ptr, f32, p, Pk, p, W_.as<TV>(), p, 1.0, 0.0);
     // Block Gram-Schmidt against V_0..V_k.
     double *col = col_region(k);
+    orthogonalize<TV>(k, col);
     const size_t gstr = (size_t)(k + 2) * SS;
-    const void *const *vt = vtab_.dev<const void>();
-    ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
-    launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
-    GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
-    launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
-    if (cfg_.reorth) {
-      tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
-      double *t = tmpcol_.as<double>();
-      launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
-      GemmArgs gb = ga;
-      gb.psi = t;
-      launch_block_gemm<TV, TV>(st_, gb, S_, ng_);
-      // col += t over blocks 0..k of each group (block k+1 of t is unused).
-      for (int g = 0; g < ng_; ++g)
-        axpby(st_, (size_t)(k + 1) * SS, 1.0, t + g * gstr, 1.0, col + g * gstr);
-    }
     // V_{k+1} C = W; C into block k+1 of each group's column.
     cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr);
     vtab_.set(k + 1, V_.block(k + 1), st_);

@@ -48,8 +48,9 @@ struct ArnoldiConfig {
   int s = 0;                   // group width
   bool f32 = false;            // basis / operator precision
   bool flexible = false;       // store Z_k = N^{-1} V_k
-  bool reorth = false;         // second block Gram-Schmidt pass
+  int orth = 0;                // ORTH_BMGS / ORTH_BCGS / ORTH_BCGS2
 };
+enum { ORTH_BMGS = 0, ORTH_BCGS = 1, ORTH_BCGS2 = 2 };
 
 // Stopping decision callback: given per-column residual 2-norm estimates of
 // the (scaled) problem and the iteration count, return true to stop.
@@ -78,6 +79,7 @@ class BlockArnoldi {
                                    double *rnorm_out);
   template <typename TV> void form(double *X, double alpha);
   template <typename TV> void cholqr2(void *W, void *Vout, double *Cout, size_t cstride);
+  template <typename TV> void orthogonalize(int k, double *col);
   double *col_region(int k);
 
   const double *scale_ = nullptr;

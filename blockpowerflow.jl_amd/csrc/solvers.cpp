@@ -128,7 +128,9 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   if (prm.verbose > 0) printf("BLOCK-GMRES: system of size %d with %d right-hand sides\n", n, p);
   ArnoldiConfig cfg;
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = false;
-  cfg.reorth = prm.reorthogonalization != 0;
+  // Block MGS as in the reference; with reorthogonalization=true the
+  // two-pass classical form.
+  cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
   while (!(solved || tired || overtimed)) {
@@ -220,7 +222,8 @@ void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   col_inf(ctx, n, p, blk.B.as<double>(), blk.stat, bnorm);
   ArnoldiConfig cfg;
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = true; cfg.flexible = true;
-  cfg.reorth = prm.reorthogonalization != 0;
+  // FP32 basis: two classical passes keep it orthonormal to working precision.
+  cfg.orth = ORTH_BCGS2;
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
   int iter = 0, step = 0;
@@ -249,7 +252,13 @@ void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
     scale_rhs(ctx, n, p, blk.R.as<double>(), blk, rn, sc);
     auto stop = [&](const double *r, int it) {
       bool ok = true;
-      for (int c = 0; c < p && ok; ++c) ok = r[c] * sc[c] <= target[c];
+      double worst = 0.0;
+      for (int c = 0; c < p; ++c) {
+        ok = ok && r[c] * sc[c] <= target[c];
+        worst = std::max(worst, r[c] * sc[c] / std::max(target[c], 1e-300));
+      }
+      if (prm.verbose > 0 && it % prm.verbose == 0)
+        printf("  step %d inner %4d  max(estimate/target) %.3e\n", step, it, worst);
       if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
       (void)it;
       return ok || overtimed;

@@ -437,17 +437,9 @@ static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int 
                           int *d_status) {
   const int cap = std::max(64, max_row_nnz);
   const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
-  if (per_wave > 160 * 1024)
+  if (per_wave > 64 * 1024)
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
-  auto allow = [](const void *fn, size_t shm) {
-    if (shm > 64 * 1024)
-      BPF_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  };
-  allow(reinterpret_cast<const void *>(&fac_level_kernel<true>), per_wave * wpb);
-  allow(reinterpret_cast<const void *>(&fac_level_kernel<false>), per_wave * wpb);
-  allow(reinterpret_cast<const void *>(&fac_tail_kernel<true>), per_wave);
-  allow(reinterpret_cast<const void *>(&fac_tail_kernel<false>), per_wave);
   FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), fac, d_status, cap};
   const auto plan = make_plan(lower, 32);
   for (const auto &seg : plan) {
