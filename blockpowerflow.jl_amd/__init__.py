@@ -360,7 +360,7 @@ def _fill_stats(stats, st: _lib.SolverStats, hist, history):
 
 def _params(memory=20, restart=False, reorthogonalization=False, atol=None, rtol=None, itmax=0,
             timemax=math.inf, verbose=0, history=False, berr_tol=0.0, precision="fp64",
-            block_width=0, max_refine=10, inner_rtol=1e-6):
+            block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0):
     prm = _lib.SolverParams()
     L.bpf_solver_params_default(C.byref(prm))
     eps = math.sqrt(np.finfo(np.float64).eps)
@@ -374,10 +374,12 @@ def _params(memory=20, restart=False, reorthogonalization=False, atol=None, rtol
     prm.verbose = int(verbose)
     prm.history = int(bool(history))
     prm.berr_tol = float(berr_tol)
-    prm.precision = _lib.BPF_PRECISION_MIXED if precision in ("mixed", 1) else _lib.BPF_PRECISION_FP64
+    prm.precision = {"fp64": _lib.BPF_PRECISION_FP64, "mixed": _lib.BPF_PRECISION_MIXED,
+                     "mixed32": _lib.BPF_PRECISION_MIXED32}[precision]
     prm.block_width = int(block_width)
     prm.max_refine = int(max_refine)
     prm.inner_rtol = float(inner_rtol)
+    prm.poly_degree = int(poly_degree)
     return prm
 
 
@@ -417,7 +419,7 @@ def _check_system(A, B):
 def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, X0=None, *, M=None,
                  N=None, ldiv=False, restart=False, reorthogonalization=False, atol=None, rtol=None,
                  itmax=0, timemax=math.inf, verbose=0, history=False, berr_tol=0.0,
-                 precision="fp64", block_width=0, max_refine=10, inner_rtol=1e-6):
+                 precision="fp64", block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0):
     """block_gmres!(solver, A, B[, X0]; kwargs...) -> solver (src/block_gmres.jl:156-411)."""
     _check_system(A, B)
     if M is not None:
@@ -427,7 +429,7 @@ def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, 
     if X0 is not None:
         solver.warm_start_(X0)
     prm = _params(solver.memory, restart, reorthogonalization, atol, rtol, itmax, timemax, verbose,
-                  history, berr_tol, precision, block_width, max_refine, inner_rtol)
+                  history, berr_tol, precision, block_width, max_refine, inner_rtol, poly_degree)
     st = _lib.SolverStats()
     cap = 1 + (itmax if itmax > 0 else 2 * (A.n // max(1, B.ncols))) + 16
     hist = np.zeros(cap)
@@ -448,7 +450,7 @@ def block_gmres(A: HipSparseMatrixCSR, B: HipMatrix, X0: HipMatrix | None = None
     """(X, stats) = block_gmres(A, B[, X0]; memory, M, N, ldiv, restart,
     reorthogonalization, atol, rtol, itmax, timemax, verbose, history)
     (src/block_gmres.jl:127-154).  Extra keywords: berr_tol, precision
-    ("fp64" | "mixed"), block_width, max_refine, inner_rtol."""
+    ("fp64" | "mixed"), block_width, max_refine, inner_rtol, poly_degree."""
     solver = BlockGMRESSolver(A, B, memory=memory)
     block_gmres_(solver, A, B, X0, **kw)
     return solver.X, solver.stats
@@ -492,6 +494,16 @@ def block_bicgstab(A, B, **kw):
     verbose, history) (src/block_bicgstab.jl:182-185)."""
     solver = BlockBicgstabSolver(A, B)
     return block_bicgstab_(solver, A, B, **kw)
+
+
+def hessenberg_eigenvalues(H) -> np.ndarray:
+    """Eigenvalues of an upper Hessenberg matrix (host utility of the library)."""
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    d = H.shape[0]
+    re, im = np.zeros(d), np.zeros(d)
+    check(L.bpf_hessenberg_eigenvalues(H.ctypes.data_as(_lib.dp), d, re.ctypes.data_as(_lib.dp),
+                                       im.ctypes.data_as(_lib.dp)))
+    return re + 1j * im
 
 
 def backward_error(A: HipSparseMatrixCSR, X: HipMatrix, B: HipMatrix) -> np.ndarray:

@@ -17,7 +17,7 @@ BPF_F64, BPF_F32 = 0, 1
 BPF_COL_MAJOR, BPF_ROW_MAJOR = 0, 1
 BPF_PREC_ILU0, BPF_PREC_IC0 = 0, 1
 BPF_STATUS_UNKNOWN, BPF_STATUS_SOLVED, BPF_STATUS_TIRED, BPF_STATUS_OVERTIMED = 0, 1, 2, 3
-BPF_PRECISION_FP64, BPF_PRECISION_MIXED = 0, 1
+BPF_PRECISION_FP64, BPF_PRECISION_MIXED, BPF_PRECISION_MIXED32 = 0, 1, 2
 ORDERINGS = {"AMD": 0, "MDQ": 1, "METIS": 2, "RCM": 3, "NATURAL": 4}
 
 
@@ -28,6 +28,7 @@ class SolverParams(C.Structure):
         ("verbose", C.c_int), ("history", C.c_int),
         ("berr_tol", C.c_double), ("precision", C.c_int), ("block_width", C.c_int),
         ("max_refine", C.c_int), ("inner_rtol", C.c_double), ("reorthogonalization", C.c_int),
+        ("poly_degree", C.c_int),
     ]
 
 
@@ -69,6 +70,8 @@ SIGNATURES = {
     "bpf_memcpy_d2d": [vp, vp, vp, C.c_size_t],
     "bpf_timer_start": [vp],
     "bpf_timer_stop": [vp, dp],
+    "bpf_profile_enable": [vp, i],
+    "bpf_profile_read": [vp, i, C.POINTER(C.c_longlong), dp, dp, dp],
     "bpf_csr_create": [vp, i, i, ip, ip, dp, C.POINTER(vp)],
     "bpf_csr_set_values": [vp, dp],
     "bpf_csr_set_values_device": [vp, vp],
@@ -86,6 +89,7 @@ SIGNATURES = {
     "bpf_block_gmres_x0": [vp, vp, vp, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_block_bicgstab": [vp, vp, vp, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_backward_error": [vp, vp, i, vp, i, vp, i, dp],
+    "bpf_hessenberg_eigenvalues": [dp, i, dp, dp],
     "bpf_batch_create": [vp, i, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), ip, C.POINTER(vp)],
     "bpf_batch_solve": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolverParams), C.POINTER(SolverStats), ip],
     "bpf_batch_destroy": [vp],

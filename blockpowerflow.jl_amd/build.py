@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbpf_amd.so")
-SOURCES = ["api.cpp", "analysis.cpp", "krylov.cpp", "solvers.cpp", "bicgstab.cpp",
+SOURCES = ["api.cpp", "profile.cpp", "analysis.cpp", "krylov.cpp", "poly.cpp", "solvers.cpp", "bicgstab.cpp",
            "refactor.cpp", "ordering.cpp", "batch.cpp",
            "sparse_kernels.hip", "dense_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -295,11 +295,11 @@ int bpf_spmm(bpf_ctx ctx, bpf_csr A, int dtype, int layout, int p, const void *d
     } else if (layout == BPF_ROW_MAJOR) {
       BPF_REQUIRE(ldx >= p && ldy >= p, "leading dimension < p");
       if (f32)
-        launch_spmm<float>(ctx->stream, A->n, A->rowptr.as<int>(), A->colidx.as<int>(), val, true, p,
+        launch_spmm<float>(ctx->stream, A->n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), val, true, p,
                            static_cast<const float *>(d_X), ldx, static_cast<float *>(d_Y), ldy,
                            alpha, beta);
       else
-        launch_spmm<double>(ctx->stream, A->n, A->rowptr.as<int>(), A->colidx.as<int>(), val, false, p,
+        launch_spmm<double>(ctx->stream, A->n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), val, false, p,
                             static_cast<const double *>(d_X), ldx, static_cast<double *>(d_Y), ldy,
                             alpha, beta);
     } else {

@@ -89,7 +89,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     while (!(solved || tired || overtimed)) {
       double *Yp = N ? Y.as<double>() : P.as<double>();
       if (N) precond(P.as<double>(), Yp);                                   // :251
-      launch_spmm<double>(s, n, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, Yp, p,
+      launch_spmm<double>(s, n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, Yp, p,
                           V.as<double>(), p, 1.0, 0.0);                      // :252
       proj(Cp, V.as<double>(), K.as<double>());                             // :253
       proj(Cp, R.as<double>(), al.as<double>());                            // :254-255
@@ -100,7 +100,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
       gemm(Yp, al.as<double>(), X.as<double>(), 1.0, 1.0);                  // :261-262
       double *Yu = N ? Y.as<double>() : U.as<double>();
       if (N) precond(U.as<double>(), Yu);                                   // :263
-      launch_spmm<double>(s, n, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, Yu, p,
+      launch_spmm<double>(s, n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, Yu, p,
                           Q.as<double>(), p, 1.0, 0.0);                      // :264
       const double qu = dot(Q.as<double>(), U.as<double>());
       const double qq = dot(Q.as<double>(), Q.as<double>());

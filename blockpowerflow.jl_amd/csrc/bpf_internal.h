@@ -167,7 +167,7 @@ namespace bpf {
 // ---------------------------------------------------------------- launchers
 // Row-major blocks: element (i, c) at X[i * ld + c].
 template <typename TV>
-void launch_spmm(hipStream_t s, int n, const int *rowptr, const int *colidx,
+void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *colidx,
                  const void *val, bool val32, int p, const TV *X, int ldx, TV *Y, int ldy,
                  double alpha, double beta);
 void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *colidx,

@@ -14,6 +14,7 @@
 //   backsub_kernel              block upper-triangular solve R Y = Z
 //   colstats kernels            per-column inf-norms / 2-norms, backward error
 #include "dense.h"
+#include "profile.h"
 
 #include <algorithm>
 #include <cmath>
@@ -598,6 +599,7 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   part.ensure((size_t)nchunk * a.nb * a.ng * S * S * sizeof(double));
   a.part = part.as<double>();
   dim3 grid(a.nb, nchunk, a.ng);
+  const int tok = prof::begin(s);
   switch (S) {
     case 16: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 16>), grid, dim3(kB), 0, s, a); break;
     case 32: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 32>), grid, dim3(kB), 0, s, a); break;
@@ -610,6 +612,11 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
                      a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
                      accumulate ? 1 : 0);
   BPF_CHECK_HIP(hipGetLastError());
+  // Algorithmic traffic: each basis block and W read once; the s x s
+  // products are 2 n w^2 flops per (block, group).
+  const double cols = (double)a.ng * S;   // upper bound on sum of group widths
+  prof::end(tok, s, prof::PROJ, (double)a.n * cols * ((a.nb) * sizeof(TVV) + sizeof(TW)),
+            2.0 * a.n * (double)S * S * a.nb * a.ng);
 }
 template void launch_proj<double, double>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
 template void launch_proj<float, float>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
@@ -622,6 +629,7 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   if (a.nb <= 0) {
     if (a.beta == 1.0) return;
   }
+  const int tok = prof::begin(s);
   switch (S) {
     case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, dim3(kB), 0, s, a); break;
     case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, dim3(kB), 0, s, a); break;
@@ -630,6 +638,10 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   BPF_CHECK_HIP(hipGetLastError());
+  const double cols = (double)ng * S;
+  prof::end(tok, s, prof::GEMM,
+            (double)a.n * cols * (a.nb * sizeof(TVV) + (a.beta == 0.0 ? 1.0 : 2.0) * sizeof(TY)),
+            2.0 * a.n * (double)S * S * a.nb * ng);
 }
 template void launch_block_gemm<double, double>(hipStream_t, GemmArgs, int, int);
 template void launch_block_gemm<float, float>(hipStream_t, GemmArgs, int, int);
@@ -865,5 +877,26 @@ void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out) {
 }
 template void launch_dot<double>(hipStream_t, size_t, const double *, const double *, double *);
 template void launch_dot<float>(hipStream_t, size_t, const float *, const float *, double *);
+
+}  // namespace bpf
+
+namespace bpf {
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void lincomb_kernel(size_t m, T alpha, const T *x, T beta, T *y) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < m) y[i] = beta == T(0) ? alpha * x[i] : fma(alpha, x[i], beta * y[i]);
+}
+}  // namespace
+
+template <typename T>
+void lincomb(hipStream_t s, size_t m, double alpha, const T *x, double beta, T *y) {
+  if (m == 0) return;
+  hipLaunchKernelGGL((lincomb_kernel<T>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
+                     static_cast<T>(alpha), x, static_cast<T>(beta), y);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void lincomb<float>(hipStream_t, size_t, double, const float *, double, float *);
+template void lincomb<double>(hipStream_t, size_t, double, const double *, double, double *);
 
 }  // namespace bpf

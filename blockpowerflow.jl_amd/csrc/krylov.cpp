@@ -230,11 +230,29 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     if (N_) {
       Pk = flex ? static_cast<TV *>(Z_.block(k)) : P_.as<TV>();
       if (flex) ztab_.set(k, Pk, st_);
-      trisolve<TV>(st_, N_->lo, p, Vk, p, Pk, p);
-      trisolve<TV>(st_, N_->up, p, Pk, p, Pk, p);
+      if (cfg_.prec32 && !f32) {
+        // FP32 preconditioner on an FP64 basis: round V_k, apply, widen Z_k.
+        const size_t fb = (size_t)n * p * sizeof(float);
+        Vf_.ensure(fb);
+        Zf_.ensure(fb);
+        float *vf = Vf_.as<float>(), *zf = Zf_.as<float>();
+        copy_cast<TV, float>(st_, n, p, Vk, p, vf, p, 1.0);
+        if (flex && poly_ && poly_->degree() > 0) {
+          poly_->apply<float>(st_, A_, N_, n, p, vf, zf);
+        } else {
+          trisolve<float>(st_, N_->lo, p, vf, p, zf, p);
+          trisolve<float>(st_, N_->up, p, zf, p, zf, p);
+        }
+        copy_cast<float, TV>(st_, n, p, zf, p, Pk, p, 1.0);
+      } else if (flex && poly_ && poly_->degree() > 0) {
+        poly_->apply<TV>(st_, A_, N_, n, p, Vk, Pk);
+      } else {
+        trisolve<TV>(st_, N_->lo, p, Vk, p, Pk, p);
+        trisolve<TV>(st_, N_->up, p, Pk, p, Pk, p);
+      }
     }
     // W = A P
-    launch_spmm<TV>(st_, n, A_->rowptr.as<int>(), A_->colidx.as<int>(),
+    launch_spmm<TV>(st_, n, A_->nnz, A_->rowptr.as<int>(), A_->colidx.as<int>(),
                     f32 ? A_->val32.ptr : A_->val.ptr, f32, p, Pk, p, W_.as<TV>(), p, 1.0, 0.0);
     // Block Gram-Schmidt against V_0..V_k.
     double *col = col_region(k);
@@ -268,7 +286,7 @@ template int BlockArnoldi::run_t<float>(const double *, double *, int, const Sto
 // ------------------------------------------------------------- utilities
 void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double *X, double *R) {
   BPF_CHECK_HIP(hipMemcpyAsync(R, B, (size_t)A->n * p * sizeof(double), hipMemcpyDeviceToDevice, s));
-  launch_spmm<double>(s, A->n, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, X, p,
+  launch_spmm<double>(s, A->n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, p, X, p,
                       R, p, -1.0, 1.0);
 }
 

@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "dense.h"
+#include "poly.h"
 
 namespace bpf {
 
@@ -49,6 +50,7 @@ struct ArnoldiConfig {
   bool f32 = false;            // basis / operator precision
   bool flexible = false;       // store Z_k = N^{-1} V_k
   int orth = 0;                // ORTH_BMGS / ORTH_BCGS / ORTH_BCGS2
+  bool prec32 = false;         // apply the preconditioner in FP32 (FP64 basis)
 };
 enum { ORTH_BMGS = 0, ORTH_BCGS = 1, ORTH_BCGS2 = 2 };
 
@@ -71,6 +73,8 @@ class BlockArnoldi {
   // Column scaling of the right-hand side handed to run(): the correction
   // added to X is multiplied column-wise by d_scale (device, p entries).
   void set_scale(const double *d_scale) { scale_ = d_scale; }
+  // Polynomial preconditioner on top of N (flexible runs only); null = off.
+  void set_poly(Poly *poly) { poly_ = poly; }
   int iterations() const { return k_; }
   bool shifted() const { return shifted_; }
 
@@ -83,6 +87,7 @@ class BlockArnoldi {
   double *col_region(int k);
 
   const double *scale_ = nullptr;
+  Poly *poly_ = nullptr;
   bpf_ctx ctx_;
   bpf_csr A_;
   bpf_prec N_;
@@ -106,6 +111,7 @@ class BlockArnoldi {
   DevBuf part_;                // projection partial sums
   DevBuf flags_;
   DevBuf Xtmp_;                // FP64 n x p scratch (non-flexible solution assembly)
+  DevBuf Vf_, Zf_;             // FP32 copies for an FP32 preconditioner
 };
 
 // Row-major FP64 residual R = B - A X.

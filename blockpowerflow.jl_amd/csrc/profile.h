@@ -1,0 +1,18 @@
+// In-process kernel timing: HIP events around instrumented launches, with the
+// algorithmic bytes / flops of each launch, summed per kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bpf {
+namespace prof {
+
+enum Family { SPMM = 0, TRSV = 1, PROJ = 2, GEMM = 3, FACTOR = 4, SMALL = 5, NFAM = 6 };
+
+bool enabled();
+// Records a start event on s when profiling is on; returns a token for end().
+int begin(hipStream_t s);
+void end(int token, hipStream_t s, Family f, double bytes, double flops);
+
+}  // namespace prof
+}  // namespace bpf

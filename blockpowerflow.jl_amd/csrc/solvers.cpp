@@ -83,10 +83,12 @@ const char *status_str(int st) {
   }
 }
 
-// Plain block GMRES (src/block_gmres.jl:170-411) in FP64 on the device.
+// Block GMRES (src/block_gmres.jl:170-411) with an FP64 basis on the device.
+// mixed = false: everything FP64 (the drop-in).  mixed = true: flexible form
+// with the preconditioner (ILU(0) solves, optional GMRES polynomial) in FP32.
 void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, int ldb,
                  const double *dX0, int ldx0, double *dX, int ldx, const bpf_solver_params &prm,
-                 bpf_solver_stats &st, double *h_hist, int hist_cap) {
+                 bpf_solver_stats &st, double *h_hist, int hist_cap, bool mixed = false) {
   const double t0 = now_s();
   const int n = A->n;
   hipStream_t s = ctx->stream;
@@ -127,13 +129,21 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   bool tired = iter >= itmax, overtimed = false;
   if (prm.verbose > 0) printf("BLOCK-GMRES: system of size %d with %d right-hand sides\n", n, p);
   ArnoldiConfig cfg;
-  cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = false;
+  cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = mixed;
+  cfg.prec32 = mixed;
   // Block MGS as in the reference; with reorthogonalization=true the
   // two-pass classical form.
   cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
+  Poly poly;
+  if (mixed && N && prm.poly_degree > 0) {
+    poly.build(ctx, A, N, prm.poly_degree);
+    arn.set_poly(&poly);
+  }
+  int cycles = 0;
   while (!(solved || tired || overtimed)) {
+    ++cycles;
     scale_rhs(ctx, n, p, blk.R.as<double>(), blk, rn, sc);
     const int maxit = prm.restart ? std::min(mem, itmax - iter) : itmax - iter;
     // Per-column targets: peek at the solution once the residual has dropped,
@@ -186,7 +196,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   st.solved = solved;
   st.status = overtimed ? BPF_STATUS_OVERTIMED : solved ? BPF_STATUS_SOLVED
                                                         : tired ? BPF_STATUS_TIRED : BPF_STATUS_UNKNOWN;
-  st.refine_steps = 0;
+  st.refine_steps = cycles - 1;
   st.nresiduals = hist.n;
   st.timer = now_s() - t0;
   if (!use_berr) {
@@ -199,9 +209,9 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   if (prm.verbose > 0) printf("%s\n", status_str(st.status));
 }
 
-// Mixed precision: FP32 flexible block GMRES (FP32 SpMM, FP32 ILU solves,
-// FP32 basis; FP64 Gram/Cholesky/Hessenberg) inside FP64 iterative
-// refinement with the true residual B - A X.
+// MIXED32: FP32 flexible block GMRES (FP32 SpMM, FP32 ILU solves, FP32
+// basis; FP64 Gram/Cholesky/Hessenberg) inside FP64 iterative refinement with
+// the true residual B - A X.
 void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, int ldb,
                  const double *dX0, int ldx0, double *dX, int ldx, const bpf_solver_params &prm,
                  bpf_solver_stats &st, double *h_hist, int hist_cap) {
@@ -221,11 +231,22 @@ void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   std::vector<double> rn, sc, bnorm, xnorm, berr(p);
   col_inf(ctx, n, p, blk.B.as<double>(), blk.stat, bnorm);
   ArnoldiConfig cfg;
-  cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = true; cfg.flexible = true;
+  cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.flexible = true;
   // FP32 basis: two classical passes keep it orthonormal to working precision.
+  cfg.f32 = true;
   cfg.orth = ORTH_BCGS2;
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
+  Poly poly;
+  if (N && prm.poly_degree > 0) {
+    poly.build(ctx, A, N, prm.poly_degree);
+    arn.set_poly(&poly);
+    if (prm.verbose > 0) {
+      printf("polynomial preconditioner: degree %d, roots", poly.degree());
+      for (auto &r : poly.roots) printf(" (%.3g,%.3g)", r.real(), r.imag());
+      printf("\n");
+    }
+  }
   int iter = 0, step = 0;
   bool solved = false, overtimed = false;
   for (;;) {
@@ -296,6 +317,7 @@ void bpf_solver_params_default(bpf_solver_params *prm) {
   prm->max_refine = 10;
   prm->inner_rtol = 1e-6;
   prm->reorthogonalization = 0;
+  prm->poly_degree = 0;
 }
 
 int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *d_B, int ldb,
@@ -310,6 +332,8 @@ int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *
       st.solved = 1;
       st.status = BPF_STATUS_SOLVED;
     } else if (prm->precision == BPF_PRECISION_MIXED) {
+      gmres_plain(ctx, A, N, p, d_B, ldb, d_X0, ldx0, d_X, ldx, *prm, st, h_history, hist_cap, true);
+    } else if (prm->precision == BPF_PRECISION_MIXED32) {
       BPF_REQUIRE(N == nullptr || N->kind == BPF_PREC_ILU0 || N->kind == BPF_PREC_IC0, "bad preconditioner");
       gmres_mixed(ctx, A, N, p, d_B, ldb, d_X0, ldx0, d_X, ldx, *prm, st, h_history, hist_cap);
     } else {

@@ -7,6 +7,7 @@
 // loads.  A group of G lanes (G = power of two <= 64) owns one matrix row and
 // spreads the p columns over its lanes (CPL columns per lane).
 #include "bpf_internal.h"
+#include "profile.h"
 
 #include <algorithm>
 #include <cmath>
@@ -343,10 +344,11 @@ __global__ __launch_bounds__(kBlock) void tri_gather_kernel(int m, const int *ga
 
 // ------------------------------------------------------------------ hosts
 template <typename TV>
-void launch_spmm(hipStream_t s, int n, const int *rowptr, const int *colidx, const void *val,
+void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *colidx, const void *val,
                  bool val32, int p, const TV *X, int ldx, TV *Y, int ldy, double alpha,
                  double beta) {
   if (n <= 0 || p <= 0) return;
+  const int tok = prof::begin(s);
   int glog, cpl;
   lane_shape(p, glog, cpl);
   const int rpb = kBlock >> glog;
@@ -362,10 +364,13 @@ void launch_spmm(hipStream_t s, int n, const int *rowptr, const int *colidx, con
   }
 #undef BPF_SPMM
   BPF_CHECK_HIP(hipGetLastError());
+  const double wv = val32 ? 4.0 : 8.0, wx = sizeof(TV);
+  prof::end(tok, s, prof::SPMM, nnz * (wv + 4.0) + 4.0 * (n + 1) + 2.0 * n * p * wx,
+            2.0 * nnz * (double)p);
 }
-template void launch_spmm<double>(hipStream_t, int, const int *, const int *, const void *, bool,
+template void launch_spmm<double>(hipStream_t, int, int, const int *, const int *, const void *, bool,
                                   int, const double *, int, double *, int, double, double);
-template void launch_spmm<float>(hipStream_t, int, const int *, const int *, const void *, bool,
+template void launch_spmm<float>(hipStream_t, int, int, const int *, const int *, const void *, bool,
                                  int, const float *, int, float *, int, double, double);
 
 void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *colidx,
@@ -405,6 +410,7 @@ void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y
       for (int l = seg.l0; l < seg.l1; ++l) {
         const int t0 = op.h_lvl[l], t1 = op.h_lvl[l + 1];
         dim3 grid((t1 - t0 + rpb - 1) / rpb);
+        const int tok = prof::begin(s);
 #define BPF_TRI(U, C)                                                                     \
   hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C>), grid, dim3(kBlock), 0, s, a, t0, t1, \
                      x, ldx, y, ldy, p, glog)
@@ -414,8 +420,12 @@ void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y
           if (cpl == 1) BPF_TRI(false, 1); else if (cpl == 2) BPF_TRI(false, 2); else BPF_TRI(false, 4);
         }
 #undef BPF_TRI
+        const double rows = t1 - t0, nz = op.h_ptr[t1] - op.h_ptr[t0], w = sizeof(TV);
+        prof::end(tok, s, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
+                  2.0 * nz * p);
       }
     } else {
+      const int tok = prof::begin(s);
 #define BPF_TAIL(U, C)                                                                       \
   hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0, \
                      seg.l1, x, ldx, y, ldy, p, glog)
@@ -425,6 +435,10 @@ void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y
         if (cpl == 1) BPF_TAIL(false, 1); else if (cpl == 2) BPF_TAIL(false, 2); else BPF_TAIL(false, 4);
       }
 #undef BPF_TAIL
+      const int t0 = op.h_lvl[seg.l0], t1 = op.h_lvl[seg.l1];
+      const double rows = t1 - t0, nz = op.h_ptr[t1] - op.h_ptr[t0], w = sizeof(TV);
+      prof::end(tok, s, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
+                2.0 * nz * p);
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
@@ -441,6 +455,7 @@ static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int 
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
   FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), fac, d_status, cap};
+  const int tok = prof::begin(s);
   const auto plan = make_plan(lower, 32);
   for (const auto &seg : plan) {
     if (seg.wide) {
@@ -459,6 +474,7 @@ static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int 
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
+  prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
 }
 
 void launch_ilu_factor(hipStream_t s, const TriOp &lower, int, const int *rowptr,

@@ -69,6 +69,18 @@ int bpf_memcpy_d2d(bpf_ctx ctx, void *d_dst, const void *d_src, size_t bytes);
 int bpf_timer_start(bpf_ctx ctx);
 int bpf_timer_stop(bpf_ctx ctx, double *ms);
 
+/* In-process kernel timing (no reference counterpart; the reference only
+ * reports wall time, src/block_utils.jl:354-356).  When enabled, instrumented
+ * launches on any context are bracketed by HIP events on their stream and
+ * summed per family with their algorithmic bytes and flops:
+ *   0 SpMM, 1 triangular-solve level launch, 2 block projection (V^T W),
+ *   3 block update (V Psi), 4 factorization, 5 small dense kernels.
+ * Enabling (or disabling) resets the counters. */
+enum { BPF_K_SPMM = 0, BPF_K_TRSV = 1, BPF_K_PROJ = 2, BPF_K_GEMM = 3, BPF_K_FACTOR = 4, BPF_K_SMALL = 5 };
+int bpf_profile_enable(bpf_ctx ctx, int on);
+int bpf_profile_read(bpf_ctx ctx, int family, long long *launches, double *ms, double *bytes,
+                     double *flops);
+
 /* ---- CSR matrices ------------------------------------------------------ */
 /* CuSparseMatrixCSR{T,Cint}(J) / ROCSparseMatrixCSR(A): upload of a square
  * CSR matrix with int32 indices and float64 values (benchmarks/benchmarks_amd.jl:43,
@@ -120,7 +132,11 @@ enum {
   BPF_STATUS_TIRED = 2,     /* "maximum number of iterations exceeded"    */
   BPF_STATUS_OVERTIMED = 3  /* "time limit exceeded"                      */
 };
-enum { BPF_PRECISION_FP64 = 0, BPF_PRECISION_MIXED = 1 };
+/* FP64: everything in FP64 (the drop-in).
+ * MIXED: FP32 SpMM / ILU(0) solves / polynomial inside the preconditioner,
+ *   FP64 flexible Krylov basis, Gram matrices, Hessenberg and residual.
+ * MIXED32: FP32 basis as well, FP64 iterative refinement on the outside. */
+enum { BPF_PRECISION_FP64 = 0, BPF_PRECISION_MIXED = 1, BPF_PRECISION_MIXED32 = 2 };
 
 typedef struct {
   int memory;          /* block-basis memory hint (reference default 20)       */
@@ -138,6 +154,8 @@ typedef struct {
   int max_refine;      /* mixed: maximum FP64 refinement steps                  */
   double inner_rtol;   /* mixed: inner reduction factor per refinement step     */
   int reorthogonalization; /* second block Gram-Schmidt pass (reference kwarg) */
+  int poly_degree;     /* mixed: degree of the GMRES polynomial applied on top of
+                          the ILU(0) preconditioner (0 = ILU(0) alone)          */
 } bpf_solver_params;
 
 typedef struct {
@@ -175,6 +193,9 @@ int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
 /* Per-RHS FP64 backward error ||Ax-b||inf/(||A||inf||x||inf+||b||inf) into
  * h_berr[p] (no reference counterpart: the reference prints norm(B - A*X),
  * benchmarks/benchmarks_amd.jl:64). */
+/* Eigenvalues of a d x d upper Hessenberg matrix (row-major), host-only
+ * utility behind the polynomial preconditioner's harmonic Ritz values. */
+int bpf_hessenberg_eigenvalues(const double *h_H, int d, double *h_re, double *h_im);
 int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx,
                        const double *d_B, int ldb, double *h_berr);
 
