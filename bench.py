@@ -137,6 +137,7 @@ def main():
     def step(precision, bw, stats_out):
         P.factor_(A)
         X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, restart=False, itmax=2000,
+                                timemax=120.0,
                                 berr_tol=BERR, precision=precision, block_width=bw, max_refine=8,
                                 poly_degree=args.poly_degree if precision == "mixed" else 0)
         be = bpf.backward_error(A, X, B)
@@ -147,6 +148,7 @@ def main():
         out = []
         for _ in range(warmup):
             step(precision, bw, out)
+            log(f"[bench] warmup {precision}: its {out[-1][0].niter} max berr {out[-1][1].max():.2e}")
         ctx.synchronize()
         if profile:
             bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))

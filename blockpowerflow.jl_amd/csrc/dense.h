@@ -54,12 +54,14 @@ struct CholArgs {
 // Hessenberg arena: block column k is a separate region cols[k] holding, per
 // group g, k+2 blocks of S x S (Psi_0..Psi_k, C; after the QR update the
 // rotated R blocks and the compact reflector of column k), then the tau
-// vectors (ng x S) and the final right-hand-side blocks Z_k (ng x S x S).
+// vectors (ng x S), the final right-hand-side blocks Z_k (ng x S x S) and the
+// triangular factors T of the compact WY form Omega_k = I - Y T Y^T (ng x S x S).
 //   block (k, g, i) = cols[k] + (g*(k+2) + i)*S*S
 //   tau   (k, g)    = cols[k] + ng*(k+2)*S*S + g*S
 //   zfin  (k, g)    = cols[k] + ng*(k+2)*S*S + ng*S + g*S*S
+//   tmat  (k, g)    = cols[k] + ng*(k+2)*S*S + ng*S + ng*S*S + g*S*S
 __host__ __device__ inline size_t hcol_elems(int k, int ng, int S) {
-  return (size_t)ng * (k + 2) * S * S + (size_t)ng * S + (size_t)ng * S * S;
+  return (size_t)ng * (k + 2) * S * S + (size_t)ng * S + 2 * (size_t)ng * S * S;
 }
 
 struct HessArgs {

@@ -62,13 +62,32 @@ __global__ void csr_norm_cast_kernel(int n, const int *rowptr, const double *val
 
 // --------------------------------------------------------- TN projection
 // part[((chunk * nb + i) * ng + g) * S*S + a*S + b] = sum_{r in chunk} V_i[r, c0+a] W[r, c0+b]
+//
+// Matrix cores: v_mfma_f64_16x16x4f64 computes a 16x16 tile D += A(16x4) B(4x16)
+// per wavefront.  Lane l supplies A[l%16][l/16] and B[l/16][l%16]; the result
+// element i of lane l is D[l/16 + 4i][l%16].  Here A = V_i^T (a x r) and
+// B = W (r x b), so each lane reads V[r0 + l/16][a0 + l%16] and
+// W[r0 + l/16][b0 + l%16] from 16-row tiles staged in LDS.  The S/16 x S/16
+// output tiles are spread over the 4 wavefronts (a 2x2 grid of tile blocks);
+// for S < 64 the wavefronts split the rows instead and are summed in LDS.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int S>
+struct TnShape {
+  static constexpr int T = S / 16;                 // tiles per dimension
+  static constexpr int WA = T >= 2 ? 2 : 1;        // wave grid over a-tiles
+  static constexpr int WB = T >= 2 ? 2 : 1;        // ... and b-tiles
+  static constexpr int KS = 4 / (WA * WB);         // waves splitting the rows
+  static constexpr int TA = T / WA, TB = T / WB;   // tiles per wave
+};
+
 template <typename TVV, typename TW, int S>
 __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
-  constexpr int TM = S / 16;
-  constexpr int RB = 16;
-  __shared__ double sv[RB][S + 1];
-  __shared__ double sw[RB][S + 1];
-  const int i = blockIdx.x;          // basis block (fastest: blocks sharing W run together)
+  using Sh = TnShape<S>;
+  constexpr int RB = 16;                           // rows staged per round
+  __shared__ double sv[RB][S + 2];
+  __shared__ double sw[RB][S + 2];
+  const int i = blockIdx.x;
   const int chunk = blockIdx.y;
   const int g = blockIdx.z;
   const GroupDesc gd = a.groups[g];
@@ -76,120 +95,215 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
   const TW *W = static_cast<const TW *>(a.W) + gd.c0;
   const int r0 = chunk * a.chunk_rows;
   const int r1 = min(a.n, r0 + a.chunk_rows);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[TM][TM];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ks = wave % Sh::KS;                    // row split index
+  const int wg = wave / Sh::KS;                    // tile-block index
+  const int wa = wg % Sh::WA, wb = wg / Sh::WA;
+  dbl4 acc[Sh::TA][Sh::TB];
 #pragma unroll
-  for (int m = 0; m < TM; ++m)
+  for (int x = 0; x < Sh::TA; ++x)
 #pragma unroll
-    for (int q = 0; q < TM; ++q) acc[m][q] = 0.0;
-  for (int rb = r0; rb < r1; rb += RB) {
-    for (int e = threadIdx.x; e < RB * S; e += kB) {
+    for (int y = 0; y < Sh::TB; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane >> 4, lc = lane & 15;
+  // Register prefetch of the next 16-row tile while the current one is used.
+  constexpr int PE = RB * S / kB;   // elements per thread per tile (>= 1)
+  double pv[PE], pw[PE];
+  auto fetch = [&](int rb) {
+#pragma unroll
+    for (int q = 0; q < PE; ++q) {
+      const int e = threadIdx.x + q * kB;
       const int rr = e / S, cc = e % S;
       const int r = rb + rr;
       const bool ok = r < r1 && cc < gd.w;
-      sv[rr][cc] = ok ? to_d(V[(size_t)r * a.ldv + cc]) : 0.0;
-      sw[rr][cc] = ok ? to_d(W[(size_t)r * a.ldw + cc]) : 0.0;
+      pv[q] = ok ? to_d(V[(size_t)r * a.ldv + cc]) : 0.0;
+      pw[q] = ok ? to_d(W[(size_t)r * a.ldw + cc]) : 0.0;
+    }
+  };
+  if (r0 < r1) fetch(r0);
+  for (int rb = r0; rb < r1; rb += RB) {
+#pragma unroll
+    for (int q = 0; q < PE; ++q) {
+      const int e = threadIdx.x + q * kB;
+      sv[e / S][e % S] = pv[q];
+      sw[e / S][e % S] = pw[q];
     }
     __syncthreads();
-#pragma unroll 4
-    for (int rr = 0; rr < RB; ++rr) {
-      double va[TM], wb[TM];
+    if (rb + RB < r1) fetch(rb + RB);
 #pragma unroll
-      for (int m = 0; m < TM; ++m) va[m] = sv[rr][ty * TM + m];
+    for (int k4 = ks; k4 < RB / 4; k4 += Sh::KS) {
+      const int rr = k4 * 4 + lr;
+      double fa[Sh::TA], fb[Sh::TB];
 #pragma unroll
-      for (int q = 0; q < TM; ++q) wb[q] = sw[rr][tx * TM + q];
+      for (int x = 0; x < Sh::TA; ++x) fa[x] = sv[rr][(wa * Sh::TA + x) * 16 + lc];
 #pragma unroll
-      for (int m = 0; m < TM; ++m)
+      for (int y = 0; y < Sh::TB; ++y) fb[y] = sw[rr][(wb * Sh::TB + y) * 16 + lc];
 #pragma unroll
-        for (int q = 0; q < TM; ++q) acc[m][q] = fma(va[m], wb[q], acc[m][q]);
+      for (int x = 0; x < Sh::TA; ++x)
+#pragma unroll
+        for (int y = 0; y < Sh::TB; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
     }
     __syncthreads();
   }
   double *out = a.part + ((size_t)(chunk * a.nb + i) * a.ng + g) * S * S;
+  if (Sh::KS == 1) {
 #pragma unroll
-  for (int m = 0; m < TM; ++m)
+    for (int x = 0; x < Sh::TA; ++x)
 #pragma unroll
-    for (int q = 0; q < TM; ++q) out[(ty * TM + m) * S + tx * TM + q] = acc[m][q];
+      for (int y = 0; y < Sh::TB; ++y)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ra = (wa * Sh::TA + x) * 16 + lr + 4 * q;
+          const int cb = (wb * Sh::TB + y) * 16 + lc;
+          out[ra * S + cb] = acc[x][y][q];
+        }
+  } else {
+    // Sum the row-split partial tiles of the KS wavefronts through LDS.
+    double *red = &sv[0][0];   // S*S <= RB*(S+2) for S <= 32
+    for (int e = threadIdx.x; e < S * S; e += kB) red[e] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < Sh::KS; ++w) {
+      if (ks == w) {
+#pragma unroll
+        for (int x = 0; x < Sh::TA; ++x)
+#pragma unroll
+          for (int y = 0; y < Sh::TB; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int ra = (wa * Sh::TA + x) * 16 + lr + 4 * q;
+              const int cb = (wb * Sh::TB + y) * 16 + lc;
+              red[ra * S + cb] += acc[x][y][q];
+            }
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < S * S; e += kB) out[e] = red[e];
+  }
 }
 
-// out[i*o_istride + g*o_gstride + e] (+)= sum_chunk part[...]
+// out[i*o_istride + g*o_gstride + e] (+)= sum_chunk part[...]: each
+// workgroup owns 64 consecutive outputs; its 4 wavefronts split the chunks
+// and combine through LDS.
 __global__ __launch_bounds__(kB) void proj_reduce_kernel(const double *part, int nchunk, int nb,
                                                          int ng, int SS, double *out,
                                                          size_t o_istride, size_t o_gstride,
                                                          int accumulate) {
+  __shared__ double red[4][64];
   const size_t total = (size_t)nb * ng * SS;
-  const size_t e = (size_t)blockIdx.x * kB + threadIdx.x;
-  if (e >= total) return;
+  const size_t e = (size_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
   double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * total + e];
-  const size_t ig = e / SS, el = e % SS;
-  const size_t i = ig / ng, g = ig % ng;
-  double *o = out + i * o_istride + g * o_gstride + el;
-  *o = accumulate ? *o + s : s;
+  if (e < total)
+    for (int c = w; c < nchunk; c += 4) s += part[(size_t)c * total + e];
+  red[w][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (w == 0 && e < total) {
+    s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    const size_t ig = e / SS, el = e % SS;
+    const size_t i = ig / ng, g = ig % ng;
+    double *o = out + i * o_istride + g * o_gstride + el;
+    *o = accumulate ? *o + s : s;
+  }
 }
 
 // ------------------------------------------------------------- NN GEMM
 // Y[r, c0+b] = beta*Y[r, c0+b] + alpha * sum_i sum_a V_i[r, c0+a] * Psi_i[a][b]
 // Psi_i for group g at psi + (i*psi_istride) + g*psi_gstride, row-major S x S.
+//
+// Matrix cores as in the projection: A = V (rows x a), B = Psi (a x b).  A
+// workgroup of 8 wavefronts owns 128 rows (one 16-row tile per wavefront)
+// and all S columns.  V (128 x 16) and Psi (16 x S) chunks are staged in LDS
+// (double buffered: the next chunk's global loads are issued before the
+// current chunk's MFMAs); each wavefront runs S/16 independent accumulators.
+// Each output row is read and written only by its own workgroup, so Y may
+// alias the (single) V block.
+constexpr int kGemmThreads = 512;
+
 template <typename TVV, typename TY, int S>
-__global__ __launch_bounds__(kB) void block_gemm_kernel(GemmArgs a) {
-  constexpr int RT = 64;            // rows per workgroup
-  constexpr int TR = RT / 16;       // rows per thread
-  constexpr int TC = S / 16;        // cols per thread
+__global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
+  constexpr int RT = 128;           // rows per workgroup
   constexpr int KA = 16;            // inner chunk
-  __shared__ double sv[RT][KA + 1];
-  __shared__ double sp[KA][S + 1];
+  constexpr int T = S / 16;
+  constexpr int NT = kGemmThreads;
+  constexpr int VE = RT * KA / NT;  // V elements per thread per chunk (4)
+  constexpr int PE = (KA * S + NT - 1) / NT;
+  __shared__ double sv[2][RT][KA + 1];
+  __shared__ double sp[2][KA][S + 2];
   const int g = blockIdx.y;
   const GroupDesc gd = a.groups[g];
   const int r0 = blockIdx.x * RT;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[TR][TC];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 acc[T];
 #pragma unroll
-  for (int m = 0; m < TR; ++m)
-#pragma unroll
-    for (int q = 0; q < TC; ++q) acc[m][q] = 0.0;
-  for (int i = 0; i < a.nb; ++i) {
+  for (int y = 0; y < T; ++y) acc[y] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int kchunks = (gd.w + KA - 1) / KA;
+  const int nchunks = a.nb * kchunks;
+  double rv[VE], rp[PE];
+  auto fetch = [&](int c) {
+    const int i = c / kchunks, k0 = (c % kchunks) * KA;
     const TVV *V = static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0;
     const double *P = a.psi + (size_t)i * a.psi_istride + (size_t)g * a.psi_gstride;
-    for (int k0 = 0; k0 < gd.w; k0 += KA) {
-      for (int e = threadIdx.x; e < RT * KA; e += kB) {
-        const int rr = e / KA, kk = e % KA;
-        const int r = r0 + rr, k = k0 + kk;
-        sv[rr][kk] = (r < a.n && k < gd.w) ? to_d(V[(size_t)r * a.ldv + k]) : 0.0;
-      }
-      for (int e = threadIdx.x; e < KA * S; e += kB) {
-        const int kk = e / S, cc = e % S;
-        const int k = k0 + kk;
-        sp[kk][cc] = (k < gd.w && cc < gd.w) ? P[(size_t)k * a.ldpsi + cc] : 0.0;
-      }
-      __syncthreads();
 #pragma unroll
-      for (int kk = 0; kk < KA; ++kk) {
-        double vr[TR], pc[TC];
-#pragma unroll
-        for (int m = 0; m < TR; ++m) vr[m] = sv[ty * TR + m][kk];
-#pragma unroll
-        for (int q = 0; q < TC; ++q) pc[q] = sp[kk][tx * TC + q];
-#pragma unroll
-        for (int m = 0; m < TR; ++m)
-#pragma unroll
-          for (int q = 0; q < TC; ++q) acc[m][q] = fma(vr[m], pc[q], acc[m][q]);
-      }
-      __syncthreads();
+    for (int q = 0; q < VE; ++q) {
+      const int e = tid + q * NT;
+      const int rr = e / KA, kk = e % KA;
+      const int r = r0 + rr, k = k0 + kk;
+      rv[q] = (r < a.n && k < gd.w) ? to_d(V[(size_t)r * a.ldv + k]) : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < PE; ++q) {
+      const int e = tid + q * NT;
+      const int kk = e / S, cc = e % S;
+      const int k = k0 + kk;
+      rp[q] = (e < KA * S && k < gd.w && cc < gd.w) ? P[(size_t)k * a.ldpsi + cc] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < VE; ++q) {
+      const int e = tid + q * NT;
+      sv[buf][e / KA][e % KA] = rv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PE; ++q) {
+      const int e = tid + q * NT;
+      if (e < KA * S) sp[buf][e / S][e % S] = rp[q];
+    }
+  };
+  if (nchunks > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) fetch(c + 1);      // global loads in flight during the MFMAs
+#pragma unroll
+    for (int k4 = 0; k4 < KA / 4; ++k4) {
+      const double fa = sv[buf][wave * 16 + lc][k4 * 4 + lr];   // A[m = lc][k = lr]
+      double fb[T];
+#pragma unroll
+      for (int y = 0; y < T; ++y) fb[y] = sp[buf][k4 * 4 + lr][y * 16 + lc];
+#pragma unroll
+      for (int y = 0; y < T; ++y) acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fb[y], acc[y], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) stash(buf ^ 1);
+    __syncthreads();
   }
   TY *Y = static_cast<TY *>(a.Y) + gd.c0;
 #pragma unroll
-  for (int m = 0; m < TR; ++m) {
-    const int r = r0 + ty * TR + m;
-    if (r >= a.n) continue;
+  for (int y = 0; y < T; ++y) {
+    const int c = y * 16 + lc;
 #pragma unroll
-    for (int q = 0; q < TC; ++q) {
-      const int c = tx * TC + q;
-      if (c >= gd.w) continue;
-      TY *y = Y + (size_t)r * a.ldy + c;
-      const double v = a.alpha * acc[m][q];
-      *y = static_cast<TY>(a.beta == 0.0 ? v : a.beta * to_d(*y) + v);
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + wave * 16 + lr + 4 * q;
+      if (r < a.n && c < gd.w) {
+        TY *yp = Y + (size_t)r * a.ldy + c;
+        const double v = a.alpha * acc[y][q];
+        *yp = static_cast<TY>(a.beta == 0.0 ? v : a.beta * to_d(*yp) + v);
+      }
     }
   }
 }
@@ -199,12 +313,12 @@ __global__ __launch_bounds__(kB) void block_gemm_kernel(GemmArgs a) {
 // Rinv = R^{-1}.  If a pivot is not positive, restart with a shift
 // sigma = 11 (n w + w (w+1)) u ||G||_F (Fukaya et al. 2020) and flag it.
 __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
-  extern __shared__ double sm[];
+  __shared__ double sm[128 * 128];   // static LDS: up to 160 KiB on gfx950
   const int g = blockIdx.x;
   const int w = a.groups[g].w;
   const int S = a.S;
-  // w x w working copy (ld S): in LDS when it fits, else in the R output.
-  double *G = use_lds ? sm : a.R + (size_t)g * a.gstride;
+  // w x w working copy (ld S) in LDS.
+  double *G = sm;
   __shared__ int s_fail;
   __shared__ double s_norm;
   const double *Gin = a.G + (size_t)g * a.gstride;
@@ -257,25 +371,32 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
     if (shift == 0.0) shift = 1e-300;
     if (threadIdx.x == 0) atomicOr(a.flags, 1);
   }
-  // R = upper(G); Rinv by column-oriented back substitution, one column per thread.
-  for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
-    const int r = e / S, c = e % S;
-    const double v = (r < w && c < w && c >= r) ? G[r * S + c] : (r == c && r >= w ? 1.0 : 0.0);
-    R[r * S + c] = v;
+  // R = upper(G).  R^{-1} (upper) is formed in LDS, stored transposed in the
+  // strict lower triangle of G with its diagonal in dinv[], one column per
+  // thread, then both are written out.
+  __shared__ double dinv[128];
+  for (int c = threadIdx.x; c < w; c += blockDim.x) dinv[c] = 1.0 / G[c * S + c];
+  __syncthreads();
+  for (int c = threadIdx.x; c < w; c += blockDim.x) {
+    // column c of R^{-1}: x[r] for r = c..0, x[c] = dinv[c]; x[r] at G[c*S + r] (r < c)
+    for (int r = c - 1; r >= 0; --r) {
+      double v = G[r * S + c] * dinv[c];                // m = c term
+      for (int m = r + 1; m < c; ++m) v += G[r * S + m] * G[c * S + m];
+      G[c * S + r] = -v * dinv[r];
+    }
   }
   __syncthreads();
-  G = R;
-  for (int c = threadIdx.x; c < S; c += blockDim.x) {
-    // solve R x = e_c (upper) for x = column c of Rinv
-    for (int r = S - 1; r >= 0; --r) {
-      double v = (r == c) ? 1.0 : 0.0;
-      if (r < w && c < w) {
-        for (int j = r + 1; j <= c; ++j) v -= G[r * S + j] * Ri[j * S + c];
-        Ri[r * S + c] = (r <= c) ? v / G[r * S + r] : 0.0;
-      } else {
-        Ri[r * S + c] = (r == c) ? 1.0 : 0.0;
-      }
+  for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
+    const int r = e / S, c = e % S;
+    double rv, iv;
+    if (r < w && c < w) {
+      rv = c >= r ? G[r * S + c] : 0.0;
+      iv = c > r ? G[c * S + r] : (c == r ? dinv[r] : 0.0);
+    } else {
+      rv = iv = (r == c) ? 1.0 : 0.0;
     }
+    R[r * S + c] = rv;
+    Ri[r * S + c] = iv;
   }
 }
 
@@ -306,11 +427,21 @@ __device__ __forceinline__ double *hzfin(double *const *cols, int k, int g, int 
   return cols[k] + (size_t)ng * (k + 2) * S * S + (size_t)ng * S + (size_t)g * S * S;
 }
 
-// Column b of group g: apply reflector blocks Omega_0..Omega_{k-1} (block i
-// is the compact 2S x S Householder factor stored in blocks i, i+1 of block
-// column i) to the new block column [Psi_0; ...; Psi_k], in place.
-// Column b of the block column is staged in LDS when (k+1)*S doubles fit,
-// otherwise updated in place in global memory (stride S).
+__device__ __forceinline__ double *htmat(double *const *cols, int k, int g, int ng, int S) {
+  return cols[k] + (size_t)ng * (k + 2) * S * S + (size_t)ng * S + (size_t)ng * S * S + (size_t)g * S * S;
+}
+
+// Reflector j of a stored block (compact 2S x S factor H, LAPACK geqrf
+// convention): Y[r][j] = 1 for r == j, H[r][j] for r > j, 0 above.
+__device__ __forceinline__ double yref(const double *H, int S, int r, int j) {
+  return r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
+}
+
+// Column b of group g: apply Omega_i^T = I - Y_i T_i^T Y_i^T, i = 0..k-1, to
+// rows [iS, iS + 2S) of the new block column [Psi_0; ...; Psi_k] (compact WY
+// form: three matrix-vector products per block instead of S reflections).
+// The column is staged in LDS when (k+1)*S doubles fit, otherwise it is
+// updated in place in global memory (stride S).
 struct ColView {
   double *base;
   int S;
@@ -322,6 +453,7 @@ struct ColView {
 
 __global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a, int use_lds) {
   extern __shared__ double ys[];      // (k+1)*S entries of column b
+  __shared__ double m1[128], m2[128];
   const int b = blockIdx.x, g = blockIdx.y;
   const int S = a.S, k = a.k;
   const int w = a.groups[g].w;
@@ -332,114 +464,240 @@ __global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a, int use_lds)
   }
   __syncthreads();
   const ColView yv{use_lds ? ys : hcol + b, S, use_lds != 0};
-  __shared__ double red[kB / 64];
   for (int i = 0; i < k; ++i) {
     const double *H = hblock(a.cols, i, g, i, S);   // 2S x S row-major
-    const double *tau = htau(a.cols, i, g, a.ng, S);
+    const double *T = htmat(a.cols, i, g, a.ng, S);
     const int off = i * S;
-    for (int j = 0; j < w; ++j) {
-      // v = [0..0, 1, H[j+1..2S-1][j]]; w = tau_j * v^T seg
-      double part = 0.0;
-      for (int r = j + threadIdx.x; r < 2 * S; r += kB) {
-        const double v = (r == j) ? 1.0 : H[(size_t)r * S + j];
-        part += v * yv[off + r];
-      }
-      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
-      __syncthreads();
-      const double dot = red[0] + red[1] + red[2] + red[3];
-      const double t = tau[j] * dot;
-      for (int r = j + threadIdx.x; r < 2 * S; r += kB) {
-        const double v = (r == j) ? 1.0 : H[(size_t)r * S + j];
-        yv[off + r] -= t * v;
-      }
-      __syncthreads();
+    // m1 = Y^T seg
+    for (int j = threadIdx.x; j < w; j += kB) {
+      double acc = yv[off + j];
+      for (int r = j + 1; r < 2 * S; ++r) acc += H[(size_t)r * S + j] * yv[off + r];
+      m1[j] = acc;
     }
+    __syncthreads();
+    // m2 = T^T m1 (T upper triangular)
+    for (int j = threadIdx.x; j < w; j += kB) {
+      double acc = 0.0;
+      for (int q = 0; q <= j; ++q) acc += T[(size_t)q * S + j] * m1[q];
+      m2[j] = acc;
+    }
+    __syncthreads();
+    // seg -= Y m2
+    for (int r = threadIdx.x; r < 2 * S; r += kB) {
+      const int jmax = min(r, w - 1);
+      double acc = 0.0;
+      for (int j = 0; j <= jmax; ++j) acc += yref(H, S, r, j) * m2[j];
+      yv[off + r] -= acc;
+    }
+    __syncthreads();
   }
   if (use_lds)
     for (int e = threadIdx.x; e < (k + 1) * S; e += kB) hcol[(size_t)(e / S) * S * S + (e % S) * S + b] = ys[e];
 }
 
-// One workgroup per group: D = [Psi_kk; C] (2S x S, row-major, in a.D) is
-// factored in place by Householder QR (LAPACK geqrf convention: reflector j
-// is v = e_j + D[j+1.., j], H = I - tau v v^T); the reflectors are applied
-// to E = [Z_k; 0] (2S x S) as they are formed.  Padding columns (>= w) are
-// left as identity.  R_kk = upper(D[0:S]), Z_k = E[0:S], Lambda = E[S:2S].
+// One workgroup per group.  D = [Psi_kk; C] (2S x S, row-major, blocks k and
+// k+1 of block column k) is factored in place by blocked Householder QR
+// (LAPACK geqrf convention: reflector j is v = e_j + D[j+1.., j] below the
+// diagonal, H_j = I - tau_j v v^T).  Panels of 16 columns are factored in
+// LDS; each panel's block reflector (I - Yp Tp Yp^T) is then applied to the
+// trailing columns in global memory.  Afterwards the triangular factor T of
+// Omega_k = H_0 ... H_{w-1} = I - Y T Y^T is formed from T^{-1} =
+// striu(Y^T Y) + diag(1/tau) (Joffrain et al. 2006) and stored, and
+// E = [Z_k; 0] is rotated: E <- Omega_k^T E.  R_kk = upper(D[0:S]); on exit
+// zfin(k) = E[0:S] and E holds [Lambda; 0] for the next column.
+constexpr int kQrPanel = 16;
+
 __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
   const int g = blockIdx.x;
   const int S = a.S;
   const int w = a.groups[g].w;
+  const int S2 = 2 * S;
   double *D = hblock(a.cols, a.k, g, a.k, S);
   double *E = a.E + (size_t)g * 2 * S * S;
   double *tau = htau(a.cols, a.k, g, a.ng, S);
+  double *Tg = htmat(a.cols, a.k, g, a.ng, S);
+  // LDS: phase 1 panel (2S x 17) + panel T (16 x 16); phase 2 T^{-1} (S x S).
+  __shared__ double lds[128 * 128];
   __shared__ double red[16];
   __shared__ double s_beta, s_tau, s_scale;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int j = 0; j < w; ++j) {
-    // norm of D[j+1.., j]
-    double part = 0.0;
-    for (int r = j + 1 + threadIdx.x; r < 2 * S; r += blockDim.x) {
-      const double v = D[(size_t)r * S + j];
-      part += v * v;
-    }
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-    if (lane == 0) red[wid] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double sig = 0.0;
-      for (int q = 0; q < nw; ++q) sig += red[q];
-      const double alpha = D[(size_t)j * S + j];
-      if (sig == 0.0) {
-        s_tau = 0.0; s_beta = alpha; s_scale = 0.0;
-      } else {
-        const double nrm = sqrt(alpha * alpha + sig);
-        const double beta = alpha <= 0.0 ? nrm : -nrm;
-        s_tau = (beta - alpha) / beta;
-        s_scale = 1.0 / (alpha - beta);
-        s_beta = beta;
-      }
-      tau[j] = s_tau;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  constexpr int PB = kQrPanel, PL = PB + 1;
+  double *pan = lds;                       // pan[r * PL + c]
+  double *Tp = lds + 256 * PL;             // Tp[q * PB + j]
+  double *W = Tp + PB * PB;                // W[c * PB + q]: panel products with trailing columns
+  for (int j0 = 0; j0 < w; j0 += PB) {
+    const int pw = min(PB, w - j0);
+    // load panel columns j0..j0+pw-1, rows j0..2S-1
+    for (int e = tid; e < S2 * PB; e += nt) {
+      const int r = e / PB, c = e % PB;
+      pan[r * PL + c] = (c < pw && r >= j0) ? D[(size_t)r * S + j0 + c] : 0.0;
     }
     __syncthreads();
-    for (int r = j + 1 + threadIdx.x; r < 2 * S; r += blockDim.x) D[(size_t)r * S + j] *= s_scale;
-    __syncthreads();
-    if (threadIdx.x == 0) D[(size_t)j * S + j] = s_beta;
-    const double t = s_tau;
-    // apply to remaining columns of D (c > j, c < w) and all w columns of E:
-    // one wave per column.
-    const int ncol = (w - j - 1) + w;
-    for (int cc = wid; cc < ncol; cc += nw) {
-      double *M = cc < (w - j - 1) ? D : E;
-      const int c = cc < (w - j - 1) ? j + 1 + cc : cc - (w - j - 1);
-      double dot = 0.0;
-      for (int r = j + lane; r < 2 * S; r += 64) {
-        const double v = (r == j) ? 1.0 : D[(size_t)r * S + j];
-        dot += v * M[(size_t)r * S + c];
+    // unblocked Householder on the panel (in LDS)
+    for (int jj = 0; jj < pw; ++jj) {
+      const int j = j0 + jj;
+      double part = 0.0;
+      for (int r = j + 1 + tid; r < S2; r += nt) part += pan[r * PL + jj] * pan[r * PL + jj];
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+      if (lane == 0) red[wid] = part;
+      __syncthreads();
+      if (tid == 0) {
+        double sig = 0.0;
+        for (int q = 0; q < nw; ++q) sig += red[q];
+        const double alpha = pan[j * PL + jj];
+        if (sig == 0.0) {
+          s_tau = 0.0; s_beta = alpha; s_scale = 0.0;
+        } else {
+          const double nrm = sqrt(alpha * alpha + sig);
+          const double beta = alpha <= 0.0 ? nrm : -nrm;
+          s_tau = (beta - alpha) / beta;
+          s_scale = 1.0 / (alpha - beta);
+          s_beta = beta;
+        }
+        tau[j] = s_tau;
       }
-      for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
-      const double f = t * dot;
-      for (int r = j + lane; r < 2 * S; r += 64) {
-        const double v = (r == j) ? 1.0 : D[(size_t)r * S + j];
-        M[(size_t)r * S + c] -= f * v;
+      __syncthreads();
+      for (int r = j + 1 + tid; r < S2; r += nt) pan[r * PL + jj] *= s_scale;
+      __syncthreads();
+      if (tid == 0) pan[j * PL + jj] = s_beta;
+      const double t = s_tau;
+      // apply to the remaining panel columns (one wave per column)
+      for (int c = jj + 1 + wid; c < pw; c += nw) {
+        double dot = 0.0;
+        for (int r = j + lane; r < S2; r += 64) dot += (r == j ? 1.0 : pan[r * PL + jj]) * pan[r * PL + c];
+        for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+        const double f = t * dot;
+        for (int r = j + lane; r < S2; r += 64) pan[r * PL + c] -= f * (r == j ? 1.0 : pan[r * PL + jj]);
       }
+      __syncthreads();
+    }
+    // panel T (pw x pw upper): T[jj][jj] = tau, T[0:jj, jj] = -tau T[0:jj,0:jj] (Yp^T y_jj)
+    for (int jj = 0; jj < pw; ++jj) {
+      const double tj = tau[j0 + jj];
+      if (tid < jj) {
+        double z = 0.0;
+        for (int r = j0 + jj; r < S2; ++r)
+          z += (r - j0 == tid ? 1.0 : (r - j0 > tid ? pan[r * PL + tid] : 0.0)) * (r == j0 + jj ? 1.0 : pan[r * PL + jj]);
+        W[tid] = z;
+      }
+      __syncthreads();
+      if (tid < jj) {
+        double acc = 0.0;
+        for (int m = tid; m < jj; ++m) acc += Tp[tid * PB + m] * W[m];
+        Tp[tid * PB + jj] = -tj * acc;
+      }
+      if (tid == 0) Tp[jj * PB + jj] = tj;
+      __syncthreads();
+    }
+    // trailing update: D[:, c] -= Yp Tp^T (Yp^T D[:, c]) for c in [j0+pw, w)
+    const int c0 = j0 + pw;
+    const int ntr = w - c0;
+    for (int cb = 0; cb < ntr; cb += 64) {
+      const int nc = min(64, ntr - cb);
+      // W[c][q] = sum_r Yp[r][q] D[r][c]
+      for (int e = tid; e < nc * pw; e += nt) {
+        const int c = e / pw, q = e % pw;
+        const int jq = j0 + q;
+        double z = D[(size_t)jq * S + c0 + cb + c];
+        for (int r = jq + 1; r < S2; ++r) z += pan[r * PL + q] * D[(size_t)r * S + c0 + cb + c];
+        W[c * PB + q] = z;
+      }
+      __syncthreads();
+      // W[c] <- Tp^T W[c]
+      for (int e = tid; e < nc * pw; e += nt) {
+        const int c = e / pw, q = e % pw;
+        double z = 0.0;
+        for (int m = 0; m <= q; ++m) z += Tp[m * PB + q] * W[c * PB + m];
+        W[(64 + c) * PB + q] = z;
+      }
+      __syncthreads();
+      // D[r][c] -= sum_q Yp[r][q] W'[c][q]
+      for (int e = tid; e < (S2 - j0) * nc; e += nt) {
+        const int r = j0 + e / nc, c = e % nc;
+        double z = 0.0;
+        for (int q = 0; q < pw; ++q) {
+          const int jq = j0 + q;
+          const double y = r == jq ? 1.0 : (r > jq ? pan[r * PL + q] : 0.0);
+          z += y * W[(64 + c) * PB + q];
+        }
+        D[(size_t)r * S + c0 + cb + c] -= z;
+      }
+      __syncthreads();
+    }
+    // write the factored panel back
+    for (int e = tid; e < S2 * PB; e += nt) {
+      const int r = e / PB, c = e % PB;
+      if (c < pw && r >= j0) D[(size_t)r * S + j0 + c] = pan[r * PL + c];
     }
     __syncthreads();
   }
-  for (int j = w + threadIdx.x; j < S; j += blockDim.x) tau[j] = 0.0;
+  for (int j = w + tid; j < S; j += nt) tau[j] = 0.0;
+  // T^{-1} = striu(Y^T Y) + diag(1/tau) in LDS (w x w, ld S), then T by
+  // column-oriented back substitution (one column per thread).
+  double *Ti = lds;
+  for (int e = tid; e < w * w; e += nt) {
+    const int q = e / w, j = e % w;
+    double v = 0.0;
+    if (q < j) {
+      for (int r = j; r < S2; ++r) v += yref(D, S, r, q) * yref(D, S, r, j);
+    } else if (q == j) {
+      v = tau[j] != 0.0 ? 1.0 / tau[j] : 0.0;
+    }
+    Ti[q * S + j] = v;
+  }
+  __syncthreads();
+  for (int c = tid; c < S; c += nt) {
+    for (int r = S - 1; r >= 0; --r) {
+      double v = 0.0;
+      if (r < w && c < w && r <= c && tau[r] != 0.0 && tau[c] != 0.0) {
+        v = (r == c) ? 1.0 : 0.0;
+        for (int m = r + 1; m <= c; ++m) v -= Ti[r * S + m] * Tg[(size_t)m * S + c];
+        v /= Ti[r * S + r];
+      }
+      Tg[(size_t)r * S + c] = v;
+    }
+  }
+  __syncthreads();
+  // E <- (I - Y T^T Y^T) E  (E = [Z_k; 0], w columns)
+  double *M1 = lds;                     // M1[q * S + c] = (Y^T E)[q][c]
+  for (int e = tid; e < w * w; e += nt) {
+    const int q = e / w, c = e % w;
+    double z = 0.0;
+    for (int r = q; r < S2; ++r) z += yref(D, S, r, q) * E[(size_t)r * S + c];
+    M1[q * S + c] = z;
+  }
+  __syncthreads();
+  for (int c = tid; c < w; c += nt) {   // M1[:, c] <- T^T M1[:, c], in place from the bottom
+    for (int q = w - 1; q >= 0; --q) {
+      double z = 0.0;
+      for (int m = 0; m <= q; ++m) z += Tg[(size_t)m * S + q] * M1[m * S + c];
+      M1[q * S + c] = z;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < S2 * w; e += nt) {
+    const int r = e / w, c = e % w;
+    double z = 0.0;
+    const int qmax = min(r, w - 1);
+    for (int q = 0; q <= qmax; ++q) z += yref(D, S, r, q) * M1[q * S + c];
+    E[(size_t)r * S + c] -= z;
+  }
+  __syncthreads();
   // residual norms of the new Lambda = E[S:2S, :w] per column
   for (int c = wid; c < w; c += nw) {
-    double s = 0.0;
-    for (int r = S + lane; r < 2 * S; r += 64) {
+    double sq = 0.0;
+    for (int r = S + lane; r < S2; r += 64) {
       const double v = E[(size_t)r * S + c];
-      s += v * v;
+      sq += v * v;
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) a.rnorm[a.groups[g].c0 + c] = sqrt(s);
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+    if (lane == 0) a.rnorm[a.groups[g].c0 + c] = sqrt(sq);
   }
   __syncthreads();
   // zfin(k) = E[0:S]; E[0:S] = E[S:2S] (next right-hand side); E[S:2S] = 0.
   double *zf = hzfin(a.cols, a.k, g, a.ng, S);
-  for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
+  for (int e = tid; e < S * S; e += nt) {
     zf[e] = E[e];
     E[e] = E[S * S + e];
     E[S * S + e] = 0.0;
@@ -472,13 +730,21 @@ __global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a, int use_lds)
   __syncthreads();
   for (int j = K - 1; j >= 0; --j) {
     const double *Rjj = hblock(a.cols, j, g, j, S);
-    // dense upper-triangular solve of block j (sequential over rows, w entries)
-    for (int r = w - 1; r >= 0; --r) {
-      __syncthreads();
-      if (threadIdx.x == 0) y[j * S + r] /= Rjj[(size_t)r * S + r];
-      __syncthreads();
-      const double yr = y[j * S + r];
-      for (int q = threadIdx.x; q < r; q += kB) y[j * S + q] -= Rjj[(size_t)q * S + r] * yr;
+    // dense upper-triangular solve of block j by wavefront 0: lane q keeps
+    // y[q] and y[q+64] in registers; the pivot value travels by shuffle.
+    if (threadIdx.x < 64) {
+      const int q0 = threadIdx.x, q1 = threadIdx.x + 64;
+      double y0 = q0 < w ? y[j * S + q0] : 0.0, y1 = q1 < w ? y[j * S + q1] : 0.0;
+      for (int r = w - 1; r >= 0; --r) {
+        const int owner = r & 63;
+        double yr = __shfl(r < 64 ? y0 : y1, owner, 64);
+        yr /= Rjj[(size_t)r * S + r];
+        if (q0 == owner) { if (r < 64) y0 = yr; else y1 = yr; }
+        if (q0 < r) y0 -= Rjj[(size_t)q0 * S + r] * yr;
+        if (q1 < r) y1 -= Rjj[(size_t)q1 * S + r] * yr;
+      }
+      if (q0 < w) y[j * S + q0] = y0;
+      if (q1 < w) y[j * S + q1] = y1;
     }
     __syncthreads();
     // y_i -= R_ij y_j for i < j
@@ -590,8 +856,9 @@ template <typename TVV, typename TW>
 void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride, size_t o_gstride,
                  bool accumulate, DevBuf &part) {
   if (a.nb <= 0 || a.ng <= 0) return;
-  // Chunk rows so that the grid has >= ~2048 workgroups.
-  int target = std::max(1, 2048 / std::max(1, a.nb * a.ng));
+  // Chunk rows so that the grid has ~512 workgroups (two per CU) while the
+  // partial sums (nchunk * nb * ng * S^2 doubles) stay small.
+  int target = std::max(1, 512 / std::max(1, a.nb * a.ng));
   int nchunk = std::max(1, std::min(target, (a.n + 255) / 256));
   a.chunk_rows = (a.n + nchunk - 1) / nchunk;
   a.chunk_rows = ((a.chunk_rows + 15) / 16) * 16;
@@ -608,7 +875,7 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   const size_t total = (size_t)a.nb * a.ng * S * S;
-  hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + kB - 1) / kB)), dim3(kB), 0, s,
+  hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(kB), 0, s,
                      a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
                      accumulate ? 1 : 0);
   BPF_CHECK_HIP(hipGetLastError());
@@ -625,16 +892,17 @@ template void launch_proj<float, double>(hipStream_t, ProjArgs, int, double *, s
 template <typename TVV, typename TY>
 void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   if (a.n <= 0 || ng <= 0) return;
-  dim3 grid((a.n + 63) / 64, ng);
+  dim3 grid((a.n + 127) / 128, ng);
   if (a.nb <= 0) {
     if (a.beta == 1.0) return;
   }
   const int tok = prof::begin(s);
+  const dim3 blk(kGemmThreads);
   switch (S) {
-    case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, dim3(kB), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, dim3(kB), 0, s, a); break;
-    case 64: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 64>), grid, dim3(kB), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 128>), grid, dim3(kB), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, blk, 0, s, a); break;
+    case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, blk, 0, s, a); break;
+    case 64: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 64>), grid, blk, 0, s, a); break;
+    case 128: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 128>), grid, blk, 0, s, a); break;
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   BPF_CHECK_HIP(hipGetLastError());
@@ -649,9 +917,8 @@ template void launch_block_gemm<float, double>(hipStream_t, GemmArgs, int, int);
 
 
 void launch_chol(hipStream_t s, CholArgs a, int ng) {
-  const size_t shm = (size_t)a.S * a.S * sizeof(double);
-  const int lds = shm <= kLdsLimit ? 1 : 0;
-  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), lds ? shm : 0, s, a, lds);
+  if (a.S > 128) BPF_THROW(BPF_E_UNSUPPORTED, "block width %d too large for Cholesky", a.S);
+  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a, 1);
   BPF_CHECK_HIP(hipGetLastError());
 }
 

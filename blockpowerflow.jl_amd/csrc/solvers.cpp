@@ -131,8 +131,9 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   ArnoldiConfig cfg;
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = mixed;
   cfg.prec32 = mixed;
-  // Block MGS as in the reference; with reorthogonalization=true the
-  // two-pass classical form.
+  // Block MGS as in the reference (src/block_gmres.jl:301-304); with
+  // reorthogonalization=true two classical passes.  (A single classical pass
+  // loses orthogonality on these Jacobians and stalls the solve.)
   cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());

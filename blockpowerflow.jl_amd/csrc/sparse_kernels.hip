@@ -530,3 +530,92 @@ void triop_refresh(TriOp &op, const double *d_fac, hipStream_t s) {
 }
 
 }  // namespace bpf
+
+// ------------------------------------------------- permutations, scatter
+namespace bpf {
+namespace {
+
+__global__ __launch_bounds__(kBlock) void scatter_values_kernel(int nnz, const int *amap,
+                                                               const double *aval, double *fac) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < nnz) fac[amap[k]] = aval[k];
+}
+
+// dst (row-major, row k) = src (column-major) row rowmap[k]
+template <typename TD>
+__global__ __launch_bounds__(kBlock) void perm_cm_to_rm_kernel(int n, int p, const double *src, int lds,
+                                                              const int *rowmap, TD *dst, int ldd) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)n * p) return;
+  const int c = (int)(e / n), k = (int)(e % n);   // consecutive threads: consecutive k
+  dst[(size_t)k * ldd + c] = static_cast<TD>(src[(size_t)c * lds + rowmap[k]]);
+}
+
+// dst (column-major) row rowmap[k] = src (row-major) row k
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void perm_rm_to_cm_kernel(int n, int p, const TS *src, int lds,
+                                                              const int *rowmap, double *dst, int ldd) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)n * p) return;
+  const int c = (int)(e / n), k = (int)(e % n);
+  dst[(size_t)c * ldd + rowmap[k]] = static_cast<double>(src[(size_t)k * lds + c]);
+}
+
+// row-major n x p: gather dst[k] = src[map[k]] or scatter dst[map[k]] = src[k]
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void perm_rows_kernel(int n, int p, const TS *src, const int *map,
+                                                          TD *dst, int scatter) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)n * p) return;
+  const int k = (int)(e / p), c = (int)(e % p);
+  if (scatter) dst[(size_t)map[k] * p + c] = static_cast<TD>(src[(size_t)k * p + c]);
+  else dst[(size_t)k * p + c] = static_cast<TD>(src[(size_t)map[k] * p + c]);
+}
+
+}  // namespace
+
+void launch_scatter_values(hipStream_t s, int nnz, const int *amap, const double *aval, double *fac) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(scatter_values_kernel, dim3((nnz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nnz,
+                     amap, aval, fac);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+template <typename TD>
+void launch_perm_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, const int *rowmap,
+                          TD *dst, int ldd) {
+  const size_t m = (size_t)n * p;
+  if (m == 0) return;
+  hipLaunchKernelGGL((perm_cm_to_rm_kernel<TD>), dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, n, p, src, lds, rowmap, dst, ldd);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void launch_perm_cm_to_rm<double>(hipStream_t, int, int, const double *, int, const int *, double *, int);
+template void launch_perm_cm_to_rm<float>(hipStream_t, int, int, const double *, int, const int *, float *, int);
+
+template <typename TS>
+void launch_perm_rm_to_cm(hipStream_t s, int n, int p, const TS *src, int lds, const int *rowmap,
+                          double *dst, int ldd) {
+  const size_t m = (size_t)n * p;
+  if (m == 0) return;
+  hipLaunchKernelGGL((perm_rm_to_cm_kernel<TS>), dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, n, p, src, lds, rowmap, dst, ldd);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void launch_perm_rm_to_cm<double>(hipStream_t, int, int, const double *, int, const int *, double *, int);
+template void launch_perm_rm_to_cm<float>(hipStream_t, int, int, const float *, int, const int *, double *, int);
+
+template <typename TS, typename TD>
+void launch_perm_rows_rm(hipStream_t s, int n, int p, const TS *src, const int *rowmap, TD *dst,
+                         bool scatter) {
+  const size_t m = (size_t)n * p;
+  if (m == 0) return;
+  hipLaunchKernelGGL((perm_rows_kernel<TS, TD>), dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, n, p, src, rowmap, dst, scatter ? 1 : 0);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void launch_perm_rows_rm<double, float>(hipStream_t, int, int, const double *, const int *, float *, bool);
+template void launch_perm_rows_rm<float, double>(hipStream_t, int, int, const float *, const int *, double *, bool);
+template void launch_perm_rows_rm<double, double>(hipStream_t, int, int, const double *, const int *, double *, bool);
+
+}  // namespace bpf
