@@ -305,7 +305,7 @@ def ldiv_(y, P, x=None):
     """ldiv!(y, P, x): y = P^{-1} x for an ILU0/IC0 preconditioner or a
     CUSOLVERRF factorization (src/preconditioners_nvidia.jl:49-59,
     src/cusolverRF.jl:317-331).  ``ldiv_(rf, x)`` solves in place."""
-    if isinstance(y, CUSOLVERRF.CusolverRfLU) and x is None:
+    if isinstance(y, CUSOLVERRF._RfBase) and x is None:
         return CUSOLVERRF._ldiv_inplace(y, P)
     if isinstance(P, CUSOLVERRF._RfBase):
         return CUSOLVERRF._ldiv(y, P, x)

@@ -1,0 +1,135 @@
+"""GPU refactorization path (CUSOLVERRF replacement) against the oracle.
+
+Replays the reference's own known-answer tests (test/cusolverRF.jl) on a
+synthetic 300-bus Jacobian, checks the device factor against the oracle's
+fixed-pattern refactorization, and runs the multi-RHS Newton sequence."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _sys(gen, nbus, p, seed):
+    grid, S = gen.make_system(nbus, p, seed=seed)
+    return grid, S
+
+
+@pytest.mark.parametrize("ordering", [":AMD", ":MDQ", ":METIS", ":RCM"])
+def test_rf_known_answers(bpf, gen, oracle, ordering):
+    _, S = _sys(gen, 300, 1, 41)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    b = np.random.default_rng(0).random(S.n)
+    sol = oracle.direct_solve(S.indptr, S.indices, S.data, b)          # J \ b (:12)
+    gb = bpf.HipMatrix.from_numpy(b.reshape(-1, 1))
+    x = bpf.HipMatrix((S.n, 1))
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, ordering=ordering)
+    bpf.ldiv_(x, rf, gb)
+    np.testing.assert_allclose(x.to_numpy().ravel(), sol, rtol=1.5e-8)   # isapprox (:21)
+    A.set_values(2.0 * S.data)                                          # nzVal .*= 2 (:24)
+    bpf.CUSOLVERRF.update_(rf, A)
+    bpf.ldiv_(x, rf, gb)
+    np.testing.assert_allclose(x.to_numpy().ravel(), 0.5 * sol, rtol=1.5e-8)   # (:28)
+    bpf.CUSOLVERRF.cudestroy_(rf)
+
+
+def test_rf_factor_matches_oracle_refactor(bpf, gen, oracle):
+    _, S = _sys(gen, 300, 1, 42)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.lu(A)
+    P, Q, rp, ci, v = rf.factors()
+    assert sorted(P) == list(range(S.n)) and sorted(Q) == list(range(S.n))
+    ref = oracle.lu_refactor(S.indptr, S.indices, S.data, P, Q, rp, ci)
+    np.testing.assert_allclose(v, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    # L U = A[P, Q]
+    F = sp.csr_matrix((v, ci, rp), shape=(S.n, S.n))
+    Lm = sp.tril(F, -1) + sp.eye(S.n)
+    U = sp.triu(F)
+    B = S.scipy()[P][:, Q]
+    assert abs(Lm @ U - B).max() <= 1e-10 * abs(B).max()
+    # new values on the same pattern: device refactor == oracle refactor
+    vals = S.data * (1.0 + 0.01 * np.random.default_rng(1).standard_normal(S.nnz))
+    A.set_values(vals)
+    bpf.CUSOLVERRF.lu_(rf, A)
+    _, _, _, _, v2 = rf.factors()
+    ref2 = oracle.lu_refactor(S.indptr, S.indices, vals, P, Q, rp, ci)
+    np.testing.assert_allclose(v2, ref2, rtol=1e-10, atol=1e-12 * np.abs(ref2).max())
+
+
+@pytest.mark.parametrize("nbus,p", [(300, 1), (2000, 64), (9000, 256)])
+def test_rf_multi_rhs(bpf, gen, oracle, nbus, p):
+    _, S = _sys(gen, nbus, p, 43)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, nrhs=p)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    X = bpf.HipMatrix(S.B.shape)
+    bpf.ldiv_(X, rf, B)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    np.testing.assert_allclose(X.to_numpy(), Xd, rtol=1e-8, atol=1e-10 * np.abs(Xd).max())
+    # in place
+    bpf.ldiv_(rf, B)
+    np.testing.assert_allclose(B.to_numpy(), Xd, rtol=1e-8, atol=1e-10 * np.abs(Xd).max())
+
+
+def test_rf_refined_newton_sequence(bpf, gen, oracle):
+    grid, S = gen.make_system(2000, 32, seed=44)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    for vals in gen.newton_value_sets(grid, 4, seed=45):
+        A.set_values(vals)
+        bpf.CUSOLVERRF.update_(rf, A)
+        X, berr = rf.solve_refined(A, B, refine=2)
+        assert berr.max() <= 1e-10, berr.max()
+        ref = oracle.backward_error(S.indptr, S.indices, vals, X.to_numpy(), S.B)
+        # both at rounding level: compare against the target, not each other
+        assert ref.max() <= 1e-10
+        np.testing.assert_allclose(berr, ref, rtol=0, atol=1e-14)
+        Xd = oracle.direct_solve(S.indptr, S.indices, vals, S.B)
+        np.testing.assert_allclose(X.to_numpy(), Xd, rtol=1e-7, atol=1e-9 * np.abs(Xd).max())
+
+
+def test_rf_batch(bpf, gen, oracle):
+    # test/cusolverRF.jl:108-143: batch of 32, RHS i*b -> i*(J\b)
+    _, S = _sys(gen, 300, 1, 46)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    nbatch = 32
+    rfb = bpf.CUSOLVERRF.CusolverRfLUBatch(A, nbatch)
+    b = np.random.default_rng(2).random(S.n)
+    sol = oracle.direct_solve(S.indptr, S.indices, S.data, b)
+    gB = [bpf.HipMatrix.from_numpy((i + 1) * b.reshape(-1, 1)) for i in range(nbatch)]
+    gX = [bpf.HipMatrix((S.n, 1)) for _ in range(nbatch)]
+    bpf.CUSOLVERRF.solve_(gX, rfb, gB)
+    for i in range(nbatch):
+        np.testing.assert_allclose(gX[i].to_numpy().ravel(), (i + 1) * sol, rtol=1.5e-8)
+    bpf.CUSOLVERRF.update_(rfb, A)
+    Bm = bpf.HipMatrix.from_numpy(np.column_stack([(i + 1) * b for i in range(nbatch)]))
+    Xm = bpf.HipMatrix((S.n, nbatch))
+    bpf.ldiv_(Xm, rfb, Bm)
+    np.testing.assert_allclose(Xm.to_numpy(), np.outer(sol, np.arange(1, nbatch + 1)), rtol=1.5e-8)
+    # rf_batch_refactor!: one value set per batch entry
+    scales = np.linspace(1.0, 2.0, nbatch)
+    rf_vals = np.column_stack([s * S.data for s in scales])
+    bpf.CUSOLVERRF.rf_batch_refactor_(rfb, rf_vals)
+    bpf.ldiv_(Xm, rfb, Bm)
+    expect = np.outer(sol, np.arange(1, nbatch + 1) / scales)
+    np.testing.assert_allclose(Xm.to_numpy(), expect, rtol=1.5e-8)
+
+
+def test_rf_batchsize_one_rejected(bpf, gen):
+    _, S = _sys(gen, 300, 1, 47)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    with pytest.raises(ValueError):
+        bpf.CUSOLVERRF.CusolverRfLUBatch(A, 1)
+
+
+def test_rf_singular_raises(bpf):
+    n = 6
+    M = sp.csr_matrix(np.diag([1.0, 2.0, 0.0, 4.0, 5.0, 6.0]) + np.diag(np.ones(n - 1), 1) * 0.0)
+    M = sp.csr_matrix(M + sp.csr_matrix(([1.0], ([2], [3])), shape=(n, n)))
+    M.eliminate_zeros()
+    # row 2 has only an entry in column 3 and column 2 is empty: structurally singular
+    M.sort_indices()
+    A = bpf.HipSparseMatrixCSR(M.indptr, M.indices, M.data)
+    with pytest.raises(bpf.SingularException):
+        bpf.CUSOLVERRF.CusolverRfLU(A)
