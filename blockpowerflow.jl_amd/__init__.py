@@ -33,6 +33,7 @@ __all__ = [
     "ilu0", "ic0", "ldiv_", "mul_", "block_gmres", "block_gmres_", "BlockGMRESSolver",
     "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
     "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
+    "block_gmres_batched",
 ]
 
 L = _lib.load()
@@ -515,4 +516,4 @@ def backward_error(A: HipSparseMatrixCSR, X: HipMatrix, B: HipMatrix) -> np.ndar
 
 
 from .cusolverrf import CUSOLVERRF  # noqa: E402
-from .batched import BatchedSystems  # noqa: E402
+from .batched import BatchedSystems, block_gmres_batched  # noqa: E402

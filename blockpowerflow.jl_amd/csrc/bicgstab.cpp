@@ -41,7 +41,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     for (DevBuf *b : {&K, &D, &al, &be}) b->ensure(SS * sizeof(double));
     piv.ensure(S * sizeof(int));
     dots.ensure(4 * sizeof(double));
-    std::vector<GroupDesc> hg{{0, p}};
+    std::vector<GroupDesc> hg{{0, p, 0, n, 0, 1 << 30, 1, 0}};
     upload(grp, hg, s);
     const GroupDesc *dg = grp.as<GroupDesc>();
     launch_cm_to_rm(s, n, p, d_B, ldb, B.ptr, p, false);

@@ -6,10 +6,19 @@
 
 namespace bpf {
 
-// One independent Krylov group: columns [c0, c0 + w) of the n x p blocks.
+// One independent Krylov group: columns [c0, c0 + w) and rows [r0, r0 + nr)
+// of the n x p blocks.  `slot` indexes the group's columns in per-column
+// arrays (residual estimates, scales); `K` caps the block columns used when
+// forming the solution; inactive groups are skipped by the Arnoldi kernels.
 struct GroupDesc {
   int c0;
   int w;
+  int r0;
+  int nr;
+  int slot;
+  int K;
+  int active;
+  int pad_;
 };
 
 // Psi_{i,g} = V_{i0+i}[:, g]^T W[:, g] for i < nb.
@@ -39,6 +48,9 @@ struct GemmArgs {
   int n;
   double alpha, beta;
   const GroupDesc *groups;
+  // 0: inactive groups are skipped.  1 (solution assembly): a group takes
+  // part iff K > 0, over its first min(nb, K) blocks.
+  int use_k = 0;
 };
 
 struct CholArgs {
@@ -119,6 +131,19 @@ void launch_scale_ycols(hipStream_t s, double *Y, int K, int ng, int S, const Gr
                         const double *scale);
 // X[:, c] *= scale[c] (row-major n x p), or /= when invert.
 void launch_scale_cols(hipStream_t s, int n, int p, double *X, int ld, const double *scale, bool invert);
+// Per-group forms for systems stacked by rows (block-diagonal operator):
+// out[slot + c] = max_r |X[r, c0 + c]| (mode 0) or sum_r X[r, c0 + c]^2
+// (mode 1) over the group's rows; out holds nslots entries.
+template <typename T>
+void colstat_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                    const T *X, int ld, int mode, double *out, int nslots);
+// X[r, c0 + c] /= scale[slot + c] (invert) or *= over each group's rows.
+void launch_scale_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                         double *X, int ld, const double *scale, bool invert);
+// Between per-group column-major blocks cm[g] (nr x w, leading dimension
+// ldc[g]) and the stacked row-major block rm (leading dimension ld).
+void launch_pack_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                        double *const *cm, const int *ldc, double *rm, int ld, bool to_rm);
 // Small dense LU with partial pivoting (one workgroup per group): A (w x w,
 // ld S) overwritten by L\U, piv (S ints); solve applies it to B (w x w).
 void launch_lu_small(hipStream_t s, double *A, int *piv, int S, const GroupDesc *groups, int ng);

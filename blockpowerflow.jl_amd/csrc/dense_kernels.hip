@@ -93,8 +93,8 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
   const GroupDesc gd = a.groups[g];
   const TVV *V = static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0;
   const TW *W = static_cast<const TW *>(a.W) + gd.c0;
-  const int r0 = chunk * a.chunk_rows;
-  const int r1 = min(a.n, r0 + a.chunk_rows);
+  const int r0 = gd.r0 + chunk * a.chunk_rows;
+  const int r1 = gd.active ? min(gd.r0 + gd.nr, r0 + a.chunk_rows) : r0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ks = wave % Sh::KS;                    // row split index
   const int wg = wave / Sh::KS;                    // tile-block index
@@ -231,15 +231,29 @@ __global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
   __shared__ double sp[2][KA][S + 2];
   const int g = blockIdx.y;
   const GroupDesc gd = a.groups[g];
-  const int r0 = blockIdx.x * RT;
+  if (blockIdx.x * RT >= gd.nr) return;
+  const int r0 = gd.r0 + blockIdx.x * RT;
+  const int rend = gd.r0 + gd.nr;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane >> 4, lc = lane & 15;
+  const bool takes_part = a.use_k ? gd.K > 0 : gd.active != 0;
+  if (!takes_part) {
+    // Skipped group: a fresh block (beta == 0) is zeroed, anything else kept.
+    if (a.beta == 0.0) {
+      TY *Y = static_cast<TY *>(a.Y) + gd.c0;
+      for (int e = tid; e < RT * gd.w; e += blockDim.x) {
+        const int r = r0 + e / gd.w, c = e % gd.w;
+        if (r < rend) Y[(size_t)r * a.ldy + c] = TY(0);
+      }
+    }
+    return;
+  }
   dbl4 acc[T];
 #pragma unroll
   for (int y = 0; y < T; ++y) acc[y] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int kchunks = (gd.w + KA - 1) / KA;
-  const int nchunks = a.nb * kchunks;
+  const int nchunks = (a.use_k ? min(a.nb, gd.K) : a.nb) * kchunks;
   double rv[VE], rp[PE];
   auto fetch = [&](int c) {
     const int i = c / kchunks, k0 = (c % kchunks) * KA;
@@ -250,7 +264,7 @@ __global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
       const int e = tid + q * NT;
       const int rr = e / KA, kk = e % KA;
       const int r = r0 + rr, k = k0 + kk;
-      rv[q] = (r < a.n && k < gd.w) ? to_d(V[(size_t)r * a.ldv + k]) : 0.0;
+      rv[q] = (r < rend && k < gd.w) ? to_d(V[(size_t)r * a.ldv + k]) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < PE; ++q) {
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = r0 + wave * 16 + lr + 4 * q;
-      if (r < a.n && c < gd.w) {
+      if (r < rend && c < gd.w) {
         TY *yp = Y + (size_t)r * a.ldy + c;
         const double v = a.alpha * acc[y][q];
         *yp = static_cast<TY>(a.beta == 0.0 ? v : a.beta * to_d(*yp) + v);
@@ -315,6 +329,7 @@ __global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
 __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
   __shared__ double sm[128 * 128];   // static LDS: up to 160 KiB on gfx950
   const int g = blockIdx.x;
+  if (!a.groups[g].active) return;
   const int w = a.groups[g].w;
   const int S = a.S;
   // w x w working copy (ld S) in LDS.
@@ -457,7 +472,7 @@ __global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a, int use_lds)
   const int b = blockIdx.x, g = blockIdx.y;
   const int S = a.S, k = a.k;
   const int w = a.groups[g].w;
-  if (b >= w) return;
+  if (b >= w || !a.groups[g].active) return;
   double *hcol = hblock(a.cols, k, g, 0, S);
   if (use_lds) {
     for (int e = threadIdx.x; e < (k + 1) * S; e += kB) ys[e] = hcol[(size_t)(e / S) * S * S + (e % S) * S + b];
@@ -509,6 +524,7 @@ constexpr int kQrPanel = 16;
 
 __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
   const int g = blockIdx.x;
+  if (!a.groups[g].active) return;
   const int S = a.S;
   const int w = a.groups[g].w;
   const int S2 = 2 * S;
@@ -692,7 +708,7 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
       sq += v * v;
     }
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-    if (lane == 0) a.rnorm[a.groups[g].c0 + c] = sqrt(sq);
+    if (lane == 0) a.rnorm[a.groups[g].slot + c] = sqrt(sq);
   }
   __syncthreads();
   // zfin(k) = E[0:S]; E[0:S] = E[S:2S] (next right-hand side); E[S:2S] = 0.
@@ -719,9 +735,14 @@ struct YView {
 __global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a, int use_lds) {
   extern __shared__ double ys[];     // K*S
   const int b = blockIdx.x, g = blockIdx.y;
-  const int S = a.S, K = a.K;
+  const int S = a.S, K = min(a.K, a.groups[g].K);
   const int w = a.groups[g].w;
   if (b >= w) return;
+  // blocks K..a.K-1 of this group's Y are unused: zero them
+  for (int e = threadIdx.x; e < (a.K - K) * S; e += kB) {
+    const int i = K + e / S, r = e % S;
+    a.Y[((size_t)i * a.ng + g) * S * S + (size_t)r * S + b] = 0.0;
+  }
   const YView y{use_lds ? ys : a.Y + (size_t)g * S * S + b, S, a.ng, use_lds != 0};
   for (int e = threadIdx.x; e < K * S; e += kB) {
     const int i = e / S, r = e % S;
@@ -892,7 +913,7 @@ template void launch_proj<float, double>(hipStream_t, ProjArgs, int, double *, s
 template <typename TVV, typename TY>
 void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   if (a.n <= 0 || ng <= 0) return;
-  dim3 grid((a.n + 127) / 128, ng);
+  dim3 grid((a.n + 127) / 128, ng);   // a.n: largest group row count
   if (a.nb <= 0) {
     if (a.beta == 1.0) return;
   }
@@ -1000,7 +1021,7 @@ __global__ __launch_bounds__(256) void scale_ycols_kernel(double *Y, int K, int 
   if (e >= (size_t)K * ng * SS) return;
   const int g = (int)((e / SS) % ng);
   const int b = (int)(e % S);
-  if (b < groups[g].w) Y[e] *= scale[groups[g].c0 + b];
+  if (b < groups[g].w) Y[e] *= scale[groups[g].slot + b];
 }
 
 __global__ __launch_bounds__(256) void scale_cols_kernel(int n, int p, double *X, int ld,
@@ -1010,6 +1031,56 @@ __global__ __launch_bounds__(256) void scale_cols_kernel(int n, int p, double *X
   const int r = (int)(e / p), c = (int)(e % p);
   const double sc = scale[c];
   X[(size_t)r * ld + c] = invert ? X[(size_t)r * ld + c] / sc : X[(size_t)r * ld + c] * sc;
+}
+
+// Grid (row chunks, group, 64-column slice); threads: 64 columns x 4 rows.
+template <typename T>
+__global__ __launch_bounds__(kB) void colstat_groups_kernel(const GroupDesc *groups, const T *X, int ld,
+                                                            int mode, double *out) {
+  const GroupDesc gd = groups[blockIdx.y];
+  const int c = blockIdx.z * 64 + (threadIdx.x & 63);
+  if (c >= gd.w) return;
+  const int rend = gd.r0 + gd.nr;
+  const int rstride = (kB / 64) * gridDim.x;
+  double m = 0.0;
+  for (int r = gd.r0 + blockIdx.x * (kB / 64) + (threadIdx.x >> 6); r < rend; r += rstride) {
+    const double v = to_d(X[(size_t)r * ld + gd.c0 + c]);
+    m = mode == 0 ? fmax(m, fabs(v)) : m + v * v;
+  }
+  if (mode == 0) atomic_max_nonneg(out + gd.slot + c, m);
+  else atomicAdd(out + gd.slot + c, m);
+}
+
+__global__ __launch_bounds__(kB) void scale_groups_kernel(const GroupDesc *groups, double *X, int ld,
+                                                          const double *scale, int invert) {
+  const GroupDesc gd = groups[blockIdx.y];
+  const int c = blockIdx.z * 64 + (threadIdx.x & 63);
+  if (c >= gd.w) return;
+  const int rend = gd.r0 + gd.nr;
+  const int rstride = (kB / 64) * gridDim.x;
+  const double sc = scale[gd.slot + c];
+  for (int r = gd.r0 + blockIdx.x * (kB / 64) + (threadIdx.x >> 6); r < rend; r += rstride) {
+    double *x = X + (size_t)r * ld + gd.c0 + c;
+    *x = invert ? *x / sc : *x * sc;
+  }
+}
+
+// Element e of group g's nr x w block: row e / w, column e % w (row-major
+// side coalesced).
+__global__ __launch_bounds__(kB) void pack_groups_kernel(const GroupDesc *groups, double *const *cm,
+                                                         const int *ldc, double *rm, int ld, int to_rm) {
+  const int g = blockIdx.y;
+  const GroupDesc gd = groups[g];
+  const size_t m = (size_t)gd.nr * gd.w;
+  double *C = cm[g];
+  const int lc = ldc[g];
+  for (size_t e = (size_t)blockIdx.x * kB + threadIdx.x; e < m; e += (size_t)kB * gridDim.x) {
+    const int i = (int)(e / gd.w), c = (int)(e % gd.w);
+    double *r = rm + (size_t)(gd.r0 + i) * ld + gd.c0 + c;
+    double *x = C + (size_t)c * lc + i;
+    if (to_rm) *r = *x;
+    else *x = *r;
+  }
 }
 
 __global__ __launch_bounds__(1024) void lu_small_kernel(double *A0, int *piv0, int S,
@@ -1119,6 +1190,44 @@ void launch_scale_cols(hipStream_t s, int n, int p, double *X, int ld, const dou
   if (m == 0) return;
   hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, n, p, X, ld,
                      scale, invert ? 1 : 0);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+namespace {
+dim3 group_grid(int ng, int gmax_rows, int wmax, int rows_per_wg) {
+  const int gx = std::max(1, std::min(256, (gmax_rows + rows_per_wg - 1) / rows_per_wg));
+  return dim3(gx, ng, (wmax + 63) / 64);
+}
+}  // namespace
+
+template <typename T>
+void colstat_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                    const T *X, int ld, int mode, double *out, int nslots) {
+  if (nslots <= 0) return;
+  fill_double(s, out, 0.0, nslots);
+  if (ng <= 0 || gmax_rows <= 0 || wmax <= 0) return;
+  hipLaunchKernelGGL((colstat_groups_kernel<T>), group_grid(ng, gmax_rows, wmax, 256), dim3(kB), 0, s,
+                     groups, X, ld, mode, out);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void colstat_groups<double>(hipStream_t, const GroupDesc *, int, int, int, const double *, int,
+                                     int, double *, int);
+
+void launch_scale_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                         double *X, int ld, const double *scale, bool invert) {
+  if (ng <= 0 || gmax_rows <= 0 || wmax <= 0) return;
+  hipLaunchKernelGGL(scale_groups_kernel, group_grid(ng, gmax_rows, wmax, 256), dim3(kB), 0, s, groups,
+                     X, ld, scale, invert ? 1 : 0);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void launch_pack_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
+                        double *const *cm, const int *ldc, double *rm, int ld, bool to_rm) {
+  if (ng <= 0 || gmax_rows <= 0 || wmax <= 0) return;
+  const size_t m = (size_t)gmax_rows * wmax;
+  const int gx = (int)std::max<size_t>(1, std::min<size_t>(1024, (m + kB * 4 - 1) / (kB * 4)));
+  hipLaunchKernelGGL(pack_groups_kernel, dim3(gx, ng), dim3(kB), 0, s, groups, cm, ldc, rm, ld,
+                     to_rm ? 1 : 0);
   BPF_CHECK_HIP(hipGetLastError());
 }
 

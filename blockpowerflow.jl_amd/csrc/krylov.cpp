@@ -61,15 +61,30 @@ void PtrTable::set(int i, void *ptr, hipStream_t s) {
 BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConfig &cfg)
     : ctx_(ctx), A_(A), N_(N), cfg_(cfg), st_(ctx->stream) {
   BPF_REQUIRE(cfg.p > 0 && cfg.n > 0, "empty block");
-  int s = cfg.s > 0 ? std::min(cfg.s, cfg.p) : cfg.p;
-  if (s > 128) {
-    // Split into near-equal groups of at most 128 columns.
-    const int ng = (cfg.p + 127) / 128;
-    s = (cfg.p + ng - 1) / ng;
+  if (!cfg.groups.empty()) {
+    h_groups_ = cfg.groups;
+    int wmax = 0;
+    for (const auto &g : h_groups_) {
+      wmax = std::max(wmax, g.w);
+      nslots_ = std::max(nslots_, g.slot + g.w);
+      gmax_rows_ = std::max(gmax_rows_, g.nr);
+    }
+    S_ = pad_width(wmax);
+    BPF_REQUIRE(S_ > 0, "block width %d unsupported", wmax);
+  } else {
+    int s = cfg.s > 0 ? std::min(cfg.s, cfg.p) : cfg.p;
+    if (s > 128) {
+      // Split into near-equal groups of at most 128 columns.
+      const int ng = (cfg.p + 127) / 128;
+      s = (cfg.p + ng - 1) / ng;
+    }
+    S_ = pad_width(s);
+    BPF_REQUIRE(S_ > 0, "block width %d unsupported", s);
+    for (int c0 = 0; c0 < cfg.p; c0 += s)
+      h_groups_.push_back({c0, std::min(s, cfg.p - c0), 0, cfg.n, c0, 1 << 30, 1, 0});
+    nslots_ = cfg.p;
+    gmax_rows_ = cfg.n;
   }
-  S_ = pad_width(s);
-  BPF_REQUIRE(S_ > 0, "block width %d unsupported", s);
-  for (int c0 = 0; c0 < cfg.p; c0 += s) h_groups_.push_back({c0, std::min(s, cfg.p - c0)});
   ng_ = (int)h_groups_.size();
   upload(d_groups_, h_groups_, st_);
   const size_t es = cfg.f32 ? sizeof(float) : sizeof(double);
@@ -81,10 +96,17 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   const size_t ss = (size_t)ng_ * S_ * S_ * sizeof(double);
   G_.ensure(ss); R1_.ensure(ss); R1i_.ensure(ss); R2_.ensure(ss); R2i_.ensure(ss);
   E_.ensure(2 * ss);
-  rnorm_.ensure(std::max(1, cfg.p) * sizeof(double));
+  rnorm_.ensure(std::max(1, nslots_) * sizeof(double));
   flags_.ensure(16);
-  ctx->ensure_small(std::max(cfg.p, 16));
+  ctx->ensure_small(std::max(nslots_, 16));
 }
+
+void BlockArnoldi::set_active(int g, bool active, int K) {
+  h_groups_[g].active = active ? 1 : 0;
+  h_groups_[g].K = K;
+}
+
+void BlockArnoldi::upload_groups() { upload(d_groups_, h_groups_, st_); }
 
 BlockArnoldi::~BlockArnoldi() = default;
 
@@ -109,13 +131,13 @@ void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
   DevBuf *Rs[2] = {&R1_, &R2_};
   DevBuf *Ris[2] = {&R1i_, &R2i_};
   for (int pass = 0; pass < 2; ++pass) {
-    ProjArgs pa{nullptr, src, 0, 1, p, src, p, n, dg, ng_, 0, nullptr};
+    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows_, dg, ng_, 0, nullptr};
     launch_proj<TV, TV>(st_, pa, S_, G_.as<double>(), 0, SS, false, part_);
     CholArgs ca{G_.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S_, n, dg,
                 flags_.as<int>()};
     launch_chol(st_, ca, ng_);
     void *dst = pass == 1 ? Vout : W;
-    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S_, dst, p, n, 1.0, 0.0, dg};
+    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S_, dst, p, gmax_rows_, 1.0, 0.0, dg};
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     src = dst;
   }
@@ -135,16 +157,16 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
   const void *const *vt = vtab_.dev<const void>();
   if (cfg_.orth == ORTH_BMGS) {
     for (int i = 0; i <= k; ++i) {
-      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
+      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr};
       launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, 0, gstr, false, part_);
-      GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
+      GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
       launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     }
     return;
   }
-  ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, n, dg, ng_, 0, nullptr};
+  ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr};
   launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
-  GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, n, -1.0, 1.0, dg};
+  GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
   launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
   if (cfg_.orth == ORTH_BCGS2) {
     tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
@@ -168,6 +190,11 @@ void BlockArnoldi::form(double *X, double alpha) {
   const int n = cfg_.n, p = cfg_.p;
   const size_t SS = (size_t)S_ * S_;
   Y_.ensure((size_t)K * ng_ * SS * sizeof(double));
+  // groups still iterating use all K block columns
+  bool changed = false;
+  for (auto &g : h_groups_)
+    if (g.active && g.K != K) { g.K = K; changed = true; }
+  if (changed) upload_groups();
   BacksubArgs ba{coltab_.dev<double>(), K, S_, ng_, Y_.as<double>(), dg};
   launch_backsub(st_, ba, ng_);
   if (scale_) launch_scale_ycols(st_, Y_.as<double>(), K, ng_, S_, dg, scale_);
@@ -175,12 +202,12 @@ void BlockArnoldi::form(double *X, double alpha) {
   const void *const *tab = flex ? ztab_.dev<const void>() : vtab_.dev<const void>();
   if (flex || !N_) {
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
-                S_, X, p, n, alpha, 1.0, dg};
+                S_, X, p, gmax_rows_, alpha, 1.0, dg, 1};
     launch_block_gemm<TV, double>(st_, ga, S_, ng_);
   } else {
     // P = sum_i V_i Y_i (TV), then X += alpha * N^{-1} P.
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
-                S_, P_.ptr, p, n, 1.0, 0.0, dg};
+                S_, P_.ptr, p, gmax_rows_, 1.0, 0.0, dg, 1};
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     TV *P = P_.as<TV>();
     trisolve<TV>(st_, N_->lo, p, P, p, P, p);
@@ -266,13 +293,13 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     launch_hess_apply(st_, ha, ng_);
     HessQrArgs qa{coltab_.dev<double>(), k, S_, ng_, E_.as<double>(), dg, rnorm_.as<double>()};
     launch_hess_qr(st_, qa, ng_);
-    BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)p * sizeof(double), hipMemcpyDeviceToHost, st_));
+    BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)nslots_ * sizeof(double), hipMemcpyDeviceToHost, st_));
     BPF_CHECK_HIP(hipStreamSynchronize(st_));
     ++it;
     k_ = it;
     if (stop(h_rn, it)) break;
   }
-  if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)p * sizeof(double));
+  if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)nslots_ * sizeof(double));
   int fl = 0;
   BPF_CHECK_HIP(hipMemcpy(&fl, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost));
   shifted_ = fl != 0;

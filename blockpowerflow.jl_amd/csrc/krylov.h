@@ -51,6 +51,9 @@ struct ArnoldiConfig {
   bool flexible = false;       // store Z_k = N^{-1} V_k
   int orth = 0;                // ORTH_BMGS / ORTH_BCGS / ORTH_BCGS2
   bool prec32 = false;         // apply the preconditioner in FP32 (FP64 basis)
+  // Explicit groups (row ranges, widths, slots) for many systems stacked in
+  // one block-diagonal operator; empty = column slices of width s.
+  std::vector<GroupDesc> groups;
 };
 enum { ORTH_BMGS = 0, ORTH_BCGS = 1, ORTH_BCGS2 = 2 };
 
@@ -76,6 +79,14 @@ class BlockArnoldi {
   // Polynomial preconditioner on top of N (flexible runs only); null = off.
   void set_poly(Poly *poly) { poly_ = poly; }
   int iterations() const { return k_; }
+  int nslots() const { return nslots_; }
+  int ngroups() const { return ng_; }
+  int width() const { return S_; }
+  const std::vector<GroupDesc> &groups() const { return h_groups_; }
+  // Stops (or resumes) the Krylov recurrence of group g; a stopped group keeps
+  // the first K block columns for its solution.
+  void set_active(int g, bool active, int K);
+  void upload_groups();
   bool shifted() const { return shifted_; }
 
  private:
@@ -92,7 +103,7 @@ class BlockArnoldi {
   bpf_csr A_;
   bpf_prec N_;
   ArnoldiConfig cfg_;
-  int S_ = 0, ng_ = 0;
+  int S_ = 0, ng_ = 0, nslots_ = 0, gmax_rows_ = 0;
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
   hipStream_t st_;

@@ -162,7 +162,7 @@ void Poly::build(bpf_ctx ctx, bpf_csr A, bpf_prec N, int degree) {
   for (auto &v : x) v /= nrm;
   BPF_CHECK_HIP(hipMemcpyAsync(Vp, x.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
   std::vector<double> H((size_t)(d + 1) * d, 0.0);
-  std::vector<GroupDesc> hg{{0, 1}};
+  std::vector<GroupDesc> hg{{0, 1, 0, n, 0, 1 << 30, 1, 0}};
   upload(grp, hg, s);
   hcoef.ensure((size_t)(d + 1) * 16 * 16 * sizeof(double));
   tab.ensure((size_t)(d + 1) * sizeof(void *));

@@ -67,9 +67,9 @@ class _RfBase:
         return P, Q, rp, ci, v
 
     def destroy(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             L.bpf_rf_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
