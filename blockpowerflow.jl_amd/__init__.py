@@ -173,6 +173,7 @@ class HipSparseMatrixCSR:
         self.indptr = indptr
         self.indices = indices
         self._data = data.copy()
+        self._dev_src = None
 
     @classmethod
     def from_scipy(cls, A, ctx=None):
@@ -190,7 +191,21 @@ class HipSparseMatrixCSR:
     @property
     def nzVal(self) -> np.ndarray:
         """Host copy of the values (A.nzVal)."""
+        if self._data is None:       # last set from a device array
+            M, col = self._dev_src
+            self._data = np.ascontiguousarray(M.to_numpy()[:, col])
+            self._dev_src = None
         return self._data.copy()
+
+    def set_values_device(self, M, col: int = 0):
+        """New values from column `col` of the device matrix M (nnz x k,
+        float64): a Newton step's Jacobian values already resident in HBM."""
+        if M.dtype != np.float64 or M.nrows != self.nnz() or not 0 <= col < M.ncols:
+            raise ValueError("values must be column col of an nnz-row float64 device matrix")
+        ptr = C.c_void_p(M.ptr.value + col * M.nrows * 8)
+        check(L.bpf_csr_set_values_device(self.handle, ptr))
+        self._data = None
+        self._dev_src = (M, col)
 
     def set_values(self, data):
         data = np.ascontiguousarray(data, dtype=np.float64)
@@ -198,6 +213,7 @@ class HipSparseMatrixCSR:
             raise ValueError("values must have nnz entries")
         check(L.bpf_csr_set_values(self.handle, data.ctypes.data_as(_lib.dp)))
         self._data = data.copy()
+        self._dev_src = None
 
     def norm_inf(self) -> float:
         v = C.c_double()

@@ -33,6 +33,10 @@ struct ProjArgs {
   int ng;
   int chunk_rows;      // set by the launcher
   double *part;        // set by the launcher
+  // Groups stacked by rows (disjoint row ranges): the sum of their row
+  // counts, which sizes the chunks; empty chunks are skipped.  0 = column
+  // groups over all n rows.
+  int rows_sum = 0;
 };
 
 // Y[:, g] = beta*Y[:, g] + alpha * sum_{i<nb} V_{i0+i}[:, g] * Psi_{i,g}
@@ -51,6 +55,7 @@ struct GemmArgs {
   // 0: inactive groups are skipped.  1 (solution assembly): a group takes
   // part iff K > 0, over its first min(nb, K) blocks.
   int use_k = 0;
+  int rows_sum = 0;    // stacked groups: sum of heights (traffic accounting)
 };
 
 struct CholArgs {

@@ -95,6 +95,8 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
   const TW *W = static_cast<const TW *>(a.W) + gd.c0;
   const int r0 = gd.r0 + chunk * a.chunk_rows;
   const int r1 = gd.active ? min(gd.r0 + gd.nr, r0 + a.chunk_rows) : r0;
+  // Stacked groups: the reduction reads only the chunks that hold rows.
+  if (a.rows_sum > 0 && r0 >= r1) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ks = wave % Sh::KS;                    // row split index
   const int wg = wave / Sh::KS;                    // tile-block index
@@ -187,14 +189,22 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
 __global__ __launch_bounds__(kB) void proj_reduce_kernel(const double *part, int nchunk, int nb,
                                                          int ng, int SS, double *out,
                                                          size_t o_istride, size_t o_gstride,
-                                                         int accumulate) {
+                                                         int accumulate, const GroupDesc *groups,
+                                                         int stacked_rows) {
   __shared__ double red[4][64];
   const size_t total = (size_t)nb * ng * SS;
   const size_t e = (size_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int w = threadIdx.x >> 6;
   double s = 0.0;
-  if (e < total)
-    for (int c = w; c < nchunk; c += 4) s += part[(size_t)c * total + e];
+  if (e < total) {
+    int nc = nchunk;
+    if (stacked_rows > 0) {
+      // chunk rows for stacked groups; chunks past the group's rows are unwritten
+      const GroupDesc gd = groups[(e / SS) % ng];
+      nc = gd.active ? min(nchunk, (gd.nr + stacked_rows - 1) / stacked_rows) : 0;
+    }
+    for (int c = w; c < nc; c += 4) s += part[(size_t)c * total + e];
+  }
   red[w][threadIdx.x & 63] = s;
   __syncthreads();
   if (w == 0 && e < total) {
@@ -877,12 +887,21 @@ template <typename TVV, typename TW>
 void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride, size_t o_gstride,
                  bool accumulate, DevBuf &part) {
   if (a.nb <= 0 || a.ng <= 0) return;
-  // Chunk rows so that the grid has ~512 workgroups (two per CU) while the
-  // partial sums (nchunk * nb * ng * S^2 doubles) stay small.
-  int target = std::max(1, 512 / std::max(1, a.nb * a.ng));
-  int nchunk = std::max(1, std::min(target, (a.n + 255) / 256));
-  a.chunk_rows = (a.n + nchunk - 1) / nchunk;
-  a.chunk_rows = ((a.chunk_rows + 15) / 16) * 16;
+  int nchunk;
+  if (a.rows_sum > 0) {
+    // Stacked groups of uneven height: ~1024 non-empty workgroups over all
+    // groups' rows (a.n is the tallest group; shorter ones leave chunks empty).
+    const long target = std::max(64, 1024 / std::max(1, a.nb));
+    int cr = (int)std::max<long>(64, (a.rows_sum + target - 1) / target);
+    a.chunk_rows = ((cr + 15) / 16) * 16;
+  } else {
+    // Chunk rows so that the grid has ~512 workgroups (two per CU) while the
+    // partial sums (nchunk * nb * ng * S^2 doubles) stay small.
+    int target = std::max(1, 512 / std::max(1, a.nb * a.ng));
+    nchunk = std::max(1, std::min(target, (a.n + 255) / 256));
+    a.chunk_rows = (a.n + nchunk - 1) / nchunk;
+    a.chunk_rows = ((a.chunk_rows + 15) / 16) * 16;
+  }
   nchunk = std::max(1, (a.n + a.chunk_rows - 1) / a.chunk_rows);
   part.ensure((size_t)nchunk * a.nb * a.ng * S * S * sizeof(double));
   a.part = part.as<double>();
@@ -898,13 +917,13 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   const size_t total = (size_t)a.nb * a.ng * S * S;
   hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(kB), 0, s,
                      a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
-                     accumulate ? 1 : 0);
+                     accumulate ? 1 : 0, a.groups, a.rows_sum > 0 ? a.chunk_rows : 0);
   BPF_CHECK_HIP(hipGetLastError());
   // Algorithmic traffic: each basis block and W read once; the s x s
   // products are 2 n w^2 flops per (block, group).
-  const double cols = (double)a.ng * S;   // upper bound on sum of group widths
-  prof::end(tok, s, prof::PROJ, (double)a.n * cols * ((a.nb) * sizeof(TVV) + sizeof(TW)),
-            2.0 * a.n * (double)S * S * a.nb * a.ng);
+  // Stacked groups: rows_sum rows of width S; column groups: n rows of ng*S.
+  const double cells = a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * a.ng * S;
+  prof::end(tok, s, prof::PROJ, cells * ((a.nb) * sizeof(TVV) + sizeof(TW)), 2.0 * cells * S * a.nb);
 }
 template void launch_proj<double, double>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
 template void launch_proj<float, float>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
@@ -927,10 +946,10 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   BPF_CHECK_HIP(hipGetLastError());
-  const double cols = (double)ng * S;
+  const double cells = a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * ng * S;
   prof::end(tok, s, prof::GEMM,
-            (double)a.n * cols * (a.nb * sizeof(TVV) + (a.beta == 0.0 ? 1.0 : 2.0) * sizeof(TY)),
-            2.0 * a.n * (double)S * S * a.nb * ng);
+            cells * (a.nb * sizeof(TVV) + (a.beta == 0.0 ? 1.0 : 2.0) * sizeof(TY)),
+            2.0 * cells * S * a.nb);
 }
 template void launch_block_gemm<double, double>(hipStream_t, GemmArgs, int, int);
 template void launch_block_gemm<float, float>(hipStream_t, GemmArgs, int, int);

@@ -68,6 +68,7 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
       wmax = std::max(wmax, g.w);
       nslots_ = std::max(nslots_, g.slot + g.w);
       gmax_rows_ = std::max(gmax_rows_, g.nr);
+      rows_sum_ += g.nr;
     }
     S_ = pad_width(wmax);
     BPF_REQUIRE(S_ > 0, "block width %d unsupported", wmax);
@@ -131,13 +132,14 @@ void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
   DevBuf *Rs[2] = {&R1_, &R2_};
   DevBuf *Ris[2] = {&R1i_, &R2i_};
   for (int pass = 0; pass < 2; ++pass) {
-    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows_, dg, ng_, 0, nullptr};
+    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
     launch_proj<TV, TV>(st_, pa, S_, G_.as<double>(), 0, SS, false, part_);
     CholArgs ca{G_.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S_, n, dg,
                 flags_.as<int>()};
     launch_chol(st_, ca, ng_);
     void *dst = pass == 1 ? Vout : W;
     GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S_, dst, p, gmax_rows_, 1.0, 0.0, dg};
+    ga.rows_sum = rows_sum_;
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     src = dst;
   }
@@ -157,16 +159,18 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
   const void *const *vt = vtab_.dev<const void>();
   if (cfg_.orth == ORTH_BMGS) {
     for (int i = 0; i <= k; ++i) {
-      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr};
+      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
       launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, 0, gstr, false, part_);
       GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
+      ga.rows_sum = rows_sum_;
       launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     }
     return;
   }
-  ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr};
+  ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
   launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
   GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
+  ga.rows_sum = rows_sum_;
   launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
   if (cfg_.orth == ORTH_BCGS2) {
     tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
@@ -174,6 +178,7 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
     launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
     GemmArgs gb = ga;
     gb.psi = t;
+    gb.rows_sum = rows_sum_;
     launch_block_gemm<TV, TV>(st_, gb, S_, ng_);
     for (int g = 0; g < ng_; ++g)
       axpby(st_, (size_t)(k + 1) * SS, 1.0, t + g * gstr, 1.0, col + g * gstr);
@@ -203,11 +208,13 @@ void BlockArnoldi::form(double *X, double alpha) {
   if (flex || !N_) {
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
                 S_, X, p, gmax_rows_, alpha, 1.0, dg, 1};
+    ga.rows_sum = rows_sum_;
     launch_block_gemm<TV, double>(st_, ga, S_, ng_);
   } else {
     // P = sum_i V_i Y_i (TV), then X += alpha * N^{-1} P.
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
                 S_, P_.ptr, p, gmax_rows_, 1.0, 0.0, dg, 1};
+    ga.rows_sum = rows_sum_;
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     TV *P = P_.as<TV>();
     trisolve<TV>(st_, N_->lo, p, P, p, P, p);

@@ -104,6 +104,7 @@ class BlockArnoldi {
   bpf_prec N_;
   ArnoldiConfig cfg_;
   int S_ = 0, ng_ = 0, nslots_ = 0, gmax_rows_ = 0;
+  int rows_sum_ = 0;           // > 0: groups stacked by rows (sum of heights)
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
   hipStream_t st_;

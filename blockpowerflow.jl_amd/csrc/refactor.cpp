@@ -44,7 +44,7 @@ struct bpf_rf_s {
   TriOp lo, up;
   DevBuf facb;                         // batch: nnzF x batch factor values
   DevBuf w1, w2, w3;                   // work blocks
-  DevBuf status;
+  DevBuf status, stat;
 };
 
 namespace {
@@ -223,8 +223,7 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
     }
     if (h_berr) {
       residual_rm(s, A, p, B, X, R);
-      DevBuf stat;
-      backward_error_rm(rf->ctx, A, p, R, X, B, h_berr, stat);
+      backward_error_rm(rf->ctx, A, p, R, X, B, h_berr, rf->stat);
     }
     launch_rm_to_cm(s, n, p, X, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));

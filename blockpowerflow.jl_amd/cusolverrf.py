@@ -92,11 +92,14 @@ class CusolverRfLU(_RfBase):
         check(L.bpf_rf_create(self.ctx.handle, A.handle, _ordering(ordering), float(tol), C.byref(h)))
         self.handle = h
 
-    def solve_refined(self, A, B, refine=2):
+    def solve_refined(self, A, B, refine=2, X=None):
         """FP32 triangular solves + `refine` FP64 refinement steps; returns
-        (X, per-RHS backward error)."""
+        (X, per-RHS backward error).  X (optional) receives the solution."""
         from . import HipMatrix
-        X = HipMatrix(B.shape, np.float64, B.ctx)
+        if X is None:
+            X = HipMatrix(B.shape, np.float64, B.ctx)
+        elif X.shape != B.shape or X.dtype != np.float64:
+            raise ValueError("X must be a float64 matrix of B's shape")
         berr = np.zeros(B.ncols)
         check(L.bpf_rf_solve_refined(self.handle, A.handle, B.ncols, B.ptr, B.nrows, X.ptr, X.nrows,
                                      int(refine), berr.ctypes.data_as(_lib.dp)))
