@@ -190,12 +190,12 @@ int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
                        const double *d_B, int ldb, const double *d_C, int ldc,
                        double *d_X, int ldx, const bpf_solver_params *prm,
                        bpf_solver_stats *stats, double *h_history, int hist_cap);
-/* Per-RHS FP64 backward error ||Ax-b||inf/(||A||inf||x||inf+||b||inf) into
- * h_berr[p] (no reference counterpart: the reference prints norm(B - A*X),
- * benchmarks/benchmarks_amd.jl:64). */
 /* Eigenvalues of a d x d upper Hessenberg matrix (row-major), host-only
  * utility behind the polynomial preconditioner's harmonic Ritz values. */
 int bpf_hessenberg_eigenvalues(const double *h_H, int d, double *h_re, double *h_im);
+/* Per-RHS FP64 backward error ||Ax-b||inf/(||A||inf||x||inf+||b||inf) into
+ * h_berr[p] (no reference counterpart: the reference prints norm(B - A*X),
+ * benchmarks/benchmarks_amd.jl:64). */
 int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx,
                        const double *d_B, int ldb, double *h_berr);
 
