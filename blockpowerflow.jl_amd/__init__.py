@@ -33,7 +33,7 @@ __all__ = [
     "ilu0", "ic0", "ldiv_", "mul_", "block_gmres", "block_gmres_", "BlockGMRESSolver",
     "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
     "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
-    "block_gmres_batched",
+    "block_gmres_batched", "householder", "block_hessenberg_lsq",
 ]
 
 L = _lib.load()
@@ -529,6 +529,42 @@ def backward_error(A: HipSparseMatrixCSR, X: HipMatrix, B: HipMatrix) -> np.ndar
     check(L.bpf_backward_error(A.ctx.handle, A.handle, B.ncols, X.ptr, X.nrows, B.ptr, B.nrows,
                                out.ctypes.data_as(_lib.dp)))
     return out
+
+
+def householder(A: HipMatrix):
+    """(Q, R) = householder(A) (src/block_utils.jl:38-53): thin QR of a
+    device n x p block by CholQR2; R upper triangular with positive diagonal.
+    Returns new device matrices Q (n x p) and R (p x p)."""
+    n, p = A.shape
+    if A.dtype != np.float64:
+        raise ValueError("householder needs a float64 block")
+    Q = HipMatrix((n, p), np.float64, A.ctx)
+    check(L.bpf_memcpy_d2d(A.ctx.handle, Q.ptr, A.ptr, A.nbytes))
+    R = HipMatrix((p, p), np.float64, A.ctx)
+    flag = C.c_int()
+    check(L.bpf_householder(A.ctx.handle, n, p, Q.ptr, n, R.ptr, p, C.byref(flag)))
+    Q.shifted = bool(flag.value)
+    return Q, R
+
+
+def block_hessenberg_lsq(H: np.ndarray, G: np.ndarray, ctx: Context | None = None):
+    """Block-Hessenberg least squares as block GMRES runs it on the device
+    (src/block_gmres.jl:318-375): H is (K+1)p x Kp, G is p x p.  Returns
+    (Y, rnorm) with Y (Kp x p) minimising ||[G; 0] - H Y|| per column and
+    rnorm[k] the per-column residual norms after block column k."""
+    ctx = ctx or default_context()
+    G = np.asfortranarray(G, dtype=np.float64)
+    H = np.asfortranarray(H, dtype=np.float64)
+    p = G.shape[0]
+    K = H.shape[1] // p
+    if G.shape != (p, p) or H.shape != ((K + 1) * p, K * p):
+        raise ValueError("H must be (K+1)p x Kp and G p x p")
+    Y = np.zeros((K * p, p), order="F")
+    rn = np.zeros((K, p))
+    check(L.bpf_block_hessenberg_lsq(ctx.handle, p, K, H.ctypes.data_as(_lib.dp), H.shape[0],
+                                     G.ctypes.data_as(_lib.dp), p, Y.ctypes.data_as(_lib.dp), K * p,
+                                     rn.ctypes.data_as(_lib.dp)))
+    return Y, rn
 
 
 from .cusolverrf import CUSOLVERRF  # noqa: E402

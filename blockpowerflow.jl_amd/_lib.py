@@ -90,6 +90,8 @@ SIGNATURES = {
     "bpf_block_bicgstab": [vp, vp, vp, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_backward_error": [vp, vp, i, vp, i, vp, i, dp],
     "bpf_hessenberg_eigenvalues": [dp, i, dp, dp],
+    "bpf_householder": [vp, i, i, vp, i, vp, i, ip],
+    "bpf_block_hessenberg_lsq": [vp, i, i, dp, i, dp, i, dp, i, dp],
     "bpf_batch_create": [vp, i, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), ip, C.POINTER(vp)],
     "bpf_batch_solve": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolverParams), C.POINTER(SolverStats), ip],
     "bpf_batch_destroy": [vp],

@@ -95,7 +95,7 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   W_.ensure(blk * es);
   P_.ensure(blk * es);
   const size_t ss = (size_t)ng_ * S_ * S_ * sizeof(double);
-  G_.ensure(ss); R1_.ensure(ss); R1i_.ensure(ss); R2_.ensure(ss); R2i_.ensure(ss);
+  cq_.init(ng_, S_);
   E_.ensure(2 * ss);
   rnorm_.ensure(std::max(1, nslots_) * sizeof(double));
   flags_.ensure(16);
@@ -122,28 +122,43 @@ double *BlockArnoldi::col_region(int k) {
   return cols_[k].as<double>();
 }
 
+void CholQrWork::init(int ng, int S) {
+  const size_t ss = (size_t)ng * S * S * sizeof(double);
+  G.ensure(ss); R1.ensure(ss); R1i.ensure(ss); R2.ensure(ss); R2i.ensure(ss);
+  flags.ensure(16);
+}
+
 template <typename TV>
-void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
+void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
+                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w) {
   // Two CholQR passes; C = R2 R1 (upper), Vout = W R1^{-1} R2^{-1}.
-  const GroupDesc *dg = d_groups_.as<GroupDesc>();
-  const int n = cfg_.n, p = cfg_.p;
-  const size_t SS = (size_t)S_ * S_;
+  const size_t SS = (size_t)S * S;
   void *src = W;
-  DevBuf *Rs[2] = {&R1_, &R2_};
-  DevBuf *Ris[2] = {&R1i_, &R2i_};
+  DevBuf *Rs[2] = {&w.R1, &w.R2};
+  DevBuf *Ris[2] = {&w.R1i, &w.R2i};
   for (int pass = 0; pass < 2; ++pass) {
-    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
-    launch_proj<TV, TV>(st_, pa, S_, G_.as<double>(), 0, SS, false, part_);
-    CholArgs ca{G_.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S_, n, dg,
-                flags_.as<int>()};
-    launch_chol(st_, ca, ng_);
+    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows, dg, ng, 0, nullptr, rows_sum};
+    launch_proj<TV, TV>(s, pa, S, w.G.as<double>(), 0, SS, false, w.part);
+    CholArgs ca{w.G.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S, n_shift, dg,
+                w.flags.as<int>()};
+    launch_chol(s, ca, ng);
     void *dst = pass == 1 ? Vout : W;
-    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S_, dst, p, gmax_rows_, 1.0, 0.0, dg};
-    ga.rows_sum = rows_sum_;
-    launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, dg};
+    ga.rows_sum = rows_sum;
+    launch_block_gemm<TV, TV>(s, ga, S, ng);
     src = dst;
   }
-  launch_trmm_upper(st_, R2_.as<double>(), R1_.as<double>(), Cout, S_, SS, cstride, dg, ng_);
+  launch_trmm_upper(s, w.R2.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
+}
+template void cholqr2_groups<double>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
+                                     void *, double *, size_t, CholQrWork &);
+template void cholqr2_groups<float>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
+                                    void *, double *, size_t, CholQrWork &);
+
+template <typename TV>
+void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
+  cholqr2_groups<TV>(st_, d_groups_.as<GroupDesc>(), ng_, S_, cfg_.p, gmax_rows_, rows_sum_, cfg_.n, W, Vout,
+                     Cout, cstride, cq_);
 }
 
 // Psi_{i,k} = V_i^T W and W -= V_i Psi_{i,k}, i = 0..k, into block column k.

@@ -43,6 +43,21 @@ struct PtrTable {
   template <typename T> T *const *dev() const { return d.as<T *>(); }
 };
 
+// Buffers of a grouped CholQR2 (per group S x S: Gram matrix, two R factors
+// and their inverses; the shift flag; projection partials).
+struct CholQrWork {
+  DevBuf G, R1, R1i, R2, R2i, flags, part;
+  void init(int ng, int S);
+};
+
+// Grouped CholQR2 of W (row-major, ld p): Vout = W R1^{-1} R2^{-1} with
+// C = R2 R1 (upper, per group at Cout + g*cstride).  W is overwritten by the
+// first pass.  n_shift is the row count used in the shift of a failed pivot;
+// w.flags bit 0 is set when a shift was needed (cleared by the caller).
+template <typename TV>
+void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
+                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w);
+
 // Host-visible description of one Krylov run.
 struct ArnoldiConfig {
   int n = 0, p = 0;            // rows, columns of the RHS block
@@ -116,12 +131,12 @@ class BlockArnoldi {
   std::vector<DevBuf> cols_;
   PtrTable coltab_, vtab_, ztab_;
   DevBuf tmpcol_;              // second-pass coefficients
-  DevBuf G_, R1_, R1i_, R2_, R2i_;
+  CholQrWork cq_;
   DevBuf E_;                   // current [Z_k; Lambda] per group
   DevBuf rnorm_;
   DevBuf Y_;                   // back substitution output
-  DevBuf part_;                // projection partial sums
-  DevBuf flags_;
+  DevBuf &part_ = cq_.part;    // projection partial sums
+  DevBuf &flags_ = cq_.flags;
   DevBuf Xtmp_;                // FP64 n x p scratch (non-flexible solution assembly)
   DevBuf Vf_, Zf_;             // FP32 copies for an FP32 preconditioner
 };

@@ -190,6 +190,24 @@ int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
                        const double *d_B, int ldb, const double *d_C, int ldc,
                        double *d_X, int ldx, const bpf_solver_params *prm,
                        bpf_solver_stats *stats, double *h_history, int hist_cap);
+/* Thin QR of a device n x p block (n >= p, p <= 128), in place:
+ * householder!(Q, R, tau) (src/block_utils.jl:38-53), the orthonormalization
+ * of block_gmres (src/block_gmres.jl:276,316).  Computed by CholQR2 (two
+ * Cholesky-QR passes, FP64 Gram matrices on matrix cores); R is upper
+ * triangular with a positive diagonal (geqrf's R up to row signs).  *shifted
+ * (optional) reports that a failed Cholesky pivot was repaired by a shift. */
+int bpf_householder(bpf_ctx ctx, int n, int p, double *d_Q, int ldq, double *d_R, int ldr,
+                    int *shifted);
+/* Block-Hessenberg least squares of block GMRES, column by column on the
+ * device exactly as inside the solver: the QR update of each new block
+ * column with the stored block reflectors (src/block_gmres.jl:318-337), the
+ * residual estimate per RHS after each column (:341-342) and the final block
+ * back substitution (:367-375).  h_H: (K+1)p x Kp block upper Hessenberg,
+ * h_G: p x p right-hand side [G; 0; ...], column-major host arrays.  Output
+ * h_Y (Kp x p) minimises ||[G; 0] - H Y||_F column-wise; h_rnorm (K x p,
+ * row k = after column k, optional) holds the residual norms. */
+int bpf_block_hessenberg_lsq(bpf_ctx ctx, int p, int K, const double *h_H, int ldh,
+                             const double *h_G, int ldg, double *h_Y, int ldy, double *h_rnorm);
 /* Eigenvalues of a d x d upper Hessenberg matrix (row-major), host-only
  * utility behind the polynomial preconditioner's harmonic Ritz values. */
 int bpf_hessenberg_eigenvalues(const double *h_H, int d, double *h_re, double *h_im);
