@@ -10,7 +10,8 @@
 //   block_gemm                  Y = beta*Y + alpha*sum_i V_i Psi_i  (NN GEMM, FP64 accumulate)
 //   chol_kernel                 R = chol(G) (upper), R^{-1}, shift on breakdown
 //   hess_apply_kernel           apply the stored block reflectors to a new block column
-//   hess_qr_kernel              Householder QR of [Psi_kk; C] and rotation of [Z_k; 0]
+//   hess_qr_kernel              Householder QR of [Psi_kk; C], compact-WY factor T
+//   hess_rot_kernel             rotation of [Z_k; 0], residual estimates
 //   backsub_kernel              block upper-triangular solve R Y = Z
 //   colstats kernels            per-column inf-norms / 2-norms, backward error
 #include "dense.h"
@@ -462,116 +463,176 @@ __device__ __forceinline__ double yref(const double *H, int S, int r, int j) {
   return r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
 }
 
-// Column b of group g: apply Omega_i^T = I - Y_i T_i^T Y_i^T, i = 0..k-1, to
-// rows [iS, iS + 2S) of the new block column [Psi_0; ...; Psi_k] (compact WY
-// form: three matrix-vector products per block instead of S reflections).
-// The column is staged in LDS when (k+1)*S doubles fit, otherwise it is
-// updated in place in global memory (stride S).
-struct ColView {
-  double *base;
-  int S;
-  bool lds;
-  __device__ __forceinline__ double &operator[](int e) const {
-    return lds ? base[e] : base[(size_t)(e / S) * S * S + (size_t)(e % S) * S];
-  }
-};
-
-__global__ __launch_bounds__(kB) void hess_apply_kernel(HessArgs a, int use_lds) {
-  extern __shared__ double ys[];      // (k+1)*S entries of column b
-  __shared__ double m1[128], m2[128];
-  const int b = blockIdx.x, g = blockIdx.y;
-  const int S = a.S, k = a.k;
-  const int w = a.groups[g].w;
-  if (b >= w || !a.groups[g].active) return;
-  double *hcol = hblock(a.cols, k, g, 0, S);
-  if (use_lds) {
-    for (int e = threadIdx.x; e < (k + 1) * S; e += kB) ys[e] = hcol[(size_t)(e / S) * S * S + (e % S) * S + b];
-  }
-  __syncthreads();
-  const ColView yv{use_lds ? ys : hcol + b, S, use_lds != 0};
-  for (int i = 0; i < k; ++i) {
-    const double *H = hblock(a.cols, i, g, i, S);   // 2S x S row-major
-    const double *T = htmat(a.cols, i, g, a.ng, S);
-    const int off = i * S;
-    // m1 = Y^T seg
-    for (int j = threadIdx.x; j < w; j += kB) {
-      double acc = yv[off + j];
-      for (int r = j + 1; r < 2 * S; ++r) acc += H[(size_t)r * S + j] * yv[off + r];
-      m1[j] = acc;
-    }
-    __syncthreads();
-    // m2 = T^T m1 (T upper triangular)
-    for (int j = threadIdx.x; j < w; j += kB) {
-      double acc = 0.0;
-      for (int q = 0; q <= j; ++q) acc += T[(size_t)q * S + j] * m1[q];
-      m2[j] = acc;
-    }
-    __syncthreads();
-    // seg -= Y m2
-    for (int r = threadIdx.x; r < 2 * S; r += kB) {
-      const int jmax = min(r, w - 1);
-      double acc = 0.0;
-      for (int j = 0; j <= jmax; ++j) acc += yref(H, S, r, j) * m2[j];
-      yv[off + r] -= acc;
-    }
-    __syncthreads();
-  }
-  if (use_lds)
-    for (int e = threadIdx.x; e < (k + 1) * S; e += kB) hcol[(size_t)(e / S) * S * S + (e % S) * S + b] = ys[e];
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
+  // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
+  // B[l/16][l%16] and holds D[l/16 + 4q][l%16] in acc[q].
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
 }
 
-// One workgroup per group.  D = [Psi_kk; C] (2S x S, row-major, blocks k and
-// k+1 of block column k) is factored in place by blocked Householder QR
-// (LAPACK geqrf convention: reflector j is v = e_j + D[j+1.., j] below the
-// diagonal, H_j = I - tau_j v v^T).  Panels of 16 columns are factored in
-// LDS; each panel's block reflector (I - Yp Tp Yp^T) is then applied to the
-// trailing columns in global memory.  Afterwards the triangular factor T of
-// Omega_k = H_0 ... H_{w-1} = I - Y T Y^T is formed from T^{-1} =
-// striu(Y^T Y) + diag(1/tau) (Joffrain et al. 2006) and stored, and
-// E = [Z_k; 0] is rotated: E <- Omega_k^T E.  R_kk = upper(D[0:S]); on exit
-// zfin(k) = E[0:S] and E holds [Lambda; 0] for the next column.
-constexpr int kQrPanel = 16;
+// A 2S x 16 column tile `seg` (LDS, rows padded to 17) is multiplied by
+// Omega^T = I - Y T^T Y^T, with Y the unit lower-trapezoidal compact factor in
+// H (2S x S row-major) and T upper triangular (S x S row-major): m1 = Y^T seg,
+// m1 = T^T m1, seg -= Y m1, all on matrix cores by the workgroup's 4
+// wavefronts.  Only rows r < rmax of seg may be non-zero on entry.  Ends on a
+// barrier.
+template <int S>
+__device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, const double *__restrict__ T,
+                                              double (*seg)[17], double (*m1)[17], int rmax) {
+  constexpr int NT = S / 16, NU = (NT + 3) / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 acc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int jt = wave + 4 * u;
+    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+    if (jt < NT) {
+      const int j = jt * 16 + lc;
+#pragma unroll 4
+      for (int r0 = jt * 16; r0 < rmax; r0 += 4) {
+        const int r = r0 + lr;
+        const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
+        a4 = mfma4(av, seg[r][lc], a4);
+      }
+    }
+    acc[u] = a4;
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int jt = wave + 4 * u;
+    if (jt < NT)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) m1[jt * 16 + lr + 4 * q][lc] = acc[u][q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int jt = wave + 4 * u;
+    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+    if (jt < NT) {
+      const int j = jt * 16 + lc;
+#pragma unroll 4
+      for (int q0 = 0; q0 < jt * 16 + 16; q0 += 4) {
+        const int q = q0 + lr;
+        const double av = q <= j ? T[(size_t)q * S + j] : 0.0;
+        a4 = mfma4(av, m1[q][lc], a4);
+      }
+    }
+    acc[u] = a4;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int jt = wave + 4 * u;
+    if (jt < NT)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) m1[jt * 16 + lr + 4 * q][lc] = acc[u][q];
+  }
+  __syncthreads();
+  for (int rt = wave; rt < 2 * S / 16; rt += 4) {
+    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+    const int r = rt * 16 + lc;
+    const int jend = min(S, rt * 16 + 16);
+#pragma unroll 4
+    for (int j0 = 0; j0 < jend; j0 += 4) {
+      const int j = j0 + lr;
+      const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
+      a4 = mfma4(av, m1[j][lc], a4);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) seg[rt * 16 + lr + 4 * q][lc] -= a4[q];
+  }
+  __syncthreads();
+}
 
+// Grid (S/16 column tiles, groups).  A workgroup owns 16 columns of the new
+// block column [Psi_0; ...; Psi_{k+1}] and applies Omega_i^T, i = 0..k-1, to
+// rows [iS, iS + 2S): a two-block window slides down the column in LDS.
+template <int S>
+__global__ __launch_bounds__(256) void hess_apply_kernel(HessArgs a) {
+  __shared__ double seg[2 * S][17];
+  __shared__ double m1[S][17];
+  const int b0 = blockIdx.x * 16, g = blockIdx.y;
+  const GroupDesc gd = a.groups[g];
+  const int k = a.k;
+  if (b0 >= gd.w || !gd.active || k <= 0) return;
+  const int tid = threadIdx.x;
+  auto xfer = [&](int i, int half, bool load) {
+    double *blk = hblock(a.cols, k, g, i, S);
+    for (int e = tid; e < S * 16; e += 256) {
+      const int r = e >> 4, c = e & 15;
+      double *gp = blk + (size_t)r * S + b0 + c;
+      if (load) seg[half * S + r][c] = *gp;
+      else *gp = seg[half * S + r][c];
+    }
+  };
+  xfer(0, 0, true);
+  xfer(1, 1, true);
+  __syncthreads();
+  for (int i = 0; i < k; ++i) {
+    wy_apply_tile<S>(hblock(a.cols, i, g, i, S), htmat(a.cols, i, g, a.ng, S), seg, m1, 2 * S);
+    if (i + 1 < k) {
+      // block i is final; slide the window down by one block
+      xfer(i, 0, false);
+      for (int e = tid; e < S * 16; e += 256) seg[e >> 4][e & 15] = seg[S + (e >> 4)][e & 15];
+      __syncthreads();
+      xfer(i + 2, 1, true);
+      __syncthreads();
+    }
+  }
+  xfer(k - 1, 0, false);
+  xfer(k, 1, false);
+}
+
+// One workgroup (16 wavefronts) per group: D = [Psi_kk; C] (2S x S, blocks
+// k and k+1 of block column k) is factored in place by blocked Householder
+// QR (LAPACK geqrf convention: reflector j is v = e_j + D[j+1.., j], H_j =
+// I - tau_j v v^T).  Each 16-column panel is factored in LDS; its block
+// reflector updates the trailing columns on matrix cores.  The triangular
+// factor T of Omega_k = H_0 ... H_{w-1} = I - Y T Y^T is accumulated panel by
+// panel (T12 = -T11 (Y1^T Y2) T22, Schreiber & Van Loan 1989) into htmat(k);
+// columns and rows of T past w stay zero.
+template <int S>
 __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
+  constexpr int S2 = 2 * S, PB = 16;
   const int g = blockIdx.x;
-  if (!a.groups[g].active) return;
-  const int S = a.S;
-  const int w = a.groups[g].w;
-  const int S2 = 2 * S;
+  const GroupDesc gd = a.groups[g];
+  if (!gd.active) return;
+  const int w = gd.w;
   double *D = hblock(a.cols, a.k, g, a.k, S);
-  double *E = a.E + (size_t)g * 2 * S * S;
   double *tau = htau(a.cols, a.k, g, a.ng, S);
   double *Tg = htmat(a.cols, a.k, g, a.ng, S);
-  // LDS: phase 1 panel (2S x 17) + panel T (16 x 16); phase 2 T^{-1} (S x S).
-  __shared__ double lds[128 * 128];
+  __shared__ double pan[S2][PB + 1];       // panel columns j0..j0+15, rows j0..2S-1
+  __shared__ double Tp[PB][PB + 1];        // panel T (upper)
+  __shared__ double scr[16][16][17];       // per-wavefront 16 x 16 scratch
+  __shared__ double zs[S][PB + 1];         // T-merge scratch (j0 x 16)
   __shared__ double red[16];
   __shared__ double s_beta, s_tau, s_scale;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-  constexpr int PB = kQrPanel, PL = PB + 1;
-  double *pan = lds;                       // pan[r * PL + c]
-  double *Tp = lds + 256 * PL;             // Tp[q * PB + j]
-  double *W = Tp + PB * PB;                // W[c * PB + q]: panel products with trailing columns
+  const int tid = threadIdx.x, nt = 1024;
+  const int lane = tid & 63, wid = tid >> 6, nw = 16;
+  const int lr = lane >> 4, lc = lane & 15;
+  for (int e = tid; e < S * S; e += nt) Tg[e] = 0.0;
+  for (int j = tid; j < S; j += nt) tau[j] = 0.0;
+  __syncthreads();
   for (int j0 = 0; j0 < w; j0 += PB) {
     const int pw = min(PB, w - j0);
-    // load panel columns j0..j0+pw-1, rows j0..2S-1
     for (int e = tid; e < S2 * PB; e += nt) {
       const int r = e / PB, c = e % PB;
-      pan[r * PL + c] = (c < pw && r >= j0) ? D[(size_t)r * S + j0 + c] : 0.0;
+      pan[r][c] = (c < pw && r >= j0) ? D[(size_t)r * S + j0 + c] : 0.0;
     }
     __syncthreads();
-    // unblocked Householder on the panel (in LDS)
+    // unblocked Householder on the panel
     for (int jj = 0; jj < pw; ++jj) {
       const int j = j0 + jj;
       double part = 0.0;
-      for (int r = j + 1 + tid; r < S2; r += nt) part += pan[r * PL + jj] * pan[r * PL + jj];
+      for (int r = j + 1 + tid; r < S2; r += nt) part += pan[r][jj] * pan[r][jj];
       for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
       if (lane == 0) red[wid] = part;
       __syncthreads();
       if (tid == 0) {
         double sig = 0.0;
         for (int q = 0; q < nw; ++q) sig += red[q];
-        const double alpha = pan[j * PL + jj];
+        const double alpha = pan[j][jj];
         if (sig == 0.0) {
           s_tau = 0.0; s_beta = alpha; s_scale = 0.0;
         } else {
@@ -584,149 +645,172 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
         tau[j] = s_tau;
       }
       __syncthreads();
-      for (int r = j + 1 + tid; r < S2; r += nt) pan[r * PL + jj] *= s_scale;
+      for (int r = j + 1 + tid; r < S2; r += nt) pan[r][jj] *= s_scale;
       __syncthreads();
-      if (tid == 0) pan[j * PL + jj] = s_beta;
+      if (tid == 0) pan[j][jj] = s_beta;
       const double t = s_tau;
-      // apply to the remaining panel columns (one wave per column)
       for (int c = jj + 1 + wid; c < pw; c += nw) {
         double dot = 0.0;
-        for (int r = j + lane; r < S2; r += 64) dot += (r == j ? 1.0 : pan[r * PL + jj]) * pan[r * PL + c];
+        for (int r = j + lane; r < S2; r += 64) dot += (r == j ? 1.0 : pan[r][jj]) * pan[r][c];
         for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
         const double f = t * dot;
-        for (int r = j + lane; r < S2; r += 64) pan[r * PL + c] -= f * (r == j ? 1.0 : pan[r * PL + jj]);
+        for (int r = j + lane; r < S2; r += 64) pan[r][c] -= f * (r == j ? 1.0 : pan[r][jj]);
       }
       __syncthreads();
     }
     // panel T (pw x pw upper): T[jj][jj] = tau, T[0:jj, jj] = -tau T[0:jj,0:jj] (Yp^T y_jj)
+    double *zv = &scr[0][0][0];
+    for (int e = tid; e < PB * (PB + 1); e += nt) Tp[e / (PB + 1)][e % (PB + 1)] = 0.0;
+    __syncthreads();
     for (int jj = 0; jj < pw; ++jj) {
       const double tj = tau[j0 + jj];
       if (tid < jj) {
         double z = 0.0;
         for (int r = j0 + jj; r < S2; ++r)
-          z += (r - j0 == tid ? 1.0 : (r - j0 > tid ? pan[r * PL + tid] : 0.0)) * (r == j0 + jj ? 1.0 : pan[r * PL + jj]);
-        W[tid] = z;
+          z += (r - j0 == tid ? 1.0 : (r - j0 > tid ? pan[r][tid] : 0.0)) * (r == j0 + jj ? 1.0 : pan[r][jj]);
+        zv[tid] = z;
       }
       __syncthreads();
       if (tid < jj) {
         double acc = 0.0;
-        for (int m = tid; m < jj; ++m) acc += Tp[tid * PB + m] * W[m];
-        Tp[tid * PB + jj] = -tj * acc;
+        for (int m = tid; m < jj; ++m) acc += Tp[tid][m] * zv[m];
+        Tp[tid][jj] = -tj * acc;
       }
-      if (tid == 0) Tp[jj * PB + jj] = tj;
+      if (tid == 0) Tp[jj][jj] = tj;
       __syncthreads();
     }
-    // trailing update: D[:, c] -= Yp Tp^T (Yp^T D[:, c]) for c in [j0+pw, w)
-    const int c0 = j0 + pw;
-    const int ntr = w - c0;
-    for (int cb = 0; cb < ntr; cb += 64) {
-      const int nc = min(64, ntr - cb);
-      // W[c][q] = sum_r Yp[r][q] D[r][c]
-      for (int e = tid; e < nc * pw; e += nt) {
-        const int c = e / pw, q = e % pw;
-        const int jq = j0 + q;
-        double z = D[(size_t)jq * S + c0 + cb + c];
-        for (int r = jq + 1; r < S2; ++r) z += pan[r * PL + q] * D[(size_t)r * S + c0 + cb + c];
-        W[c * PB + q] = z;
-      }
-      __syncthreads();
-      // W[c] <- Tp^T W[c]
-      for (int e = tid; e < nc * pw; e += nt) {
-        const int c = e / pw, q = e % pw;
-        double z = 0.0;
-        for (int m = 0; m <= q; ++m) z += Tp[m * PB + q] * W[c * PB + m];
-        W[(64 + c) * PB + q] = z;
-      }
-      __syncthreads();
-      // D[r][c] -= sum_q Yp[r][q] W'[c][q]
-      for (int e = tid; e < (S2 - j0) * nc; e += nt) {
-        const int r = j0 + e / nc, c = e % nc;
-        double z = 0.0;
-        for (int q = 0; q < pw; ++q) {
-          const int jq = j0 + q;
-          const double y = r == jq ? 1.0 : (r > jq ? pan[r * PL + q] : 0.0);
-          z += y * W[(64 + c) * PB + q];
-        }
-        D[(size_t)r * S + c0 + cb + c] -= z;
-      }
-      __syncthreads();
-    }
-    // write the factored panel back
+    // factored panel back to D (R above, reflectors below the diagonal)
     for (int e = tid; e < S2 * PB; e += nt) {
       const int r = e / PB, c = e % PB;
-      if (c < pw && r >= j0) D[(size_t)r * S + j0 + c] = pan[r * PL + c];
+      if (c < pw && r >= j0) D[(size_t)r * S + j0 + c] = pan[r][c];
+    }
+    // Panel reflector Yp (2S x 16): Yp[r][q] = 1 at r == j0 + q, pan[r][q] below.
+    auto ypan = [&](int r, int q) -> double {
+      return r == j0 + q ? 1.0 : (r > j0 + q ? pan[r][q] : 0.0);
+    };
+    // Trailing update, 16-column tiles, one tile per wavefront:
+    // D[:, tile] -= Yp Tp^T (Yp^T D[:, tile]).
+    const int c0 = j0 + PB;
+    double (*sw)[17] = scr[wid];
+    for (int tt = wid; c0 + tt * 16 < w; tt += nw) {
+      const int tc = c0 + tt * 16 + lc;
+      const bool okc = tc < w;
+      dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int r0 = j0; r0 < S2; r0 += 4) {
+        const int r = r0 + lr;
+        a4 = mfma4(ypan(r, lc), okc ? D[(size_t)r * S + tc] : 0.0, a4);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = a4[q];
+      // W' = Tp^T W  (16 x 16)
+      dbl4 b4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m0 = 0; m0 < PB; m0 += 4) {
+        const int m = m0 + lr;
+        b4 = mfma4(m <= lc ? Tp[m][lc] : 0.0, sw[m][lc], b4);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = b4[q];
+      // D[r][tile] -= sum_q Yp[r][q] W'[q][tile] for rows j0..2S-1
+      for (int rt = j0 / 16; rt < S2 / 16; ++rt) {
+        dbl4 d4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q0 = 0; q0 < PB; q0 += 4) d4 = mfma4(ypan(rt * 16 + lc, q0 + lr), sw[q0 + lr][lc], d4);
+        if (okc)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) D[(size_t)(rt * 16 + lr + 4 * q) * S + tc] -= d4[q];
+      }
+    }
+    // T accumulation: T[j0:j0+16, j0:j0+16] = Tp; T[0:j0, j0:j0+16] =
+    // -T[0:j0, 0:j0] (Y[:, 0:j0]^T Yp) Tp.  Y[r][m] for m < j0 <= r is D[r][m].
+    for (int e = tid; e < PB * PB; e += nt) {
+      const int q = e / PB, c = e % PB;
+      if (q < pw && c < pw) Tg[(size_t)(j0 + q) * S + j0 + c] = Tp[q][c];
+    }
+    if (j0 > 0) {
+      for (int mt = wid; mt < j0 / 16; mt += nw) {
+        dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+        const int m = mt * 16 + lc;
+#pragma unroll 4
+        for (int r0 = j0; r0 < S2; r0 += 4) {
+          const int r = r0 + lr;
+          a4 = mfma4(D[(size_t)r * S + m], ypan(r, lc), a4);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zs[mt * 16 + lr + 4 * q][lc] = a4[q];
+      }
+      __syncthreads();
+      dbl4 z2[(S / 16 + 15) / 16];
+#pragma unroll
+      for (int u = 0; u < (S / 16 + 15) / 16; ++u) {
+        const int mt = wid + nw * u;
+        dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+        if (mt < j0 / 16) {
+          const int m = mt * 16 + lc;
+          for (int k0 = mt * 16; k0 < j0; k0 += 4) {
+            const int kk = k0 + lr;
+            a4 = mfma4(kk >= m ? Tg[(size_t)m * S + kk] : 0.0, zs[kk][lc], a4);
+          }
+        }
+        z2[u] = a4;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (S / 16 + 15) / 16; ++u) {
+        const int mt = wid + nw * u;
+        if (mt < j0 / 16)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) zs[mt * 16 + lr + 4 * q][lc] = z2[u][q];
+      }
+      __syncthreads();
+      for (int mt = wid; mt < j0 / 16; mt += nw) {
+        dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q0 = 0; q0 < PB; q0 += 4) a4 = mfma4(zs[mt * 16 + lc][q0 + lr], Tp[q0 + lr][lc], a4);
+        if (lc < pw)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) Tg[(size_t)(mt * 16 + lr + 4 * q) * S + j0 + lc] = -a4[q];
+      }
     }
     __syncthreads();
   }
-  for (int j = w + tid; j < S; j += nt) tau[j] = 0.0;
-  // T^{-1} = striu(Y^T Y) + diag(1/tau) in LDS (w x w, ld S), then T by
-  // column-oriented back substitution (one column per thread).
-  double *Ti = lds;
-  for (int e = tid; e < w * w; e += nt) {
-    const int q = e / w, j = e % w;
-    double v = 0.0;
-    if (q < j) {
-      for (int r = j; r < S2; ++r) v += yref(D, S, r, q) * yref(D, S, r, j);
-    } else if (q == j) {
-      v = tau[j] != 0.0 ? 1.0 / tau[j] : 0.0;
-    }
-    Ti[q * S + j] = v;
-  }
+}
+
+// Grid (S/16 column tiles, groups): E = [Z_k; 0] <- Omega_k^T E on the tile's
+// 16 columns, the residual estimates ||Lambda[:, c]|| of the new Lambda =
+// E[S:2S], then zfin(k) = E[0:S], E[0:S] = Lambda, E[S:2S] = 0.
+template <int S>
+__global__ __launch_bounds__(256) void hess_rot_kernel(HessQrArgs a) {
+  __shared__ double seg[2 * S][17];
+  __shared__ double m1[S][17];
+  __shared__ double part[16][17];
+  const int b0 = blockIdx.x * 16, g = blockIdx.y;
+  const GroupDesc gd = a.groups[g];
+  if (b0 >= gd.w || !gd.active) return;
+  const int tid = threadIdx.x;
+  double *E = a.E + (size_t)g * 2 * S * S;
+  for (int e = tid; e < 2 * S * 16; e += 256) seg[e >> 4][e & 15] = E[(size_t)(e >> 4) * S + b0 + (e & 15)];
   __syncthreads();
-  for (int c = tid; c < S; c += nt) {
-    for (int r = S - 1; r >= 0; --r) {
-      double v = 0.0;
-      if (r < w && c < w && r <= c && tau[r] != 0.0 && tau[c] != 0.0) {
-        v = (r == c) ? 1.0 : 0.0;
-        for (int m = r + 1; m <= c; ++m) v -= Ti[r * S + m] * Tg[(size_t)m * S + c];
-        v /= Ti[r * S + r];
-      }
-      Tg[(size_t)r * S + c] = v;
-    }
-  }
-  __syncthreads();
-  // E <- (I - Y T^T Y^T) E  (E = [Z_k; 0], w columns)
-  double *M1 = lds;                     // M1[q * S + c] = (Y^T E)[q][c]
-  for (int e = tid; e < w * w; e += nt) {
-    const int q = e / w, c = e % w;
-    double z = 0.0;
-    for (int r = q; r < S2; ++r) z += yref(D, S, r, q) * E[(size_t)r * S + c];
-    M1[q * S + c] = z;
-  }
-  __syncthreads();
-  for (int c = tid; c < w; c += nt) {   // M1[:, c] <- T^T M1[:, c], in place from the bottom
-    for (int q = w - 1; q >= 0; --q) {
-      double z = 0.0;
-      for (int m = 0; m <= q; ++m) z += Tg[(size_t)m * S + q] * M1[m * S + c];
-      M1[q * S + c] = z;
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < S2 * w; e += nt) {
-    const int r = e / w, c = e % w;
-    double z = 0.0;
-    const int qmax = min(r, w - 1);
-    for (int q = 0; q <= qmax; ++q) z += yref(D, S, r, q) * M1[q * S + c];
-    E[(size_t)r * S + c] -= z;
-  }
-  __syncthreads();
-  // residual norms of the new Lambda = E[S:2S, :w] per column
-  for (int c = wid; c < w; c += nw) {
+  wy_apply_tile<S>(hblock(a.cols, a.k, g, a.k, S), htmat(a.cols, a.k, g, a.ng, S), seg, m1, S);
+  {
+    const int c = tid & 15, rg = tid >> 4;
     double sq = 0.0;
-    for (int r = S + lane; r < S2; r += 64) {
-      const double v = E[(size_t)r * S + c];
-      sq += v * v;
-    }
-    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-    if (lane == 0) a.rnorm[a.groups[g].slot + c] = sqrt(sq);
+    for (int r = S + rg; r < 2 * S; r += 16) sq += seg[r][c] * seg[r][c];
+    part[rg][c] = sq;
   }
   __syncthreads();
-  // zfin(k) = E[0:S]; E[0:S] = E[S:2S] (next right-hand side); E[S:2S] = 0.
+  if (tid < 16 && b0 + tid < gd.w) {
+    double sq = 0.0;
+    for (int q = 0; q < 16; ++q) sq += part[q][tid];
+    a.rnorm[gd.slot + b0 + tid] = sqrt(sq);
+  }
   double *zf = hzfin(a.cols, a.k, g, a.ng, S);
-  for (int e = tid; e < S * S; e += nt) {
-    zf[e] = E[e];
-    E[e] = E[S * S + e];
-    E[S * S + e] = 0.0;
+  for (int e = tid; e < S * 16; e += 256) {
+    const int r = e >> 4, c = e & 15;
+    zf[(size_t)r * S + b0 + c] = seg[r][c];
+    E[(size_t)r * S + b0 + c] = seg[S + r][c];
+    E[(size_t)(S + r) * S + b0 + c] = 0.0;
   }
 }
 
@@ -971,14 +1055,38 @@ void launch_trmm_upper(hipStream_t s, const double *A, const double *B, double *
 
 void launch_hess_apply(hipStream_t s, HessArgs a, int ng) {
   if (a.k <= 0) return;
-  const size_t shm = (size_t)(a.k + 1) * a.S * sizeof(double);
-  const int lds = shm <= kLdsLimit ? 1 : 0;
-  hipLaunchKernelGGL(hess_apply_kernel, dim3(a.S, ng), dim3(kB), lds ? shm : 0, s, a, lds);
+  const dim3 grid(a.S / 16, ng);
+  switch (a.S) {
+    case 16: hipLaunchKernelGGL(hess_apply_kernel<16>, grid, dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(hess_apply_kernel<32>, grid, dim3(256), 0, s, a); break;
+    case 64: hipLaunchKernelGGL(hess_apply_kernel<64>, grid, dim3(256), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(hess_apply_kernel<128>, grid, dim3(256), 0, s, a); break;
+    default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", a.S);
+  }
   BPF_CHECK_HIP(hipGetLastError());
 }
 
 void launch_hess_qr(hipStream_t s, HessQrArgs a, int ng) {
-  hipLaunchKernelGGL(hess_qr_kernel, dim3(ng), dim3(1024), 0, s, a);
+  const dim3 grid(a.S / 16, ng);
+  switch (a.S) {
+    case 16:
+      hipLaunchKernelGGL(hess_qr_kernel<16>, dim3(ng), dim3(1024), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<16>, grid, dim3(256), 0, s, a);
+      break;
+    case 32:
+      hipLaunchKernelGGL(hess_qr_kernel<32>, dim3(ng), dim3(1024), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<32>, grid, dim3(256), 0, s, a);
+      break;
+    case 64:
+      hipLaunchKernelGGL(hess_qr_kernel<64>, dim3(ng), dim3(1024), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<64>, grid, dim3(256), 0, s, a);
+      break;
+    case 128:
+      hipLaunchKernelGGL(hess_qr_kernel<128>, dim3(ng), dim3(1024), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<128>, grid, dim3(256), 0, s, a);
+      break;
+    default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", a.S);
+  }
   BPF_CHECK_HIP(hipGetLastError());
 }
 
