@@ -33,7 +33,7 @@ __all__ = [
     "ilu0", "ic0", "ldiv_", "mul_", "block_gmres", "block_gmres_", "BlockGMRESSolver",
     "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
     "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
-    "block_gmres_batched", "householder", "block_hessenberg_lsq",
+    "block_gmres_batched", "householder", "block_hessenberg_lsq", "set_option", "get_option",
 ]
 
 L = _lib.load()
@@ -79,6 +79,22 @@ def default_context() -> Context:
     if _DEFAULT is None:
         _DEFAULT = Context(0)
     return _DEFAULT
+
+
+_OPTIONS = {"syncfree": 1}
+
+
+def set_option(name: str, value) -> None:
+    """Process-wide execution option: "syncfree" (default False) runs each
+    triangular solve / factorization as one sync-free launch instead of one
+    launch per wide dependency level."""
+    check(L.bpf_set_option(_OPTIONS[name], int(value)))
+
+
+def get_option(name: str) -> int:
+    v = C.c_longlong()
+    check(L.bpf_get_option(_OPTIONS[name], C.byref(v)))
+    return v.value
 
 
 def device_count() -> int:

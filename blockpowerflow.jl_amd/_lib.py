@@ -59,6 +59,8 @@ d = C.c_double
 SIGNATURES = {
     "bpf_last_error": [],
     "bpf_version": [],
+    "bpf_set_option": [i, C.c_longlong],
+    "bpf_get_option": [i, C.POINTER(C.c_longlong)],
     "bpf_device_count": [C.POINTER(C.c_int)],
     "bpf_ctx_create": [i, C.POINTER(vp)],
     "bpf_ctx_destroy": [vp],

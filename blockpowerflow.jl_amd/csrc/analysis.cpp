@@ -116,6 +116,9 @@ void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
   op.val32.ensure(m * sizeof(float));
   op.diag.ensure(std::max(1, n) * sizeof(double));
   op.diag32.ensure(std::max(1, n) * sizeof(float));
+  upload(op.flags, std::vector<int>(std::max(1, n), 0), s);
+  upload(op.status, std::vector<int>(1, 0), s);
+  op.epoch = 0;
 }
 
 }  // namespace bpf

@@ -72,6 +72,21 @@ extern "C" {
 const char *bpf_last_error(void) { return g_err; }
 const char *bpf_version(void) { return "bpf_amd 0.1 (gfx950)"; }
 
+int bpf_set_option(int option, long long value) {
+  return guard([&] {
+    if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
+    else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
+  });
+}
+
+int bpf_get_option(int option, long long *value) {
+  return guard([&] {
+    BPF_REQUIRE(value, "null argument");
+    if (option == BPF_OPT_SYNCFREE) *value = g_syncfree ? 1 : 0;
+    else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
+  });
+}
+
 int bpf_device_count(int *count) {
   return guard([&] {
     BPF_REQUIRE(count, "null count");
@@ -331,6 +346,7 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A) {
     int status = 0;
     BPF_CHECK_HIP(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, s));
     BPF_CHECK_HIP(hipStreamSynchronize(s));
+    triop_check(P->lo, s);
     triop_refresh(P->lo, P->fac.as<double>(), s);
     triop_refresh(P->up, P->fac.as<double>(), s);
     if (status < P->n)
@@ -410,6 +426,8 @@ int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p, const void *d_X, in
     } else {
       BPF_THROW(BPF_E_INVALID, "bad layout");
     }
+    triop_check(P->lo, s);
+    triop_check(P->up, s);
   });
 }
 

@@ -125,6 +125,10 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     }
     launch_rm_to_cm(s, n, p, X.ptr, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));
+    if (N) {
+      triop_check(N->lo, s);
+      triop_check(N->up, s);
+    }
     bpf_solver_stats st{};
     st.niter = iter;
     st.solved = solved;

@@ -130,7 +130,19 @@ struct TriOp {
   DevBuf order, ptr, col, gather, dgather, lvl;
   DevBuf val, val32, diag, diag32;   // refreshed from the factor values
   int max_row_nnz = 0;
+  // Sync-free execution: per-row completion flags (original row numbering)
+  // tagged with the epoch of the launch that produced them, and a status
+  // word set when a bounded dependency wait ran out.
+  DevBuf flags, status;
+  int epoch = 0;
 };
+
+// Process-wide switch between the sync-free single-launch triangular
+// solves / factorizations (default) and the level-by-level launches.
+extern bool g_syncfree;
+// Reads the op's status word (blocking); throws BPF_E_HIP if a sync-free
+// dependency wait ran out, after switching to level-by-level launches.
+void triop_check(TriOp &op, hipStream_t s);
 
 // Launch plan: consecutive levels run either one launch per level ("wide")
 // or many levels in one single-workgroup launch ("narrow").  A level is wide
@@ -176,15 +188,15 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
 
 // y = op^{-1} x for p columns (row-major).  x may alias y.
 template <typename TV>
-void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy);
+void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy);
 
 // Numeric IKJ factorization on a combined pattern (ILU0 when the pattern is
 // A's, exact LU when it is the full fill pattern).  The values must already
 // hold the (permuted) matrix entries.  Uses the lower TriOp's levels.
-void launch_ilu_factor(hipStream_t s, const TriOp &lower, int n, const int *rowptr,
+void launch_ilu_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                        const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                        int *d_status);
-void launch_ic_factor(hipStream_t s, const TriOp &lower, int n, const int *rowptr,
+void launch_ic_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                       int *d_status);
 

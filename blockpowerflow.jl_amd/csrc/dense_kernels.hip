@@ -488,7 +488,7 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
     dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
     if (jt < NT) {
       const int j = jt * 16 + lc;
-#pragma unroll 4
+#pragma unroll 8
       for (int r0 = jt * 16; r0 < rmax; r0 += 4) {
         const int r = r0 + lr;
         const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
@@ -511,7 +511,7 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
     dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
     if (jt < NT) {
       const int j = jt * 16 + lc;
-#pragma unroll 4
+#pragma unroll 8
       for (int q0 = 0; q0 < jt * 16 + 16; q0 += 4) {
         const int q = q0 + lr;
         const double av = q <= j ? T[(size_t)q * S + j] : 0.0;
@@ -533,7 +533,7 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
     dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
     const int r = rt * 16 + lc;
     const int jend = min(S, rt * 16 + 16);
-#pragma unroll 4
+#pragma unroll 8
     for (int j0 = 0; j0 < jend; j0 += 4) {
       const int j = j0 + lr;
       const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
@@ -658,22 +658,28 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
       }
       __syncthreads();
     }
-    // panel T (pw x pw upper): T[jj][jj] = tau, T[0:jj, jj] = -tau T[0:jj,0:jj] (Yp^T y_jj)
-    double *zv = &scr[0][0][0];
+    // Panel reflector Yp (2S x 16): Yp[r][q] = 1 at r == j0 + q, pan[r][q] below.
+    auto ypan = [&](int r, int q) -> double {
+      return r == j0 + q ? 1.0 : (r > j0 + q ? pan[r][q] : 0.0);
+    };
+    // panel T (pw x pw upper) by the forward recurrence T[jj][jj] = tau_jj,
+    // T[0:jj, jj] = -tau_jj T[0:jj, 0:jj] Gp[0:jj, jj] with the panel Gram
+    // matrix Gp = Yp^T Yp (one wavefront, matrix cores).
+    double (*gp)[17] = scr[0];
     for (int e = tid; e < PB * (PB + 1); e += nt) Tp[e / (PB + 1)][e % (PB + 1)] = 0.0;
+    if (wid == 0) {
+      dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+      for (int r0 = j0; r0 < S2; r0 += 4) a4 = mfma4(ypan(r0 + lr, lc), ypan(r0 + lr, lc), a4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gp[lr + 4 * q][lc] = a4[q];
+    }
     __syncthreads();
     for (int jj = 0; jj < pw; ++jj) {
       const double tj = tau[j0 + jj];
       if (tid < jj) {
-        double z = 0.0;
-        for (int r = j0 + jj; r < S2; ++r)
-          z += (r - j0 == tid ? 1.0 : (r - j0 > tid ? pan[r][tid] : 0.0)) * (r == j0 + jj ? 1.0 : pan[r][jj]);
-        zv[tid] = z;
-      }
-      __syncthreads();
-      if (tid < jj) {
         double acc = 0.0;
-        for (int m = tid; m < jj; ++m) acc += Tp[tid][m] * zv[m];
+        for (int m = tid; m < jj; ++m) acc += Tp[tid][m] * gp[m][jj];
         Tp[tid][jj] = -tj * acc;
       }
       if (tid == 0) Tp[jj][jj] = tj;
@@ -684,10 +690,6 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
       const int r = e / PB, c = e % PB;
       if (c < pw && r >= j0) D[(size_t)r * S + j0 + c] = pan[r][c];
     }
-    // Panel reflector Yp (2S x 16): Yp[r][q] = 1 at r == j0 + q, pan[r][q] below.
-    auto ypan = [&](int r, int q) -> double {
-      return r == j0 + q ? 1.0 : (r > j0 + q ? pan[r][q] : 0.0);
-    };
     // Trailing update, 16-column tiles, one tile per wavefront:
     // D[:, tile] -= Yp Tp^T (Yp^T D[:, tile]).
     const int c0 = j0 + PB;
@@ -696,7 +698,7 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
       const int tc = c0 + tt * 16 + lc;
       const bool okc = tc < w;
       dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+#pragma unroll 8
       for (int r0 = j0; r0 < S2; r0 += 4) {
         const int r = r0 + lr;
         a4 = mfma4(ypan(r, lc), okc ? D[(size_t)r * S + tc] : 0.0, a4);
@@ -732,7 +734,7 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
       for (int mt = wid; mt < j0 / 16; mt += nw) {
         dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
         const int m = mt * 16 + lc;
-#pragma unroll 4
+#pragma unroll 8
         for (int r0 = j0; r0 < S2; r0 += 4) {
           const int r = r0 + lr;
           a4 = mfma4(D[(size_t)r * S + m], ypan(r, lc), a4);
@@ -748,6 +750,7 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
         dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
         if (mt < j0 / 16) {
           const int m = mt * 16 + lc;
+#pragma unroll 8
           for (int k0 = mt * 16; k0 < j0; k0 += 4) {
             const int kk = k0 + lr;
             a4 = mfma4(kk >= m ? Tg[(size_t)m * S + kk] : 0.0, zs[kk][lc], a4);

@@ -326,6 +326,10 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   BPF_CHECK_HIP(hipMemcpy(&fl, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost));
   shifted_ = fl != 0;
   form<TV>(X, 1.0);
+  if (N_) {
+    triop_check(N_->lo, st_);
+    triop_check(N_->up, st_);
+  }
   return it;
 }
 

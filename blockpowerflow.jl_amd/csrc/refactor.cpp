@@ -67,6 +67,7 @@ void refactor_into(bpf_rf rf, const double *d_aval, double *fac) {
   int st = 0;
   BPF_CHECK_HIP(hipMemcpyAsync(&st, rf->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   BPF_CHECK_HIP(hipStreamSynchronize(s));
+  triop_check(rf->lo, s);
   if (st < rf->n) BPF_THROW(BPF_E_SINGULAR, "refactorization: zero pivot at step %d", st);
 }
 
@@ -190,6 +191,8 @@ int bpf_rf_solve(bpf_rf rf, int dtype, int nrhs, const double *d_B, int ldb, dou
       launch_perm_rm_to_cm<float>(s, n, nrhs, z, nrhs, rf->dQ.as<int>(), d_X, ldx);
     }
     BPF_CHECK_HIP(hipStreamSynchronize(s));
+    triop_check(rf->lo, s);
+    triop_check(rf->up, s);
   });
 }
 
@@ -227,6 +230,8 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
     }
     launch_rm_to_cm(s, n, p, X, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));
+    triop_check(rf->lo, s);
+    triop_check(rf->up, s);
   });
 }
 
@@ -313,6 +318,8 @@ int bpf_rf_batch_solve(bpf_rf rf, const double *d_B, int ldb, double *d_X, int l
       launch_perm_rm_to_cm<double>(s, n, 1, z, 1, rf->dQ.as<int>(), d_X + (size_t)b * ldx, ldx);
     }
     BPF_CHECK_HIP(hipStreamSynchronize(s));
+    triop_check(rf->lo, s);
+    triop_check(rf->up, s);
   });
 }
 

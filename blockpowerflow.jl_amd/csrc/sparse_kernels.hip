@@ -150,21 +150,122 @@ __global__ __launch_bounds__(kBlock) void tri_level_kernel(TriDev a, int t0, int
   tri_row<TF, TV, UNIT, CPL>(a, t, threadIdx.x & (G - 1), G, x, ldx, y, ldy, p);
 }
 
+// ------------------------------------------------------------- sync-free
+// One launch for the whole operator (Liu et al., "A synchronization-free
+// algorithm for parallel sparse triangular solves", 2016): one wavefront per
+// row in level order; a row waits on the completion flags of the rows it
+// depends on, then publishes its own.  Rows are dispatched in level order, so
+// every row waited on belongs to a workgroup dispatched earlier.  The wait is
+// bounded: when it runs out (or another wavefront already reported that) the
+// status word is set and the row still publishes, so every wavefront exits.
+constexpr int kSpinLimit = 1 << 20;
+
+__device__ __forceinline__ bool sf_wait(const int *flags, const int *cols, int k0, int k1, int epoch,
+                                        int lane, int *status) {
+  // Relaxed polls (coherent at device scope, no cache invalidation); one
+  // acquire fence once every dependency is complete.
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int k = kb + lane;
+    const int j = k < k1 ? cols[k] : -1;
+    for (int spins = 0;; ++spins) {
+      const bool ready =
+          j < 0 || __hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+      if (__all(ready)) break;
+      if (spins >= kSpinLimit ||
+          __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        if (lane == 0) atomicOr(status, 1);
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+__device__ __forceinline__ void sf_publish(int *flags, int i, int epoch, int lane) {
+  // every lane's stores of row i reach device scope before the flag does
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) __hip_atomic_store(flags + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename TF, typename TV, bool UNIT, int CPL>
+__global__ __launch_bounds__(kBlock) void tri_syncfree_kernel(TriDev a, int n, const TV *x, int ldx, TV *y,
+                                                              int ldy, int p, int *flags, int epoch,
+                                                              int *status) {
+  const int t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (t >= n) return;
+  const int lane = threadIdx.x & 63;
+  if (sf_wait(flags, a.col, a.ptr[t], a.ptr[t + 1], epoch, lane, status))
+    tri_row<TF, TV, UNIT, CPL>(a, t, lane, 64, x, ldx, y, ldy, p);
+  sf_publish(flags, a.order[t], epoch, lane);
+}
+
 // One single-workgroup launch walks many narrow levels [l0, l1), with a
 // workgroup barrier between levels.
 constexpr int kTailBlock = 1024;
+// With one row per wavefront (G = 64) and a single column chunk (p <= 64
+// CPL), a level with fewer rows than wavefronts splits each row's non-zeros
+// over several wavefronts; the partial sums meet in LDS.  Deep elimination
+// tails (one long row per level, as in fill-reducing LU orderings) then run
+// at the speed of the whole workgroup instead of one wavefront.
 template <typename TF, typename TV, bool UNIT, int CPL>
 __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, int l1,
                                                               const TV *x, int ldx, TV *y,
                                                               int ldy, int p, int glog) {
+  constexpr int NW = kTailBlock / 64;
+  __shared__ TV part[NW][64 * CPL];
   const int G = 1 << glog;
   const int ngroups = kTailBlock >> glog;
   const int gid = threadIdx.x >> glog;
   const int lane = threadIdx.x & (G - 1);
+  const bool can_split = G == 64 && p <= 64 * CPL;
+  const TF *val = static_cast<const TF *>(a.val);
   for (int l = l0; l < l1; ++l) {
-    const int t1 = a.lvl[l + 1];
-    for (int t = a.lvl[l] + gid; t < t1; t += ngroups)
-      tri_row<TF, TV, UNIT, CPL>(a, t, lane, G, x, ldx, y, ldy, p);
+    const int t0 = a.lvl[l], t1 = a.lvl[l + 1];
+    const int rows = t1 - t0;
+    if (!can_split || rows >= NW) {
+      for (int t = t0 + gid; t < t1; t += ngroups)
+        tri_row<TF, TV, UNIT, CPL>(a, t, lane, G, x, ldx, y, ldy, p);
+    } else {
+      const int wpr = NW / rows;          // wavefronts per row
+      const int r = gid / wpr, sub = gid % wpr;
+      if (r < rows) {
+        const int t = t0 + r;
+        const int k0 = a.ptr[t], k1 = a.ptr[t + 1];
+        TV acc[CPL];
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
+        for (int k = k0 + sub; k < k1; k += wpr) {
+          const int j = a.col[k];
+          const TV v = static_cast<TV>(val[k]);
+          const TV *yr = y + (size_t)j * ldy;
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const int c = lane + m * 64;
+            if (c < p) acc[m] = fma(v, yr[c], acc[m]);
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) part[gid][lane + m * 64] = acc[m];
+      }
+      __syncthreads();
+      if (r < rows && sub == 0) {
+        const int t = t0 + r;
+        const int i = a.order[t];
+        TV dinv = TV(1);
+        if (!UNIT) dinv = TV(1) / static_cast<TV>(static_cast<const TF *>(a.diag)[t]);
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+          const int c = lane + m * 64;
+          if (c < p) {
+            TV s = x[(size_t)i * ldx + c];
+            for (int q = 0; q < wpr; ++q) s -= part[gid + q][c];
+            y[(size_t)i * ldy + c] = UNIT ? s : s * dinv;
+          }
+        }
+      }
+    }
     __syncthreads();
   }
 }
@@ -274,6 +375,27 @@ __global__ void fac_level_kernel(FacDev f, int t0, int t1) {
                   (size_t)wave * f.cap;
   if (IC) ic_row(f, f.order[t], lane, s_col, s_val);
   else ilu_row(f, f.order[t], lane, s_col, s_val);
+}
+
+// Sync-free factorization: one wavefront per row in level order; row i waits
+// for the rows of its strictly lower part (dependency list dptr/dcol of the
+// lower operator), eliminates, then publishes.
+template <bool IC>
+__global__ void fac_syncfree_kernel(FacDev f, int n, const int *dptr, const int *dcol, int *flags,
+                                    int epoch, int *status) {
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= n) return;
+  int *s_col = reinterpret_cast<int *>(smem) + (size_t)wave * f.cap;
+  double *s_val = reinterpret_cast<double *>(smem + (size_t)(blockDim.x >> 6) * f.cap * sizeof(int)) +
+                  (size_t)wave * f.cap;
+  const int i = f.order[t];
+  if (sf_wait(flags, dcol, dptr[t], dptr[t + 1], epoch, lane, status)) {
+    if (IC) ic_row(f, i, lane, s_col, s_val);
+    else ilu_row(f, i, lane, s_col, s_val);
+  }
+  sf_publish(flags, i, epoch, lane);
 }
 
 template <bool IC>
@@ -395,12 +517,50 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
   BPF_CHECK_HIP(hipGetLastError());
 }
 
+bool g_syncfree = false;
+
+void triop_check(TriOp &op, hipStream_t s) {
+  if (op.n <= 0) return;
+  int st = 0;
+  BPF_CHECK_HIP(hipMemcpyAsync(&st, op.status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  BPF_CHECK_HIP(hipStreamSynchronize(s));
+  if (st != 0) {
+    const int zero = 0;
+    BPF_CHECK_HIP(hipMemcpy(op.status.ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+    g_syncfree = false;
+    BPF_THROW(BPF_E_HIP,
+              "sync-free triangular solve/factorization: a dependency wait ran out; results of the "
+              "last call are invalid (level-by-level launches are used from now on)");
+  }
+}
+
 template <typename TV>
-void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy) {
+void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int ldy) {
   if (op.n <= 0 || p <= 0) return;
   constexpr bool F32 = std::is_same<TV, float>::value;
   TriDev a{op.order.as<int>(), op.ptr.as<int>(), op.col.as<int>(), op.lvl.as<int>(),
            F32 ? op.val32.ptr : op.val.ptr, F32 ? op.diag32.ptr : op.diag.ptr};
+  if (g_syncfree) {
+    // one wavefront per row, columns spread over the 64 lanes
+    const int cpl = p <= 64 ? 1 : (p <= 128 ? 2 : 4);
+    const int epoch = ++op.epoch;
+    dim3 grid((op.n + kBlock / 64 - 1) / (kBlock / 64));
+    const int tok = prof::begin(s);
+#define BPF_SF(U, C)                                                                               \
+  hipLaunchKernelGGL((tri_syncfree_kernel<TV, TV, U, C>), grid, dim3(kBlock), 0, s, a, op.n, x, ldx, y, \
+                     ldy, p, op.flags.as<int>(), epoch, op.status.as<int>())
+    if (op.unit) {
+      if (cpl == 1) BPF_SF(true, 1); else if (cpl == 2) BPF_SF(true, 2); else BPF_SF(true, 4);
+    } else {
+      if (cpl == 1) BPF_SF(false, 1); else if (cpl == 2) BPF_SF(false, 2); else BPF_SF(false, 4);
+    }
+#undef BPF_SF
+    BPF_CHECK_HIP(hipGetLastError());
+    const double rows = op.n, nz = (double)op.h_col.size(), w = sizeof(TV);
+    prof::end(tok, s, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
+              2.0 * nz * p);
+    return;
+  }
   int glog, cpl;
   lane_shape(p, glog, cpl);
   const int rpb = kBlock >> glog;
@@ -443,10 +603,10 @@ void trisolve(hipStream_t s, const TriOp &op, int p, const TV *x, int ldx, TV *y
   }
   BPF_CHECK_HIP(hipGetLastError());
 }
-template void trisolve<double>(hipStream_t, const TriOp &, int, const double *, int, double *, int);
-template void trisolve<float>(hipStream_t, const TriOp &, int, const float *, int, float *, int);
+template void trisolve<double>(hipStream_t, TriOp &, int, const double *, int, double *, int);
+template void trisolve<float>(hipStream_t, TriOp &, int, const float *, int, float *, int);
 
-static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int *rowptr,
+static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowptr,
                           const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                           int *d_status) {
   const int cap = std::max(64, max_row_nnz);
@@ -456,6 +616,22 @@ static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int 
   int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
   FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), fac, d_status, cap};
   const int tok = prof::begin(s);
+  if (g_syncfree && lower.n > 0) {
+    const size_t shm = per_wave * wpb;
+    const int epoch = ++lower.epoch;
+    dim3 grid((lower.n + wpb - 1) / wpb);
+    if (ic)
+      hipLaunchKernelGGL((fac_syncfree_kernel<true>), grid, dim3(64 * wpb), shm, s, f, lower.n,
+                         lower.ptr.as<int>(), lower.col.as<int>(), lower.flags.as<int>(), epoch,
+                         lower.status.as<int>());
+    else
+      hipLaunchKernelGGL((fac_syncfree_kernel<false>), grid, dim3(64 * wpb), shm, s, f, lower.n,
+                         lower.ptr.as<int>(), lower.col.as<int>(), lower.flags.as<int>(), epoch,
+                         lower.status.as<int>());
+    BPF_CHECK_HIP(hipGetLastError());
+    prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
+    return;
+  }
   const auto plan = make_plan(lower, 32);
   for (const auto &seg : plan) {
     if (seg.wide) {
@@ -477,13 +653,13 @@ static void launch_factor(bool ic, hipStream_t s, const TriOp &lower, const int 
   prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
 }
 
-void launch_ilu_factor(hipStream_t s, const TriOp &lower, int, const int *rowptr,
+void launch_ilu_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
                        const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                        int *d_status) {
   launch_factor(false, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);
 }
 
-void launch_ic_factor(hipStream_t s, const TriOp &lower, int, const int *rowptr,
+void launch_ic_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                       int *d_status) {
   launch_factor(true, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);

@@ -51,6 +51,18 @@ typedef struct bpf_batch_s *bpf_batch;
 const char *bpf_last_error(void);
 const char *bpf_version(void);
 
+/* Process-wide execution options (no reference counterpart).
+ * BPF_OPT_SYNCFREE: 1 = triangular solves and ILU(0)/IC(0)/LU
+ * factorizations run as one sync-free launch (one wavefront per row waiting
+ * on its dependencies' completion flags); 0 (default) = one launch per wide
+ * dependency level and one single-workgroup launch for runs of narrow levels.
+ * On MI355X the device-scope flag handshake costs more than the launches it
+ * saves (each release/acquire writes back / invalidates the per-XCD L2), so
+ * the sync-free form is slower there; it is kept as an option. */
+enum { BPF_OPT_SYNCFREE = 1 };
+int bpf_set_option(int option, long long value);
+int bpf_get_option(int option, long long *value);
+
 /* ---- context, device memory ------------------------------------------ */
 /* Replaces the implicit CUDA.jl / AMDGPU.jl task-local device and default
  * stream that every reference call runs on (src/BlockPowerFlow.jl:9-15). */

@@ -136,3 +136,29 @@ def test_backward_error_matches_host(bpf, gen, oracle):
     got = bpf.backward_error(A, bpf.HipMatrix.from_numpy(Xh), bpf.HipMatrix.from_numpy(S.B))
     ref = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
     np.testing.assert_allclose(got, ref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("p", [16, 128])
+def test_syncfree_matches_level_schedule(bpf, gen, oracle, p):
+    # The optional single-launch sync-free factorization computes every row
+    # with the same operation order as the level-by-level launches (bitwise);
+    # the solves agree to rounding (the narrow-level kernel splits long rows
+    # over several wavefronts, which reorders their sums).
+    _, S = _system(gen, 2000, p, 23)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    X = bpf.HipMatrix.from_numpy(S.B)
+    out = {}
+    try:
+        for sf in (0, 1):
+            bpf.set_option("syncfree", sf)
+            P = bpf.ilu0(A, p)
+            Y = bpf.HipMatrix(S.B.shape)
+            bpf.ldiv_(Y, P, X)
+            out[sf] = (P.factor_values(), Y.to_numpy())
+    finally:
+        bpf.set_option("syncfree", 0)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-13, atol=1e-15 * np.abs(out[0][1]).max())
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    ref = oracle.ilu0_apply(S.indptr, S.indices, F, S.B)
+    np.testing.assert_allclose(out[1][1], ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
