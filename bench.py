@@ -469,14 +469,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block-width", type=int, default=0)
     ap.add_argument("--poly-degree", type=int, default=None,
-                    help="GMRES-polynomial degree of the mixed form (default 8 for c2/c3, 0 for c4)")
+                    help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default=None,
                     help="block Gram-Schmidt: bmgs (the reference's loop) or bcgs2 (two classical "
                          "passes); default bmgs for c2/c3, bcgs2 for c4")
     args = ap.parse_args()
     batched = args.workload == "c4"
     if args.poly_degree is None:
-        args.poly_degree = 0 if batched else 8
+        args.poly_degree = 0 if batched else 4
     if args.orth is None:
         args.orth = "bcgs2" if batched else "bmgs"
     if args.gpus != 1:

@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kB) void proj_reduce_kernel(const double *part, int
 constexpr int kGemmThreads = 512;
 
 template <typename TVV, typename TY, int S>
-__global__ __launch_bounds__(kGemmThreads) void block_gemm_kernel(GemmArgs a) {
+__global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a) {
   constexpr int RT = 128;           // rows per workgroup
   constexpr int KA = 16;            // inner chunk
   constexpr int T = S / 16;
@@ -472,75 +472,121 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
 // A 2S x 16 column tile `seg` (LDS, rows padded to 17) is multiplied by
 // Omega^T = I - Y T^T Y^T, with Y the unit lower-trapezoidal compact factor in
 // H (2S x S row-major) and T upper triangular (S x S row-major): m1 = Y^T seg,
-// m1 = T^T m1, seg -= Y m1, all on matrix cores by the workgroup's 4
-// wavefronts.  Only rows r < rmax of seg may be non-zero on entry.  Ends on a
-// barrier.
+// m1 = T^T m1, seg -= Y m1, on matrix cores by the workgroup's 8 wavefronts.
+// H and T stream through LDS (`hs`) in chunks of kWyRows(S) rows, the next
+// chunk prefetched into registers while the current one is used.  Only rows
+// r < rmax (S or 2S) of seg may be non-zero on entry.  Ends on a barrier.
+constexpr int kWyThreads = 512;
+__host__ __device__ constexpr int kWyRows(int S) { return S < 64 ? S : 64; }
+
 template <int S>
 __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, const double *__restrict__ T,
-                                              double (*seg)[17], double (*m1)[17], int rmax) {
-  constexpr int NT = S / 16, NU = (NT + 3) / 4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                              double (*seg)[17], double (*m1)[17], double (*hs)[S + 1],
+                                              int rmax) {
+  constexpr int NW = kWyThreads / 64;
+  constexpr int NT = S / 16, NU = (NT + NW - 1) / NW, CR = kWyRows(S);
+  constexpr int PER = (CR * S + kWyThreads - 1) / kWyThreads;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane >> 4, lc = lane & 15;
+  double pre[PER];
+  auto fetch = [&](const double *M, int r0) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * kWyThreads;
+      pre[q] = e < CR * S ? M[(size_t)(r0 + e / S) * S + (e % S)] : 0.0;
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * kWyThreads;
+      if (e < CR * S) hs[e / S][e % S] = pre[q];
+    }
+  };
+  // m1 = Y^T seg over rows r < rmax
   dbl4 acc[NU];
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int jt = wave + 4 * u;
-    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
-    if (jt < NT) {
-      const int j = jt * 16 + lc;
-#pragma unroll 8
-      for (int r0 = jt * 16; r0 < rmax; r0 += 4) {
-        const int r = r0 + lr;
-        const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
-        a4 = mfma4(av, seg[r][lc], a4);
+  for (int u = 0; u < NU; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  fetch(H, 0);
+  for (int r0 = 0; r0 < rmax; r0 += CR) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (r0 + CR < rmax) fetch(H, r0 + CR);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int jt = wave + NW * u;
+      if (jt < NT && jt * 16 < r0 + CR) {
+        const int j = jt * 16 + lc;
+#pragma unroll 4
+        for (int kk = 0; kk < CR; kk += 4) {
+          const int r = r0 + kk + lr;
+          const double h = hs[kk + lr][j];
+          acc[u] = mfma4(r == j ? 1.0 : (r > j ? h : 0.0), seg[r][lc], acc[u]);
+        }
       }
     }
-    acc[u] = a4;
   }
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
-    const int jt = wave + 4 * u;
+    const int jt = wave + NW * u;
     if (jt < NT)
 #pragma unroll
       for (int q = 0; q < 4; ++q) m1[jt * 16 + lr + 4 * q][lc] = acc[u][q];
   }
-  __syncthreads();
+  // m2 = T^T m1: M2[j][c] = sum_{q <= j} T[q][j] M1[q][c], T streamed by rows q
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int jt = wave + 4 * u;
-    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
-    if (jt < NT) {
-      const int j = jt * 16 + lc;
-#pragma unroll 8
-      for (int q0 = 0; q0 < jt * 16 + 16; q0 += 4) {
-        const int q = q0 + lr;
-        const double av = q <= j ? T[(size_t)q * S + j] : 0.0;
-        a4 = mfma4(av, m1[q][lc], a4);
+  for (int u = 0; u < NU; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  fetch(T, 0);
+  for (int q0 = 0; q0 < S; q0 += CR) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (q0 + CR < S) fetch(T, q0 + CR);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int jt = wave + NW * u;
+      if (jt < NT && q0 < jt * 16 + 16) {
+        const int j = jt * 16 + lc;
+#pragma unroll 4
+        for (int kk = 0; kk < CR; kk += 4) {
+          const int q = q0 + kk + lr;
+          acc[u] = mfma4(q <= j ? hs[kk + lr][j] : 0.0, m1[q][lc], acc[u]);
+        }
       }
     }
-    acc[u] = a4;
   }
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
-    const int jt = wave + 4 * u;
+    const int jt = wave + NW * u;
     if (jt < NT)
 #pragma unroll
       for (int q = 0; q < 4; ++q) m1[jt * 16 + lr + 4 * q][lc] = acc[u][q];
   }
-  __syncthreads();
-  for (int rt = wave; rt < 2 * S / 16; rt += 4) {
-    dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
-    const int r = rt * 16 + lc;
-    const int jend = min(S, rt * 16 + 16);
-#pragma unroll 8
-    for (int j0 = 0; j0 < jend; j0 += 4) {
-      const int j = j0 + lr;
-      const double av = r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
-      a4 = mfma4(av, m1[j][lc], a4);
-    }
+  // seg -= Y m2, Y streamed by rows; wavefront w < CR/16 owns row tile w of
+  // the chunk
+  fetch(H, 0);
+  for (int r0 = 0; r0 < 2 * S; r0 += CR) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (r0 + CR < 2 * S) fetch(H, r0 + CR);
+    if (wave < CR / 16) {
+      const int rt = r0 / 16 + wave;
+      dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+      const int r = rt * 16 + lc;
+      const int jend = min(S, rt * 16 + 16);
+#pragma unroll 4
+      for (int j0 = 0; j0 < jend; j0 += 4) {
+        const int j = j0 + lr;
+        const double h = hs[r - r0][j];
+        a4 = mfma4(r == j ? 1.0 : (r > j ? h : 0.0), m1[j][lc], a4);
+      }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) seg[rt * 16 + lr + 4 * q][lc] -= a4[q];
+      for (int q = 0; q < 4; ++q) seg[rt * 16 + lr + 4 * q][lc] -= a4[q];
+    }
   }
   __syncthreads();
 }
@@ -549,9 +595,10 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
 // block column [Psi_0; ...; Psi_{k+1}] and applies Omega_i^T, i = 0..k-1, to
 // rows [iS, iS + 2S): a two-block window slides down the column in LDS.
 template <int S>
-__global__ __launch_bounds__(256) void hess_apply_kernel(HessArgs a) {
+__global__ __launch_bounds__(kWyThreads) void hess_apply_kernel(HessArgs a) {
   __shared__ double seg[2 * S][17];
   __shared__ double m1[S][17];
+  __shared__ double hs[kWyRows(S)][S + 1];
   const int b0 = blockIdx.x * 16, g = blockIdx.y;
   const GroupDesc gd = a.groups[g];
   const int k = a.k;
@@ -559,7 +606,7 @@ __global__ __launch_bounds__(256) void hess_apply_kernel(HessArgs a) {
   const int tid = threadIdx.x;
   auto xfer = [&](int i, int half, bool load) {
     double *blk = hblock(a.cols, k, g, i, S);
-    for (int e = tid; e < S * 16; e += 256) {
+    for (int e = tid; e < S * 16; e += kWyThreads) {
       const int r = e >> 4, c = e & 15;
       double *gp = blk + (size_t)r * S + b0 + c;
       if (load) seg[half * S + r][c] = *gp;
@@ -570,11 +617,11 @@ __global__ __launch_bounds__(256) void hess_apply_kernel(HessArgs a) {
   xfer(1, 1, true);
   __syncthreads();
   for (int i = 0; i < k; ++i) {
-    wy_apply_tile<S>(hblock(a.cols, i, g, i, S), htmat(a.cols, i, g, a.ng, S), seg, m1, 2 * S);
+    wy_apply_tile<S>(hblock(a.cols, i, g, i, S), htmat(a.cols, i, g, a.ng, S), seg, m1, hs, 2 * S);
     if (i + 1 < k) {
       // block i is final; slide the window down by one block
       xfer(i, 0, false);
-      for (int e = tid; e < S * 16; e += 256) seg[e >> 4][e & 15] = seg[S + (e >> 4)][e & 15];
+      for (int e = tid; e < S * 16; e += kWyThreads) seg[e >> 4][e & 15] = seg[S + (e >> 4)][e & 15];
       __syncthreads();
       xfer(i + 2, 1, true);
       __syncthreads();
@@ -784,32 +831,34 @@ __global__ __launch_bounds__(1024) void hess_qr_kernel(HessQrArgs a) {
 // 16 columns, the residual estimates ||Lambda[:, c]|| of the new Lambda =
 // E[S:2S], then zfin(k) = E[0:S], E[0:S] = Lambda, E[S:2S] = 0.
 template <int S>
-__global__ __launch_bounds__(256) void hess_rot_kernel(HessQrArgs a) {
+__global__ __launch_bounds__(kWyThreads) void hess_rot_kernel(HessQrArgs a) {
   __shared__ double seg[2 * S][17];
   __shared__ double m1[S][17];
-  __shared__ double part[16][17];
+  __shared__ double hs[kWyRows(S)][S + 1];
+  constexpr int RG = kWyThreads / 16;
+  __shared__ double part[RG][17];
   const int b0 = blockIdx.x * 16, g = blockIdx.y;
   const GroupDesc gd = a.groups[g];
   if (b0 >= gd.w || !gd.active) return;
   const int tid = threadIdx.x;
   double *E = a.E + (size_t)g * 2 * S * S;
-  for (int e = tid; e < 2 * S * 16; e += 256) seg[e >> 4][e & 15] = E[(size_t)(e >> 4) * S + b0 + (e & 15)];
+  for (int e = tid; e < 2 * S * 16; e += kWyThreads) seg[e >> 4][e & 15] = E[(size_t)(e >> 4) * S + b0 + (e & 15)];
   __syncthreads();
-  wy_apply_tile<S>(hblock(a.cols, a.k, g, a.k, S), htmat(a.cols, a.k, g, a.ng, S), seg, m1, S);
+  wy_apply_tile<S>(hblock(a.cols, a.k, g, a.k, S), htmat(a.cols, a.k, g, a.ng, S), seg, m1, hs, S);
   {
     const int c = tid & 15, rg = tid >> 4;
     double sq = 0.0;
-    for (int r = S + rg; r < 2 * S; r += 16) sq += seg[r][c] * seg[r][c];
+    for (int r = S + rg; r < 2 * S; r += RG) sq += seg[r][c] * seg[r][c];
     part[rg][c] = sq;
   }
   __syncthreads();
   if (tid < 16 && b0 + tid < gd.w) {
     double sq = 0.0;
-    for (int q = 0; q < 16; ++q) sq += part[q][tid];
+    for (int q = 0; q < RG; ++q) sq += part[q][tid];
     a.rnorm[gd.slot + b0 + tid] = sqrt(sq);
   }
   double *zf = hzfin(a.cols, a.k, g, a.ng, S);
-  for (int e = tid; e < S * 16; e += 256) {
+  for (int e = tid; e < S * 16; e += kWyThreads) {
     const int r = e >> 4, c = e & 15;
     zf[(size_t)r * S + b0 + c] = seg[r][c];
     E[(size_t)r * S + b0 + c] = seg[S + r][c];
@@ -1060,10 +1109,10 @@ void launch_hess_apply(hipStream_t s, HessArgs a, int ng) {
   if (a.k <= 0) return;
   const dim3 grid(a.S / 16, ng);
   switch (a.S) {
-    case 16: hipLaunchKernelGGL(hess_apply_kernel<16>, grid, dim3(256), 0, s, a); break;
-    case 32: hipLaunchKernelGGL(hess_apply_kernel<32>, grid, dim3(256), 0, s, a); break;
-    case 64: hipLaunchKernelGGL(hess_apply_kernel<64>, grid, dim3(256), 0, s, a); break;
-    case 128: hipLaunchKernelGGL(hess_apply_kernel<128>, grid, dim3(256), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(hess_apply_kernel<16>, grid, dim3(kWyThreads), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(hess_apply_kernel<32>, grid, dim3(kWyThreads), 0, s, a); break;
+    case 64: hipLaunchKernelGGL(hess_apply_kernel<64>, grid, dim3(kWyThreads), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(hess_apply_kernel<128>, grid, dim3(kWyThreads), 0, s, a); break;
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", a.S);
   }
   BPF_CHECK_HIP(hipGetLastError());
@@ -1074,19 +1123,19 @@ void launch_hess_qr(hipStream_t s, HessQrArgs a, int ng) {
   switch (a.S) {
     case 16:
       hipLaunchKernelGGL(hess_qr_kernel<16>, dim3(ng), dim3(1024), 0, s, a);
-      hipLaunchKernelGGL(hess_rot_kernel<16>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<16>, grid, dim3(kWyThreads), 0, s, a);
       break;
     case 32:
       hipLaunchKernelGGL(hess_qr_kernel<32>, dim3(ng), dim3(1024), 0, s, a);
-      hipLaunchKernelGGL(hess_rot_kernel<32>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<32>, grid, dim3(kWyThreads), 0, s, a);
       break;
     case 64:
       hipLaunchKernelGGL(hess_qr_kernel<64>, dim3(ng), dim3(1024), 0, s, a);
-      hipLaunchKernelGGL(hess_rot_kernel<64>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<64>, grid, dim3(kWyThreads), 0, s, a);
       break;
     case 128:
       hipLaunchKernelGGL(hess_qr_kernel<128>, dim3(ng), dim3(1024), 0, s, a);
-      hipLaunchKernelGGL(hess_rot_kernel<128>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(hess_rot_kernel<128>, grid, dim3(kWyThreads), 0, s, a);
       break;
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", a.S);
   }
