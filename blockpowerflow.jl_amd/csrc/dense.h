@@ -109,6 +109,8 @@ struct BacksubArgs {
   int ng;
   double *Y;           // Y_i of group g at Y + (i*ng + g)*S*S
   const GroupDesc *groups;
+  double *Z;           // scratch, same layout as Y: the running right-hand side
+  double *Rinv;        // scratch, same layout as Y: inverses of the diagonal blocks
 };
 
 int pad_width(int w);

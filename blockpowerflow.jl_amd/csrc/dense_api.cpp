@@ -86,7 +86,10 @@ int bpf_block_hessenberg_lsq(bpf_ctx ctx, int p, int K, const double *h_H, int l
       if (h_rnorm) std::memcpy(h_rnorm + (size_t)k * p, rn.data(), p * sizeof(double));
     }
     Y.ensure((size_t)K * SS * sizeof(double));
-    BacksubArgs ba{tab.dev<double>(), K, S, 1, Y.as<double>(), dg};
+    DevBuf Zb, Rinvb;
+    Zb.ensure((size_t)K * SS * sizeof(double));
+    Rinvb.ensure((size_t)K * SS * sizeof(double));
+    BacksubArgs ba{tab.dev<double>(), K, S, 1, Y.as<double>(), dg, Zb.as<double>(), Rinvb.as<double>()};
     launch_backsub(s, ba, 1);
     std::vector<double> y((size_t)K * SS);
     BPF_CHECK_HIP(hipMemcpyAsync(y.data(), Y.ptr, y.size() * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -187,16 +187,17 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
 // out[i*o_istride + g*o_gstride + e] (+)= sum_chunk part[...]: each
 // workgroup owns 64 consecutive outputs; its 4 wavefronts split the chunks
 // and combine through LDS.
-__global__ __launch_bounds__(kB) void proj_reduce_kernel(const double *part, int nchunk, int nb,
-                                                         int ng, int SS, double *out,
-                                                         size_t o_istride, size_t o_gstride,
-                                                         int accumulate, const GroupDesc *groups,
-                                                         int stacked_rows) {
-  __shared__ double red[4][64];
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(64 * kRedWaves) void proj_reduce_kernel(const double *part, int nchunk, int nb,
+                                                                     int ng, int SS, double *out,
+                                                                     size_t o_istride, size_t o_gstride,
+                                                                     int accumulate, const GroupDesc *groups,
+                                                                     int stacked_rows) {
+  __shared__ double red[kRedWaves][64];
   const size_t total = (size_t)nb * ng * SS;
   const size_t e = (size_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int w = threadIdx.x >> 6;
-  double s = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   if (e < total) {
     int nc = nchunk;
     if (stacked_rows > 0) {
@@ -204,12 +205,21 @@ __global__ __launch_bounds__(kB) void proj_reduce_kernel(const double *part, int
       const GroupDesc gd = groups[(e / SS) % ng];
       nc = gd.active ? min(nchunk, (gd.nr + stacked_rows - 1) / stacked_rows) : 0;
     }
-    for (int c = w; c < nc; c += 4) s += part[(size_t)c * total + e];
+    // four independent sums keep several loads in flight per lane
+    int c = w;
+    for (; c + 3 * kRedWaves < nc; c += 4 * kRedWaves) {
+      s0 += part[(size_t)c * total + e];
+      s1 += part[(size_t)(c + kRedWaves) * total + e];
+      s2 += part[(size_t)(c + 2 * kRedWaves) * total + e];
+      s3 += part[(size_t)(c + 3 * kRedWaves) * total + e];
+    }
+    for (; c < nc; c += kRedWaves) s0 += part[(size_t)c * total + e];
   }
-  red[w][threadIdx.x & 63] = s;
+  red[w][threadIdx.x & 63] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   if (w == 0 && e < total) {
-    s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    double s = 0.0;
+    for (int q = 0; q < kRedWaves; ++q) s += red[q][threadIdx.x];
     const size_t ig = e / SS, el = e % SS;
     const size_t i = ig / ng, g = ig % ng;
     double *o = out + i * o_istride + g * o_gstride + el;
@@ -867,69 +877,106 @@ __global__ __launch_bounds__(kWyThreads) void hess_rot_kernel(HessQrArgs a) {
 }
 
 // ---------------------------------------------------- back substitution
-// Column b of group g: solve the block upper-triangular system with blocks
-// Rcol(j)[i] (block i of block column j, S x S row-major) for y = Zrhs(:, b).
-struct YView {
-  double *base;   // LDS array, or &Y[g-block 0][0][b]
-  int S, ng;
-  bool lds;
-  __device__ __forceinline__ double &operator[](int e) const {
-    return lds ? base[e] : base[((size_t)(e / S) * ng) * S * S + (size_t)(e % S) * S];
-  }
-};
-
-__global__ __launch_bounds__(kB) void backsub_kernel(BacksubArgs a, int use_lds) {
-  extern __shared__ double ys[];     // K*S
-  const int b = blockIdx.x, g = blockIdx.y;
-  const int S = a.S, K = min(a.K, a.groups[g].K);
+// Block upper-triangular solve R Y = Zfin (R_ij = block i of column region j;
+// R_jj is the upper triangle of the factored block).  Two stages:
+//  1. trinv_kernel, one workgroup per (j, g), all at once: Rinv_j = R_jj^{-1}
+//     (column-oriented back substitution in LDS) and Z_j = zfin(j);
+//  2. for j = K-1 .. 0, backsub_step_kernel: y_j = Rinv_j Z_j on matrix cores
+//     (written to Y_j by the row-j workgroups), and Z_i -= R_ij y_j for i < j
+//     by the other workgroups, each recomputing y_j for its 16-column tile.
+// A group uses its first min(K, K_g) block columns; its other Y blocks are 0.
+__global__ __launch_bounds__(1024) void trinv_kernel(BacksubArgs a) {
+  __shared__ double G[128 * 128];
+  __shared__ double dinv[128];
+  const int j = blockIdx.x, g = blockIdx.y;
+  const int S = a.S;
+  const int K = min(a.K, a.groups[g].K);
   const int w = a.groups[g].w;
-  if (b >= w) return;
-  // blocks K..a.K-1 of this group's Y are unused: zero them
-  for (int e = threadIdx.x; e < (a.K - K) * S; e += kB) {
-    const int i = K + e / S, r = e % S;
-    a.Y[((size_t)i * a.ng + g) * S * S + (size_t)r * S + b] = 0.0;
-  }
-  const YView y{use_lds ? ys : a.Y + (size_t)g * S * S + b, S, a.ng, use_lds != 0};
-  for (int e = threadIdx.x; e < K * S; e += kB) {
-    const int i = e / S, r = e % S;
-    y[e] = hzfin(a.cols, i, g, a.ng, S)[(size_t)r * S + b];
-  }
-  __syncthreads();
-  for (int j = K - 1; j >= 0; --j) {
-    const double *Rjj = hblock(a.cols, j, g, j, S);
-    // dense upper-triangular solve of block j by wavefront 0: lane q keeps
-    // y[q] and y[q+64] in registers; the pivot value travels by shuffle.
-    if (threadIdx.x < 64) {
-      const int q0 = threadIdx.x, q1 = threadIdx.x + 64;
-      double y0 = q0 < w ? y[j * S + q0] : 0.0, y1 = q1 < w ? y[j * S + q1] : 0.0;
-      for (int r = w - 1; r >= 0; --r) {
-        const int owner = r & 63;
-        double yr = __shfl(r < 64 ? y0 : y1, owner, 64);
-        yr /= Rjj[(size_t)r * S + r];
-        if (q0 == owner) { if (r < 64) y0 = yr; else y1 = yr; }
-        if (q0 < r) y0 -= Rjj[(size_t)q0 * S + r] * yr;
-        if (q1 < r) y1 -= Rjj[(size_t)q1 * S + r] * yr;
-      }
-      if (q0 < w) y[j * S + q0] = y0;
-      if (q1 < w) y[j * S + q1] = y1;
-    }
-    __syncthreads();
-    // y_i -= R_ij y_j for i < j
-    for (int e = threadIdx.x; e < j * S; e += kB) {
-      const int i = e / S, r = e % S;
-      if (r >= w) continue;
-      const double *Rij = hblock(a.cols, j, g, i, S);
-      double v = 0.0;
-      for (int q = 0; q < w; ++q) v += Rij[(size_t)r * S + q] * y[j * S + q];
-      y[e] -= v;
-    }
-    __syncthreads();
+  const size_t SS = (size_t)S * S;
+  double *Ri = a.Rinv + ((size_t)j * a.ng + g) * SS;
+  double *Z = a.Z + ((size_t)j * a.ng + g) * SS;
+  if (j >= K) return;
+  const double *zf = hzfin(a.cols, j, g, a.ng, S);
+  for (int e = threadIdx.x; e < S * S; e += blockDim.x) Z[e] = zf[e];
+  const double *D = hblock(a.cols, j, g, j, S);
+  for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+    const int r = e / w, c = e % w;
+    G[r * S + c] = c >= r ? D[(size_t)r * S + c] : 0.0;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < K * S; e += kB) {
-    const int i = e / S, r = e % S;
-    const double v = (r < w) ? y[e] : 0.0;
-    if (use_lds || r >= w) a.Y[((size_t)i * a.ng + g) * S * S + (size_t)r * S + b] = v;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) dinv[c] = 1.0 / G[c * S + c];
+  __syncthreads();
+  // column c of R^{-1}: x[c] = dinv[c]; x[r] (r < c) kept at G[c*S + r]
+  for (int c = threadIdx.x; c < w; c += blockDim.x) {
+    for (int r = c - 1; r >= 0; --r) {
+      double v = G[r * S + c] * dinv[c];
+      for (int m = r + 1; m < c; ++m) v += G[r * S + m] * G[c * S + m];
+      G[c * S + r] = -v * dinv[r];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
+    const int r = e / S, c = e % S;
+    double v = 0.0;
+    if (r < w && c < w) v = c > r ? G[c * S + r] : (c == r ? dinv[r] : 0.0);
+    Ri[e] = v;
+  }
+}
+
+// Grid (S/16 column tiles, j + 1 block rows, groups), 4 wavefronts.
+template <int S>
+__global__ __launch_bounds__(256) void backsub_step_kernel(BacksubArgs a, int j) {
+  constexpr int NT = S / 16, NU = (NT + 3) / 4;
+  __shared__ double ys[S][17];
+  const int t = blockIdx.x, i = blockIdx.y, g = blockIdx.z;
+  const GroupDesc gd = a.groups[g];
+  const int b0 = t * 16;
+  if (b0 >= gd.w) return;
+  const int K = min(a.K, gd.K);
+  const size_t SS = (size_t)S * S;
+  double *Yj = a.Y + ((size_t)j * a.ng + g) * SS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  if (j >= K) {
+    if (i == j)
+      for (int e = tid; e < S * 16; e += 256) Yj[(size_t)(e >> 4) * S + b0 + (e & 15)] = 0.0;
+    return;
+  }
+  // y = Rinv_j Z_j[:, tile]
+  const double *Ri = a.Rinv + ((size_t)j * a.ng + g) * SS;
+  const double *Zj = a.Z + ((size_t)j * a.ng + g) * SS;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int rt = wave + 4 * u;
+    if (rt < NT) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int r = rt * 16 + lc;
+#pragma unroll 8
+      for (int k0 = rt * 16; k0 < S; k0 += 4)      // Rinv_j is upper triangular
+        acc = mfma4(Ri[(size_t)r * S + k0 + lr], Zj[(size_t)(k0 + lr) * S + b0 + lc], acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ys[rt * 16 + lr + 4 * q][lc] = acc[q];
+    }
+  }
+  __syncthreads();
+  if (i == j) {
+    for (int e = tid; e < S * 16; e += 256) Yj[(size_t)(e >> 4) * S + b0 + (e & 15)] = ys[e >> 4][e & 15];
+    return;
+  }
+  // Z_i[:, tile] -= R_ij y
+  const double *Rij = hblock(a.cols, j, g, i, S);
+  double *Zi = a.Z + ((size_t)i * a.ng + g) * SS;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int rt = wave + 4 * u;
+    if (rt < NT) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int r = rt * 16 + lc;
+#pragma unroll 8
+      for (int k0 = 0; k0 < S; k0 += 4) acc = mfma4(Rij[(size_t)r * S + k0 + lr], ys[k0 + lr][lc], acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Zi[(size_t)(rt * 16 + lr + 4 * q) * S + b0 + lc] -= acc[q];
+    }
   }
 }
 
@@ -1051,7 +1098,7 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   const size_t total = (size_t)a.nb * a.ng * S * S;
-  hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(kB), 0, s,
+  hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64 * kRedWaves), 0, s,
                      a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
                      accumulate ? 1 : 0, a.groups, a.rows_sum > 0 ? a.chunk_rows : 0);
   BPF_CHECK_HIP(hipGetLastError());
@@ -1144,9 +1191,18 @@ void launch_hess_qr(hipStream_t s, HessQrArgs a, int ng) {
 
 void launch_backsub(hipStream_t s, BacksubArgs a, int ng) {
   if (a.K <= 0) return;
-  const size_t shm = (size_t)a.K * a.S * sizeof(double);
-  const int lds = shm <= kLdsLimit ? 1 : 0;
-  hipLaunchKernelGGL(backsub_kernel, dim3(a.S, ng), dim3(kB), lds ? shm : 0, s, a, lds);
+  BPF_REQUIRE(a.Z && a.Rinv, "back substitution needs its scratch blocks");
+  hipLaunchKernelGGL(trinv_kernel, dim3(a.K, ng), dim3(1024), 0, s, a);
+  for (int j = a.K - 1; j >= 0; --j) {
+    const dim3 grid(a.S / 16, j + 1, ng);
+    switch (a.S) {
+      case 16: hipLaunchKernelGGL(backsub_step_kernel<16>, grid, dim3(256), 0, s, a, j); break;
+      case 32: hipLaunchKernelGGL(backsub_step_kernel<32>, grid, dim3(256), 0, s, a, j); break;
+      case 64: hipLaunchKernelGGL(backsub_step_kernel<64>, grid, dim3(256), 0, s, a, j); break;
+      case 128: hipLaunchKernelGGL(backsub_step_kernel<128>, grid, dim3(256), 0, s, a, j); break;
+      default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", a.S);
+    }
+  }
   BPF_CHECK_HIP(hipGetLastError());
 }
 

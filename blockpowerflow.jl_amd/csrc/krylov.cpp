@@ -215,7 +215,9 @@ void BlockArnoldi::form(double *X, double alpha) {
   for (auto &g : h_groups_)
     if (g.active && g.K != K) { g.K = K; changed = true; }
   if (changed) upload_groups();
-  BacksubArgs ba{coltab_.dev<double>(), K, S_, ng_, Y_.as<double>(), dg};
+  Zb_.ensure((size_t)K * ng_ * SS * sizeof(double));
+  Rinvb_.ensure((size_t)K * ng_ * SS * sizeof(double));
+  BacksubArgs ba{coltab_.dev<double>(), K, S_, ng_, Y_.as<double>(), dg, Zb_.as<double>(), Rinvb_.as<double>()};
   launch_backsub(st_, ba, ng_);
   if (scale_) launch_scale_ycols(st_, Y_.as<double>(), K, ng_, S_, dg, scale_);
   const bool flex = cfg_.flexible && N_;

@@ -135,6 +135,7 @@ class BlockArnoldi {
   DevBuf E_;                   // current [Z_k; Lambda] per group
   DevBuf rnorm_;
   DevBuf Y_;                   // back substitution output
+  DevBuf Zb_, Rinvb_;          // back substitution scratch
   DevBuf &part_ = cq_.part;    // projection partial sums
   DevBuf &flags_ = cq_.flags;
   DevBuf Xtmp_;                // FP64 n x p scratch (non-flexible solution assembly)
