@@ -73,6 +73,12 @@ __global__ void csr_norm_cast_kernel(int n, const int *rowptr, const double *val
 // for S < 64 the wavefronts split the rows instead and are summed in LDS.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
+  // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
+  // B[l/16][l%16] and holds D[l/16 + 4q][l%16] in acc[q].
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+
 template <int S>
 struct TnShape {
   static constexpr int T = S / 16;                 // tiles per dimension
@@ -344,59 +350,114 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
 }
 
 // -------------------------------------------------------- Cholesky (FP64)
-// One workgroup per group: G (w x w, ld S) -> R upper (R^T R = G + shift I),
-// Rinv = R^{-1}.  If a pivot is not positive, restart with a shift
-// sigma = 11 (n w + w (w+1)) u ||G||_F (Fukaya et al. 2020) and flag it.
-__global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
-  __shared__ double sm[128 * 128];   // static LDS: up to 160 KiB on gfx950
+// One workgroup (16 wavefronts) per group: G (w x w, ld S) -> R upper with
+// R^T R = G + shift I, and Rinv = R^{-1}.  Blocked right-looking Cholesky in
+// LDS with 16-column panels: the diagonal block by one wavefront, the panel
+// row by forward substitution (one thread per column), the trailing update
+// on matrix cores.  If a pivot is not positive the factorization restarts
+// with a shift sigma = 11 (n w + w (w+1)) u ||G||_F (Fukaya et al. 2020) and
+// flags it.  R^{-1} is formed blockwise: diagonal 16 x 16 inverses, then
+// X_IJ = -X_II sum_{I<M<=J} R_IM X_MJ, one block column per wavefront; X is
+// kept transposed in the strict lower triangle with its diagonal in dinv.
+constexpr int kCholLd = 129;
+
+__device__ __forceinline__ double chol_xblk(const double *G, const double *dinv, int I, int J, int k, int n) {
+  // X_{IJ}[k][n] for I <= J (upper-triangular inverse, stored transposed)
+  const int r = I * 16 + k, c = J * 16 + n;
+  return r < c ? G[c * kCholLd + r] : (r == c ? dinv[r] : 0.0);
+}
+
+__global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
+  __shared__ double G[128 * kCholLd];
+  __shared__ double dinv[128];
+  __shared__ double scr[8][16][17];
+  __shared__ int s_fail;
+  __shared__ double s_norm;
   const int g = blockIdx.x;
   if (!a.groups[g].active) return;
   const int w = a.groups[g].w;
   const int S = a.S;
-  // w x w working copy (ld S) in LDS.
-  double *G = sm;
-  __shared__ int s_fail;
-  __shared__ double s_norm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
   const double *Gin = a.G + (size_t)g * a.gstride;
   double *R = a.R + (size_t)g * a.gstride;
   double *Ri = a.Rinv + (size_t)g * a.gstride;
-  if (threadIdx.x == 0) s_norm = 0.0;
+  if (tid == 0) s_norm = 0.0;
   __syncthreads();
   double loc = 0.0;
-  for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+  for (int e = tid; e < w * w; e += 1024) {
     const double v = Gin[(e / w) * S + e % w];
     loc += v * v;
   }
   for (int off = 32; off > 0; off >>= 1) loc += __shfl_xor(loc, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(&s_norm, loc);
+  if (lane == 0) atomicAdd(&s_norm, loc);
   __syncthreads();
   const double gnorm = sqrt(s_norm);
+  const int nb = (w + 15) / 16;
   double shift = 0.0;
   for (int attempt = 0; attempt < 4; ++attempt) {
-    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
-      const int r = e / w, c = e % w;
-      G[r * S + c] = Gin[r * S + c] + (r == c ? shift : 0.0);
+    for (int e = tid; e < 128 * kCholLd; e += 1024) {
+      const int r = e / kCholLd, c = e % kCholLd;
+      G[e] = (r < w && c < w) ? Gin[r * S + c] + (r == c ? shift : 0.0) : 0.0;
     }
-    if (threadIdx.x == 0) s_fail = 0;
+    if (tid == 0) s_fail = 0;
     __syncthreads();
-    // Right-looking Cholesky, upper factor stored in G's upper triangle.
-    for (int k = 0; k < w; ++k) {
-      const double d = G[k * S + k];
-      if (!(d > 0.0) || !isfinite(d)) {
-        if (threadIdx.x == 0) s_fail = 1;
+    for (int j0 = 0; j0 < w; j0 += 16) {
+      const int pw = min(16, w - j0);
+      // diagonal block by wavefront 0
+      if (wave == 0) {
+        for (int kk = 0; kk < pw; ++kk) {
+          const int k = j0 + kk;
+          const double d = G[k * kCholLd + k];
+          if (!(d > 0.0) || !isfinite(d)) {
+            if (lane == 0) s_fail = 1;
+            break;
+          }
+          const double rkk = sqrt(d);
+          if (lane > kk && lane < pw) G[k * kCholLd + j0 + lane] /= rkk;
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          if (lane == 0) G[k * kCholLd + k] = rkk;
+          for (int e = lane; e < 256; e += 64) {
+            const int i = e >> 4, j = e & 15;
+            if (i > kk && j >= i && j < pw)
+              G[(j0 + i) * kCholLd + j0 + j] -= G[k * kCholLd + j0 + i] * G[k * kCholLd + j0 + j];
+          }
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
       }
       __syncthreads();
       if (s_fail) break;
-      const double rkk = sqrt(d);
-      // row k: R[k][j] = G[k][j] / rkk for j > k
-      for (int j = k + 1 + threadIdx.x; j < w; j += blockDim.x) G[k * S + j] /= rkk;
+      // panel row: R[j0:j0+pw, c] = R_PP^{-T} G[j0:j0+pw, c] for trailing columns c
+      for (int c = j0 + pw + tid; c < w; c += 1024) {
+        for (int kk = 0; kk < pw; ++kk) {
+          const int k = j0 + kk;
+          double v = G[k * kCholLd + c];
+          for (int m = 0; m < kk; ++m) v -= G[(j0 + m) * kCholLd + k] * G[(j0 + m) * kCholLd + c];
+          G[k * kCholLd + c] = v / G[k * kCholLd + k];
+        }
+      }
       __syncthreads();
-      if (threadIdx.x == 0) G[k * S + k] = rkk;
-      // trailing update G[i][j] -= R[k][i] R[k][j], i <= j
-      const int m = w - k - 1;
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int i = k + 1 + e / m, j = k + 1 + e % m;
-        if (j >= i) G[i * S + j] -= G[k * S + i] * G[k * S + j];
+      // trailing update of the upper tiles: G[i][j] -= sum_k R[j0+k][i] R[j0+k][j]
+      const int t0 = j0 + 16;
+      const int nt = (w - t0 + 15) / 16;
+      const int ntiles = nt > 0 ? nt * (nt + 1) / 2 : 0;
+      for (int tt = wave; tt < ntiles; tt += 16) {
+        int ti = 0, rem = tt;
+        while (rem >= nt - ti) { rem -= nt - ti; ++ti; }
+        const int tj = ti + rem;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const int kr = (j0 + k0 + lr) * kCholLd;
+          acc = mfma4(G[kr + t0 + ti * 16 + lc], G[kr + t0 + tj * 16 + lc], acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = t0 + ti * 16 + lr + 4 * q, j = t0 + tj * 16 + lc;
+          if (i < w && j < w) G[i * kCholLd + j] -= acc[q];
+        }
       }
       __syncthreads();
     }
@@ -405,29 +466,63 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a, int use_lds) {
     shift = shift == 0.0 ? 11.0 * ((double)a.n * w + (double)w * (w + 1)) * 1.1102230246251565e-16 * gnorm
                          : shift * 100.0;
     if (shift == 0.0) shift = 1e-300;
-    if (threadIdx.x == 0) atomicOr(a.flags, 1);
+    if (tid == 0) atomicOr(a.flags, 1);
   }
-  // R = upper(G).  R^{-1} (upper) is formed in LDS, stored transposed in the
-  // strict lower triangle of G with its diagonal in dinv[], one column per
-  // thread, then both are written out.
-  __shared__ double dinv[128];
-  for (int c = threadIdx.x; c < w; c += blockDim.x) dinv[c] = 1.0 / G[c * S + c];
+  // diagonal 16 x 16 inverses (one thread per column)
+  for (int c = tid; c < w; c += 1024) dinv[c] = 1.0 / G[c * kCholLd + c];
   __syncthreads();
-  for (int c = threadIdx.x; c < w; c += blockDim.x) {
-    // column c of R^{-1}: x[r] for r = c..0, x[c] = dinv[c]; x[r] at G[c*S + r] (r < c)
-    for (int r = c - 1; r >= 0; --r) {
-      double v = G[r * S + c] * dinv[c];                // m = c term
-      for (int m = r + 1; m < c; ++m) v += G[r * S + m] * G[c * S + m];
-      G[c * S + r] = -v * dinv[r];
+  for (int c = tid; c < w; c += 1024) {
+    const int rb = (c / 16) * 16;
+    for (int r = c - 1; r >= rb; --r) {
+      double v = G[r * kCholLd + c] * dinv[c];
+      for (int m = r + 1; m < c; ++m) v += G[r * kCholLd + m] * G[c * kCholLd + m];
+      G[c * kCholLd + r] = -v * dinv[r];
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < S * S; e += blockDim.x) {
+  // off-diagonal blocks, block column J by wavefront J
+  if (wave < nb) {
+    const int J = wave;
+    double (*sw)[17] = scr[wave];
+    for (int I = J - 1; I >= 0; --I) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int M = I + 1; M <= J; ++M) {
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const int k = k0 + lr;
+          const int rr = I * 16 + lc, cc = M * 16 + k;
+          const double av = (rr < w && cc < w) ? G[rr * kCholLd + cc] : 0.0;
+          const double bv = (M * 16 + k < w && J * 16 + lc < w) ? chol_xblk(G, dinv, M, J, k, lc) : 0.0;
+          acc = mfma4(av, bv, acc);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = acc[q];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      dbl4 x4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) {
+        const int k = k0 + lr;
+        const double av = (I * 16 + lc < w && I * 16 + k < w) ? chol_xblk(G, dinv, I, I, lc, k) : 0.0;
+        x4 = mfma4(av, sw[k][lc], x4);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = I * 16 + lr + 4 * q, c = J * 16 + lc;
+        if (r < w && c < w) G[c * kCholLd + r] = -x4[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < S * S; e += 1024) {
     const int r = e / S, c = e % S;
     double rv, iv;
     if (r < w && c < w) {
-      rv = c >= r ? G[r * S + c] : 0.0;
-      iv = c > r ? G[c * S + r] : (c == r ? dinv[r] : 0.0);
+      rv = c >= r ? G[r * kCholLd + c] : 0.0;
+      iv = c > r ? G[c * kCholLd + r] : (c == r ? dinv[r] : 0.0);
     } else {
       rv = iv = (r == c) ? 1.0 : 0.0;
     }
@@ -471,12 +566,6 @@ __device__ __forceinline__ double *htmat(double *const *cols, int k, int g, int 
 // convention): Y[r][j] = 1 for r == j, H[r][j] for r > j, 0 above.
 __device__ __forceinline__ double yref(const double *H, int S, int r, int j) {
   return r == j ? 1.0 : (r > j ? H[(size_t)r * S + j] : 0.0);
-}
-
-__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
-  // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
-  // B[l/16][l%16] and holds D[l/16 + 4q][l%16] in acc[q].
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
 }
 
 // A 2S x 16 column tile `seg` (LDS, rows padded to 17) is multiplied by
@@ -1141,7 +1230,7 @@ template void launch_block_gemm<float, double>(hipStream_t, GemmArgs, int, int);
 
 void launch_chol(hipStream_t s, CholArgs a, int ng) {
   if (a.S > 128) BPF_THROW(BPF_E_UNSUPPORTED, "block width %d too large for Cholesky", a.S);
-  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a, 1);
+  hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a);
   BPF_CHECK_HIP(hipGetLastError());
 }
 
