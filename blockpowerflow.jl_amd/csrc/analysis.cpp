@@ -21,24 +21,35 @@ std::vector<Seg> make_plan(const TriOp &op, int tail_rows) {
 }
 
 void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
-                 const std::vector<int> &colidx, int kind, hipStream_t s) {
-  op.n = n;
-  op.unit = (kind == 0);
+                 const std::vector<int> &colidx, int kind, hipStream_t s, const TriWindow &win) {
+  const int row_lo = win.row_lo, row_hi = win.row_hi < 0 ? n : win.row_hi;
+  const int col_lo = win.col_lo, col_hi = win.col_hi < 0 ? n : win.col_hi;
+  BPF_REQUIRE(kind != 3 || (row_lo == 0 && row_hi == n && col_lo == 0 && col_hi == n),
+              "windowed transpose operator");
+  op.unit = (kind == 0) || win.unit;
   op.lower = (kind == 0 || kind == 2);
+  auto keep = [&](int i, int j) {
+    if (j < col_lo || j >= col_hi) return false;
+    if (kind == 0 || kind == 2) return j < i;
+    if (kind == 1) return j > i;
+    return j < i;
+  };
   // Per-row entry lists (positions into the combined array) and diagonals.
   std::vector<int> dpos(n, -1);
   std::vector<int> cnt(n, 0);
   for (int i = 0; i < n; ++i) {
+    const bool in_rows = kind == 3 || (i >= row_lo && i < row_hi);
+    if (!in_rows) continue;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
       const int j = colidx[k];
       if (j == i) dpos[i] = k;
-      if (kind == 0 || kind == 2) { if (j < i) ++cnt[i]; }
-      else if (kind == 1) { if (j > i) ++cnt[i]; }
-      else { if (j < i) ++cnt[j]; }   // transpose: entry (i,j) -> row j
+      if (!keep(i, j)) continue;
+      if (kind == 3) ++cnt[j];   // transpose: entry (i,j) -> row j
+      else ++cnt[i];
     }
   }
-  if (kind != 0)
-    for (int i = 0; i < n; ++i)
+  if (!op.unit)
+    for (int i = row_lo; i < row_hi; ++i)
       if (dpos[i] < 0) BPF_THROW(BPF_E_SINGULAR, "structurally zero diagonal in row %d", i);
   // Row lists in original numbering (CSR of the operator).
   std::vector<int> optr(n + 1, 0);
@@ -46,45 +57,56 @@ void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
   std::vector<int> ocol(optr[n]), opos(optr[n]);
   std::vector<int> fill(optr.begin(), optr.end() - 1);
   for (int i = 0; i < n; ++i) {
+    const bool in_rows = kind == 3 || (i >= row_lo && i < row_hi);
+    if (!in_rows) continue;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
       const int j = colidx[k];
-      if (kind == 0 || kind == 2) {
-        if (j < i) { ocol[fill[i]] = j; opos[fill[i]++] = k; }
-      } else if (kind == 1) {
-        if (j > i) { ocol[fill[i]] = j; opos[fill[i]++] = k; }
-      } else if (j < i) {
-        ocol[fill[j]] = i; opos[fill[j]++] = k;
-      }
+      if (!keep(i, j)) continue;
+      if (kind == 3) { ocol[fill[j]] = i; opos[fill[j]++] = k; }
+      else { ocol[fill[i]] = j; opos[fill[i]++] = k; }
     }
   }
-  // Levels.
-  std::vector<int> level(n, 0);
+  // Rows of the operator: the window's rows (all rows for a transpose),
+  // optionally without rows that have no entries.
+  std::vector<char> member(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const bool in_rows = kind == 3 || (i >= row_lo && i < row_hi);
+    member[i] = in_rows && !(win.skip_empty && cnt[i] == 0);
+  }
+  // Levels; dependencies outside the operator's rows are already solved.
+  std::vector<int> level(n, -1);
   int nlev = 0;
+  auto lev = [&](int i) {
+    int lv = 0;
+    for (int k = optr[i]; k < optr[i + 1]; ++k)
+      if (member[ocol[k]]) lv = std::max(lv, level[ocol[k]] + 1);
+    level[i] = lv;
+    nlev = std::max(nlev, lv + 1);
+  };
   if (op.lower) {
-    for (int i = 0; i < n; ++i) {
-      int lv = 0;
-      for (int k = optr[i]; k < optr[i + 1]; ++k) lv = std::max(lv, level[ocol[k]] + 1);
-      level[i] = lv;
-      nlev = std::max(nlev, lv + 1);
-    }
+    for (int i = 0; i < n; ++i)
+      if (member[i]) lev(i);
   } else {
-    for (int i = n - 1; i >= 0; --i) {
-      int lv = 0;
-      for (int k = optr[i]; k < optr[i + 1]; ++k) lv = std::max(lv, level[ocol[k]] + 1);
-      level[i] = lv;
-      nlev = std::max(nlev, lv + 1);
-    }
+    for (int i = n - 1; i >= 0; --i)
+      if (member[i]) lev(i);
   }
-  if (n == 0) nlev = 0;
+  int nrows = 0;
+  for (int i = 0; i < n; ++i) nrows += member[i];
+  op.n = nrows;
+  op.ndim = n;
+  if (nrows == 0) nlev = 0;
   op.nlevels = nlev;
   op.h_lvl.assign(nlev + 1, 0);
-  for (int i = 0; i < n; ++i) op.h_lvl[level[i] + 1]++;
+  for (int i = 0; i < n; ++i)
+    if (member[i]) op.h_lvl[level[i] + 1]++;
   for (int l = 0; l < nlev; ++l) op.h_lvl[l + 1] += op.h_lvl[l];
-  op.h_order.assign(n, 0);
+  op.h_order.assign(nrows, 0);
   {
     std::vector<int> f(op.h_lvl.begin(), op.h_lvl.end() - 1);
-    for (int i = 0; i < n; ++i) op.h_order[f[level[i]]++] = i;
+    for (int i = 0; i < n; ++i)
+      if (member[i]) op.h_order[f[level[i]]++] = i;
   }
+  n = nrows;
   op.h_ptr.assign(n + 1, 0);
   op.max_row_nnz = 0;
   for (int t = 0; t < n; ++t) {
@@ -116,7 +138,7 @@ void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
   op.val32.ensure(m * sizeof(float));
   op.diag.ensure(std::max(1, n) * sizeof(double));
   op.diag32.ensure(std::max(1, n) * sizeof(float));
-  upload(op.flags, std::vector<int>(std::max(1, n), 0), s);
+  upload(op.flags, std::vector<int>(std::max(1, op.ndim), 0), s);
   upload(op.status, std::vector<int>(1, 0), s);
   op.epoch = 0;
 }

@@ -120,7 +120,8 @@ namespace bpf {
 // maps stored entry -> position in the source value array (combined factor),
 // `dgather` maps stored row -> position of its diagonal.
 struct TriOp {
-  int n = 0;
+  int n = 0;                   // rows of the operator (entries of order)
+  int ndim = 0;                // dimension of the vectors it acts on
   bool unit = true;            // implicit unit diagonal
   bool lower = true;           // dependencies on smaller (true) / larger indices
   int nlevels = 0;
@@ -153,8 +154,17 @@ std::vector<Seg> make_plan(const TriOp &op, int tail_rows);
 // Builds a TriOp from a CSR pattern.  kind: 0 = strict lower part (unit diag),
 // 1 = upper part (diag separate), 2 = lower part with diag separate,
 // 3 = transpose of the lower part (upper operator, diag separate).
+// A window restricts the operator to rows [row_lo, row_hi) and entries with
+// columns in [col_lo, col_hi) (-1 = n); dependencies on rows outside it are
+// taken as solved.  unit: no diagonal scaling; skip_empty: rows without
+// entries are left out (their y is not written).
+struct TriWindow {
+  int row_lo = 0, row_hi = -1, col_lo = 0, col_hi = -1;
+  bool unit = false, skip_empty = false;
+};
 void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
-                 const std::vector<int> &colidx, int kind, hipStream_t s);
+                 const std::vector<int> &colidx, int kind, hipStream_t s,
+                 const TriWindow &win = TriWindow());
 // Copies values (FP64 and FP32) from a combined factor array on the device.
 void triop_refresh(TriOp &op, const double *d_fac, hipStream_t s);
 
@@ -193,6 +203,10 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
 // Numeric IKJ factorization on a combined pattern (ILU0 when the pattern is
 // A's, exact LU when it is the full fill pattern).  The values must already
 // hold the (permuted) matrix entries.  Uses the lower TriOp's levels.
+// Rows [r0, r1), independent of each other, eliminated against the (final)
+// pivot rows k < kmax only.
+void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
+                           const int *diagpos, double *fac, int max_row_nnz, int *d_status);
 void launch_ilu_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                        const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                        int *d_status);

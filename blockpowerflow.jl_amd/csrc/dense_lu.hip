@@ -1,0 +1,292 @@
+// Dense tail of a fixed-pivot sparse LU refactorization.
+//
+// With a fill-reducing ordering the last rows of the factor form a deep,
+// narrow elimination tail (one long row per dependency level).  Eliminated
+// row by row it is a long sequential chain; held as a dense m x m Schur
+// complement S it is a blocked right-looking LU (no pivoting: the pivots are
+// the ones fixed by the host analysis) whose updates are GEMMs on matrix
+// cores, and the matching triangular solves are blocked the same way:
+//
+//   for each nb-column panel j0 of S:
+//     dlu_diag_kernel   LU of the nb x nb diagonal block in LDS (16-column
+//                       sub-panels, MFMA updates) + inverses of its L and U
+//     dgemm_kernel      P = S21 U11^{-1},  Q = L11^{-1} S12   (scratch)
+//     copy              S21 <- P, S12 <- Q
+//     dgemm_kernel      S22 -= P Q
+//
+//   solves (m x p right-hand sides, FP64): Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b,
+//   then Z_b = U_bb^{-1} Y_b, Y_{<b} -= U_{<b,b} Z_b.
+#include "dense_lu.h"
+
+#include <algorithm>
+
+namespace bpf {
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
+  // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
+  // B[l/16][l%16] and holds D[l/16 + 4q][l%16] in acc[q].
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+
+// C (M x N) = alpha A (M x K) B (K x N) (+ C when ACCUM), row-major FP64.
+// 64 x 64 output tile per workgroup (4 wavefronts of 16 x 64), K in chunks
+// of 16 staged in LDS.  C must not overlap A or B.
+template <bool ACCUM>
+__global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double alpha, const double *A,
+                                                    int lda, const double *B, int ldb, double *C, int ldc) {
+  __shared__ double As[64][17];
+  __shared__ double Bs[16][65];
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
+      const int r = e >> 4, k = e & 15;
+      As[r][k] = (m0 + r < M && k0 + k < K) ? A[(size_t)(m0 + r) * lda + k0 + k] : 0.0;
+      const int rb = e >> 6, c = e & 63;
+      Bs[rb][c] = (k0 + rb < K && n0 + c < N) ? B[(size_t)(k0 + rb) * ldb + n0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < 4; ++k4) {
+      const double a = As[wave * 16 + lc][k4 * 4 + lr];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma4(a, Bs[k4 * 4 + lr][t * 16 + lc], acc[t]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = m0 + wave * 16 + lr + 4 * q, c = n0 + t * 16 + lc;
+      if (r < M && c < N) {
+        double *cp = C + (size_t)r * ldc + c;
+        *cp = ACCUM ? *cp + alpha * acc[t][q] : alpha * acc[t][q];
+      }
+    }
+}
+
+constexpr int kLd = kDluNb + 1;
+
+// LU without pivoting of the w x w block at S (ld lds), w <= kDluNb, in LDS:
+// 16-column sub-panels; the sub-panel's 16 x 16 diagonal block by one
+// wavefront, its L column block and U row block by substitution (one thread
+// per row / column), the trailing update on matrix cores.  Then U^{-1} and
+// L^{-1} (unit) by column-oriented substitution, one thread per column.
+// A zero pivot at block row r sets *status = min(*status, base + r).
+__global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int w, double *Linv, double *Uinv,
+                                                        int *status, int base) {
+  __shared__ double G[kDluNb * kLd];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  for (int e = tid; e < kDluNb * kLd; e += 1024) {
+    const int r = e / kLd, c = e % kLd;
+    G[e] = (r < w && c < w) ? S[(size_t)r * lds + c] : 0.0;
+  }
+  __syncthreads();
+  for (int p0 = 0; p0 < w; p0 += 16) {
+    const int pw = min(16, w - p0);
+    if (wave == 0) {
+      for (int kk = 0; kk < pw; ++kk) {
+        const int k = p0 + kk;
+        double d = G[k * kLd + k];
+        if (d == 0.0) {
+          if (lane == 0) atomicMin(status, base + k);
+          d = 1.0;
+          if (lane == 0) G[k * kLd + k] = 1.0;
+        }
+        if (lane > kk && lane < pw) G[(p0 + lane) * kLd + k] /= d;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int e = lane; e < 256; e += 64) {
+          const int i = e >> 4, j = e & 15;
+          if (i > kk && j > kk && i < pw && j < pw)
+            G[(p0 + i) * kLd + p0 + j] -= G[(p0 + i) * kLd + k] * G[k * kLd + p0 + j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      }
+    }
+    __syncthreads();
+    // L column block: rows r >= p0 + pw, G[r][p0:p0+pw] <- G[r][p0:p0+pw] U_PP^{-1}
+    for (int r = p0 + pw + tid; r < w; r += 1024) {
+      for (int kk = 0; kk < pw; ++kk) {
+        const int k = p0 + kk;
+        double v = G[r * kLd + k];
+        for (int mm = 0; mm < kk; ++mm) v -= G[r * kLd + p0 + mm] * G[(p0 + mm) * kLd + k];
+        G[r * kLd + k] = v / G[k * kLd + k];
+      }
+    }
+    // U row block: columns c >= p0 + pw, G[p0:p0+pw][c] <- L_PP^{-1} G[p0:p0+pw][c]
+    for (int c = p0 + pw + tid; c < w; c += 1024) {
+      for (int kk = 0; kk < pw; ++kk) {
+        const int k = p0 + kk;
+        double v = G[k * kLd + c];
+        for (int mm = 0; mm < kk; ++mm) v -= G[k * kLd + p0 + mm] * G[(p0 + mm) * kLd + c];
+        G[k * kLd + c] = v;
+      }
+    }
+    __syncthreads();
+    // trailing update G[r][c] -= sum_k L[r][p0+k] U[p0+k][c], r, c >= p0 + 16
+    const int t0 = p0 + 16;
+    const int nt = (w - t0 + 15) / 16;
+    for (int tt = wave; tt < nt * nt; tt += 16) {
+      const int ti = tt / nt, tj = tt % nt;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) {
+        const int k = p0 + k0 + lr;
+        acc = mfma4(G[(t0 + ti * 16 + lc) * kLd + k], G[k * kLd + t0 + tj * 16 + lc], acc);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = t0 + ti * 16 + lr + 4 * q, j = t0 + tj * 16 + lc;
+        if (i < w && j < w) G[i * kLd + j] -= acc[q];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < w * w; e += 1024) {
+    const int r = e / w, c = e % w;
+    S[(size_t)r * lds + c] = G[r * kLd + c];
+  }
+  __syncthreads();
+  // U^{-1}, column c by thread c: x[c] = 1/U[c][c],
+  // x[r] = -(sum_{r<mm<=c} U[r][mm] x[mm]) / U[r][r]; x[r] (r < c) is kept at
+  // G[c][r] (the strict lower part, whose L is already saved in S).
+  if (tid < w) {
+    const int c = tid;
+    const double dc = 1.0 / G[c * kLd + c];
+    for (int r = c - 1; r >= 0; --r) {
+      double v = G[r * kLd + c] * dc;
+      for (int mm = r + 1; mm < c; ++mm) v += G[r * kLd + mm] * G[c * kLd + mm];
+      G[c * kLd + r] = -v / G[r * kLd + r];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kDluNb * kDluNb; e += 1024) {
+    const int r = e / kDluNb, c = e % kDluNb;
+    double v = 0.0;
+    if (r < w && c < w) v = r < c ? G[c * kLd + r] : (r == c ? 1.0 / G[r * kLd + r] : 0.0);
+    Uinv[e] = v;
+  }
+  __syncthreads();
+  // L^{-1} (unit lower): L moves transposed into the strict upper part
+  // (G[mm][r] = L[r][mm], r > mm), then column c by thread c: x[c] = 1,
+  // x[r] = -sum_{c<=mm<r} L[r][mm] x[mm], x[r] (r > c) kept at G[r][c].
+  for (int e = tid; e < w * w; e += 1024) {
+    const int r = e / w, mm = e % w;
+    if (r > mm) G[mm * kLd + r] = S[(size_t)r * lds + mm];
+  }
+  __syncthreads();
+  if (tid < w) {
+    const int c = tid;
+    for (int r = c + 1; r < w; ++r) {
+      double v = G[c * kLd + r];   // mm = c term (x[c] = 1)
+      for (int mm = c + 1; mm < r; ++mm) v += G[mm * kLd + r] * G[mm * kLd + c];
+      G[r * kLd + c] = -v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kDluNb * kDluNb; e += 1024) {
+    const int r = e / kDluNb, c = e % kDluNb;
+    double v = 0.0;
+    if (r < w && c < w) v = r > c ? G[r * kLd + c] : (r == c ? 1.0 : 0.0);
+    Linv[e] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_block_kernel(int rows, int cols, const double *src, int lds,
+                                                         double *dst, int ldd) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)rows * cols) return;
+  const int r = (int)(e / cols), c = (int)(e % cols);
+  dst[(size_t)r * ldd + c] = src[(size_t)r * lds + c];
+}
+
+__global__ __launch_bounds__(256) void tail_move_kernel(int cnt, const int *pos, const int *didx, double *fac,
+                                                        double *S, int to_dense) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= cnt) return;
+  if (to_dense) S[didx[e]] = fac[pos[e]];
+  else fac[pos[e]] = S[didx[e]];
+}
+
+void gemm(hipStream_t s, bool accum, int M, int N, int K, double alpha, const double *A, int lda,
+          const double *B, int ldb, double *C, int ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  const dim3 grid((N + 63) / 64, (M + 63) / 64);
+  if (accum) hipLaunchKernelGGL(dgemm_kernel<true>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+  else hipLaunchKernelGGL(dgemm_kernel<false>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+}
+
+void copy_block(hipStream_t s, int rows, int cols, const double *src, int lds, double *dst, int ldd) {
+  const size_t m = (size_t)rows * cols;
+  if (m == 0) return;
+  hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, rows, cols, src,
+                     lds, dst, ldd);
+}
+
+}  // namespace
+
+void dense_tail_move(hipStream_t s, int cnt, const int *pos, const int *didx, double *fac, double *S,
+                     bool to_dense) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(tail_move_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, pos, didx, fac, S,
+                     to_dense ? 1 : 0);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void dense_lu_factor(hipStream_t s, int m, double *S, double *Linv, double *Uinv, double *P, double *Q,
+                     int *status, int base) {
+  const int nb = kDluNb;
+  for (int j0 = 0, b = 0; j0 < m; j0 += nb, ++b) {
+    const int w = std::min(nb, m - j0);
+    const int rest = m - j0 - w;
+    double *S11 = S + (size_t)j0 * m + j0;
+    double *Li = Linv + (size_t)b * nb * nb, *Ui = Uinv + (size_t)b * nb * nb;
+    hipLaunchKernelGGL(dlu_diag_kernel, dim3(1), dim3(1024), 0, s, S11, m, w, Li, Ui, status, base + j0);
+    if (rest > 0) {
+      double *S21 = S + (size_t)(j0 + w) * m + j0, *S12 = S11 + w, *S22 = S21 + w;
+      gemm(s, false, rest, w, w, 1.0, S21, m, Ui, nb, P, nb);    // P = S21 U11^{-1}
+      gemm(s, false, w, rest, w, 1.0, Li, nb, S12, m, Q, m);     // Q = L11^{-1} S12
+      copy_block(s, rest, w, P, nb, S21, m);
+      copy_block(s, w, rest, Q, m, S12, m);
+      gemm(s, true, rest, rest, w, -1.0, P, nb, Q, m, S22, m);   // S22 -= P Q
+    }
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void dense_lu_solve(hipStream_t s, int m, int p, const double *S, const double *Linv, const double *Uinv,
+                    double *X, double *Y) {
+  // Forward: Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b.
+  const int nb = kDluNb;
+  const int nblk = (m + nb - 1) / nb;
+  for (int b = 0; b < nblk; ++b) {
+    const int j0 = b * nb, w = std::min(nb, m - j0), rest = m - j0 - w;
+    gemm(s, false, w, p, w, 1.0, Linv + (size_t)b * nb * nb, nb, X + (size_t)j0 * p, p, Y + (size_t)j0 * p, p);
+    if (rest > 0)
+      gemm(s, true, rest, p, w, -1.0, S + (size_t)(j0 + w) * m + j0, m, Y + (size_t)j0 * p, p,
+           X + (size_t)(j0 + w) * p, p);
+  }
+  // Backward: X_b = U_bb^{-1} Y_b, Y_{<b} -= U_{<b,b} X_b.
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int j0 = b * nb, w = std::min(nb, m - j0);
+    gemm(s, false, w, p, w, 1.0, Uinv + (size_t)b * nb * nb, nb, Y + (size_t)j0 * p, p, X + (size_t)j0 * p, p);
+    if (j0 > 0)
+      gemm(s, true, j0, p, w, -1.0, S + j0, m, X + (size_t)j0 * p, p, Y, p);
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace bpf

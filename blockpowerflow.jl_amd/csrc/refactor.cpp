@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "dense_lu.h"
 #include "krylov.h"
 #include "ordering.h"
 
@@ -45,6 +46,17 @@ struct bpf_rf_s {
   DevBuf facb;                         // batch: nnzF x batch factor values
   DevBuf w1, w2, w3;                   // work blocks
   DevBuf status, stat;
+  // Dense tail (single handles with n >= kDenseMinN): rows/columns [t, n)
+  // of B = A[P, Q] are refactored and solved as a dense m x m block.
+  int t = 0, m = 0, ntail = 0;
+  TriOp lo_h, up_h;                    // head operators (rows and columns < t)
+  TriOp l21, u12;                      // couplings: tail rows x head columns, head rows x tail columns
+  DevBuf S, Linv, Uinv, P, Q;          // dense tail factor, diagonal-block inverses, scratch
+  DevBuf tpos, tdidx;                  // tail entries: factor position, dense index
+  DevBuf Xd, Yd;                       // FP64 m x p solve blocks
+  bool dense() const { return m > 0; }
+  TriOp &lop() { return dense() ? lo_h : lo; }
+  TriOp &uop() { return dense() ? up_h : up; }
 };
 
 namespace {
@@ -55,23 +67,58 @@ void check_pattern(bpf_rf rf, bpf_csr A) {
               "matrix pattern differs from the analysed pattern");
 }
 
-// fac <- values of A scattered into the L\U pattern, then IKJ elimination.
+// fac <- values of A scattered into the L\U pattern, then IKJ elimination
+// (with a dense tail: IKJ on the head rows, the tail rows eliminated against
+// the head only, then a dense LU of their Schur complement).
 void refactor_into(bpf_rf rf, const double *d_aval, double *fac) {
   hipStream_t s = rf->ctx->stream;
   fill_double(s, fac, 0.0, rf->nnzF);
   launch_scatter_values(s, rf->nnzA, rf->amap.as<int>(), d_aval, fac);
   rf->status.ensure(16);
   fill_int(s, rf->status.as<int>(), rf->n, 1);
-  launch_ilu_factor(s, rf->lo, rf->n, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(), fac,
+  launch_ilu_factor(s, rf->lop(), rf->n, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(), fac,
                     rf->max_row, rf->status.as<int>());
+  if (rf->dense()) {
+    launch_partial_factor(s, rf->t, rf->n, rf->t, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(),
+                          fac, rf->max_row, rf->status.as<int>());
+    double *S = rf->S.as<double>();
+    fill_double(s, S, 0.0, (size_t)rf->m * rf->m);
+    dense_tail_move(s, rf->ntail, rf->tpos.as<int>(), rf->tdidx.as<int>(), fac, S, true);
+    dense_lu_factor(s, rf->m, S, rf->Linv.as<double>(), rf->Uinv.as<double>(), rf->P.as<double>(),
+                    rf->Q.as<double>(), rf->status.as<int>(), rf->t);
+    dense_tail_move(s, rf->ntail, rf->tpos.as<int>(), rf->tdidx.as<int>(), fac, S, false);
+  }
   int st = 0;
   BPF_CHECK_HIP(hipMemcpyAsync(&st, rf->status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   BPF_CHECK_HIP(hipStreamSynchronize(s));
-  triop_check(rf->lo, s);
+  triop_check(rf->lop(), s);
   if (st < rf->n) BPF_THROW(BPF_E_SINGULAR, "refactorization: zero pivot at step %d", st);
 }
 
-void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf) {
+// Refreshes the triangular operators' value copies from the factor values.
+void refresh_ops(bpf_rf rf, const double *fac) {
+  hipStream_t s = rf->ctx->stream;
+  triop_refresh(rf->lop(), fac, s);
+  triop_refresh(rf->uop(), fac, s);
+  if (rf->dense()) {
+    triop_refresh(rf->l21, fac, s);
+    triop_refresh(rf->u12, fac, s);
+  }
+}
+
+void check_ops(bpf_rf rf) {
+  hipStream_t s = rf->ctx->stream;
+  triop_check(rf->lop(), s);
+  triop_check(rf->uop(), s);
+  if (rf->dense()) {
+    triop_check(rf->l21, s);
+    triop_check(rf->u12, s);
+  }
+}
+
+constexpr int kDenseMinN = 16384;
+
+void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool allow_dense) {
   rf->ctx = ctx;
   rf->n = A->n;
   rf->nnzA = A->nnz;
@@ -125,15 +172,77 @@ void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf) {
   upload(rf->fdiag, rf->h_fdiag, s);
   upload(rf->amap, amap, s);
   rf->fac.ensure(std::max(1, rf->nnzF) * sizeof(double));
-  build_triop(rf->lo, n, lu.frow, lu.fcol, 0, s);
-  build_triop(rf->up, n, lu.frow, lu.fcol, 1, s);
+  // Dense tail for large single systems: the last m rows (m a multiple of
+  // the dense block width, up to 32 blocks, about n/16).
+  if (allow_dense && n >= kDenseMinN) {
+    const int nb = kDluNb;
+    rf->m = std::min(32 * nb, (n / 16) / nb * nb);
+  }
+  if (!rf->dense()) {
+    build_triop(rf->lo, n, lu.frow, lu.fcol, 0, s);
+    build_triop(rf->up, n, lu.frow, lu.fcol, 1, s);
+    return;
+  }
+  const int m = rf->m, t = n - m;
+  rf->t = t;
+  TriWindow head;
+  head.row_hi = t;
+  head.col_hi = t;
+  build_triop(rf->lo_h, n, lu.frow, lu.fcol, 0, s, head);
+  build_triop(rf->up_h, n, lu.frow, lu.fcol, 1, s, head);
+  TriWindow w21;
+  w21.row_lo = t;
+  w21.col_hi = t;
+  w21.unit = true;
+  w21.skip_empty = true;
+  build_triop(rf->l21, n, lu.frow, lu.fcol, 0, s, w21);
+  TriWindow w12;
+  w12.row_hi = t;
+  w12.col_lo = t;
+  w12.unit = true;
+  w12.skip_empty = true;
+  build_triop(rf->u12, n, lu.frow, lu.fcol, 1, s, w12);
+  std::vector<int> tpos, tdidx;
+  for (int i = t; i < n; ++i)
+    for (int k = lu.frow[i]; k < lu.frow[i + 1]; ++k)
+      if (lu.fcol[k] >= t) {
+        tpos.push_back(k);
+        tdidx.push_back((i - t) * m + (lu.fcol[k] - t));
+      }
+  rf->ntail = (int)tpos.size();
+  upload(rf->tpos, tpos, s);
+  upload(rf->tdidx, tdidx, s);
+  const int nblk = (m + kDluNb - 1) / kDluNb;
+  rf->S.ensure((size_t)m * m * sizeof(double));
+  rf->Linv.ensure((size_t)nblk * kDluNb * kDluNb * sizeof(double));
+  rf->Uinv.ensure((size_t)nblk * kDluNb * kDluNb * sizeof(double));
+  rf->P.ensure((size_t)m * kDluNb * sizeof(double));
+  rf->Q.ensure((size_t)m * kDluNb * sizeof(double));
 }
 
-// Row-major solve of B z = y with the current triangular operators.
+// Row-major solve of B z = y with the current triangular operators.  With a
+// dense tail: head forward sweep, tail rows minus their head couplings, dense
+// tail solve (FP64), head rows minus their tail couplings, head backward sweep.
 template <typename TV>
 void tri_solve(bpf_rf rf, int p, TV *x) {
-  trisolve<TV>(rf->ctx->stream, rf->lo, p, x, p, x, p);
-  trisolve<TV>(rf->ctx->stream, rf->up, p, x, p, x, p);
+  hipStream_t s = rf->ctx->stream;
+  if (!rf->dense()) {
+    trisolve<TV>(s, rf->lo, p, x, p, x, p);
+    trisolve<TV>(s, rf->up, p, x, p, x, p);
+    return;
+  }
+  const int t = rf->t, m = rf->m;
+  trisolve<TV>(s, rf->lo_h, p, x, p, x, p);
+  trisolve<TV>(s, rf->l21, p, x, p, x, p);
+  rf->Xd.ensure((size_t)m * p * sizeof(double));
+  rf->Yd.ensure((size_t)m * p * sizeof(double));
+  TV *xt = x + (size_t)t * p;
+  copy_cast<TV, double>(s, m, p, xt, p, rf->Xd.as<double>(), p, 1.0);
+  dense_lu_solve(s, m, p, rf->S.as<double>(), rf->Linv.as<double>(), rf->Uinv.as<double>(), rf->Xd.as<double>(),
+                 rf->Yd.as<double>());
+  copy_cast<double, TV>(s, m, p, rf->Xd.as<double>(), p, xt, p, 1.0);
+  trisolve<TV>(s, rf->u12, p, x, p, x, p);
+  trisolve<TV>(s, rf->up_h, p, x, p, x, p);
 }
 
 }  // namespace
@@ -145,10 +254,9 @@ int bpf_rf_create(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf *out)
     BPF_REQUIRE(ctx && A && out, "null argument");
     auto *rf = new bpf_rf_s();
     try {
-      setup(ctx, A, ordering, tol, rf);
+      setup(ctx, A, ordering, tol, rf, true);
       refactor_into(rf, A->val.as<double>(), rf->fac.as<double>());
-      triop_refresh(rf->lo, rf->fac.as<double>(), ctx->stream);
-      triop_refresh(rf->up, rf->fac.as<double>(), ctx->stream);
+      refresh_ops(rf, rf->fac.as<double>());
     } catch (...) {
       delete rf;
       throw;
@@ -163,8 +271,7 @@ int bpf_rf_refactor(bpf_rf rf, bpf_csr A) {
     BPF_REQUIRE(rf->batch == 0, "use bpf_rf_batch_refactor for a batch handle");
     check_pattern(rf, A);
     refactor_into(rf, A->val.as<double>(), rf->fac.as<double>());
-    triop_refresh(rf->lo, rf->fac.as<double>(), rf->ctx->stream);
-    triop_refresh(rf->up, rf->fac.as<double>(), rf->ctx->stream);
+    refresh_ops(rf, rf->fac.as<double>());
   });
 }
 
@@ -191,8 +298,7 @@ int bpf_rf_solve(bpf_rf rf, int dtype, int nrhs, const double *d_B, int ldb, dou
       launch_perm_rm_to_cm<float>(s, n, nrhs, z, nrhs, rf->dQ.as<int>(), d_X, ldx);
     }
     BPF_CHECK_HIP(hipStreamSynchronize(s));
-    triop_check(rf->lo, s);
-    triop_check(rf->up, s);
+    check_ops(rf);
   });
 }
 
@@ -230,8 +336,7 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
     }
     launch_rm_to_cm(s, n, p, X, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));
-    triop_check(rf->lo, s);
-    triop_check(rf->up, s);
+    check_ops(rf);
   });
 }
 
@@ -240,8 +345,8 @@ int bpf_rf_info(bpf_rf rf, int *n, int *nnz_factor, int *nl, int *nu) {
     BPF_REQUIRE(rf, "null handle");
     if (n) *n = rf->n;
     if (nnz_factor) *nnz_factor = rf->nnzF;
-    if (nl) *nl = rf->lo.nlevels;
-    if (nu) *nu = rf->up.nlevels;
+    if (nl) *nl = rf->lop().nlevels;
+    if (nu) *nu = rf->uop().nlevels;
   });
 }
 
@@ -276,7 +381,7 @@ int bpf_rf_batch_create(bpf_ctx ctx, bpf_csr A, int batchsize, int ordering, dou
                 "Please specify a number of batch greater than 1 for `CuSolverRfLUBatch`");
     auto *rf = new bpf_rf_s();
     try {
-      setup(ctx, A, ordering, tol, rf);
+      setup(ctx, A, ordering, tol, rf, false);
       rf->batch = batchsize;
       rf->facb.ensure((size_t)rf->nnzF * batchsize * sizeof(double));
       // every batch entry starts from A's values (src/cusolverRF.jl:399-413)
@@ -318,8 +423,7 @@ int bpf_rf_batch_solve(bpf_rf rf, const double *d_B, int ldb, double *d_X, int l
       launch_perm_rm_to_cm<double>(s, n, 1, z, 1, rf->dQ.as<int>(), d_X + (size_t)b * ldx, ldx);
     }
     BPF_CHECK_HIP(hipStreamSynchronize(s));
-    triop_check(rf->lo, s);
-    triop_check(rf->up, s);
+    check_ops(rf);
   });
 }
 

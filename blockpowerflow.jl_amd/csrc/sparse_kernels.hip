@@ -120,7 +120,22 @@ __device__ __forceinline__ void tri_row(const TriDev &a, int t, int lane, int G,
       const int c = cbase + lane + m * G;
       acc[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
     }
-    for (int k = k0; k < k1; ++k) {
+    int k = k0;
+    for (; k + 1 < k1; k += 2) {        // two non-zeros per step (loads overlap)
+      const int j0 = a.col[k], j1 = a.col[k + 1];
+      const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + 1]);
+      const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = cbase + lane + m * G;
+        if (c < p) {
+          const TV a0 = y0[c], a1 = y1[c];
+          acc[m] = fma(-v0, a0, acc[m]);
+          acc[m] = fma(-v1, a1, acc[m]);
+        }
+      }
+    }
+    if (k < k1) {
       const int j = a.col[k];
       const TV v = static_cast<TV>(val[k]);
       const TV *yr = y + (size_t)j * ldy;
@@ -236,7 +251,31 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
         TV acc[CPL];
 #pragma unroll
         for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
-        for (int k = k0 + sub; k < k1; k += wpr) {
+        TV acc2[CPL];
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc2[m] = TV(0);
+        int k = k0 + sub;
+        // four non-zeros per step: their column indices, values and y rows
+        // are loaded together, two accumulator sets
+        for (; k + 3 * wpr < k1; k += 4 * wpr) {
+          const int j0 = a.col[k], j1 = a.col[k + wpr], j2 = a.col[k + 2 * wpr], j3 = a.col[k + 3 * wpr];
+          const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + wpr]);
+          const TV v2 = static_cast<TV>(val[k + 2 * wpr]), v3 = static_cast<TV>(val[k + 3 * wpr]);
+          const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
+          const TV *y2 = y + (size_t)j2 * ldy, *y3 = y + (size_t)j3 * ldy;
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const int c = lane + m * 64;
+            if (c < p) {
+              const TV a0 = y0[c], a1 = y1[c], a2 = y2[c], a3 = y3[c];
+              acc[m] = fma(v0, a0, acc[m]);
+              acc2[m] = fma(v1, a1, acc2[m]);
+              acc[m] = fma(v2, a2, acc[m]);
+              acc2[m] = fma(v3, a3, acc2[m]);
+            }
+          }
+        }
+        for (; k < k1; k += wpr) {
           const int j = a.col[k];
           const TV v = static_cast<TV>(val[k]);
           const TV *yr = y + (size_t)j * ldy;
@@ -247,7 +286,7 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
           }
         }
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) part[gid][lane + m * 64] = acc[m];
+        for (int m = 0; m < CPL; ++m) part[gid][lane + m * 64] = acc[m] + acc2[m];
       }
       __syncthreads();
       if (r < rows && sub == 0) {
@@ -296,7 +335,9 @@ struct FacDev {
 
 // IKJ elimination of row i (one wave).  Row i's pattern and values are staged
 // in this wave's LDS slice; rows k < i referenced by it are final.
-__device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_val) {
+// kmax < i: only pivot rows k < kmax are eliminated (the rest of the row is
+// left as the Schur complement of the leading kmax rows).
+__device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_val, int kmax = 0x7fffffff) {
   const int base = f.rowptr[i];
   const int len = f.rowptr[i + 1] - base;
   const int dloc = f.diagpos[i] - base;
@@ -307,6 +348,7 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
   wave_sync();
   for (int kk = 0; kk < dloc; ++kk) {
     const int k = s_col[kk];
+    if (k >= kmax) break;
     const double ukk = f.fac[f.diagpos[k]];
     const double l = s_val[kk] / ukk;
     const int j0 = f.diagpos[k] + 1, j1 = f.rowptr[k + 1];
@@ -322,7 +364,7 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
     wave_sync();
   }
   for (int q = lane; q < len; q += 64) f.fac[base + q] = s_val[q];
-  if (lane == 0 && s_val[dloc] == 0.0) atomicMin(f.status, i);
+  if (lane == 0 && kmax > i && s_val[dloc] == 0.0) atomicMin(f.status, i);
   wave_sync();
 }
 
@@ -396,6 +438,19 @@ __global__ void fac_syncfree_kernel(FacDev f, int n, const int *dptr, const int 
     else ilu_row(f, i, lane, s_col, s_val);
   }
   sf_publish(flags, i, epoch, lane);
+}
+
+// Rows [r0, r1) (mutually independent) eliminated against pivot rows
+// k < kmax, one wavefront per row.
+__global__ void fac_partial_kernel(FacDev f, int r0, int r1, int kmax) {
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = r0 + blockIdx.x * (blockDim.x >> 6) + wave;
+  if (i >= r1) return;
+  int *s_col = reinterpret_cast<int *>(smem) + (size_t)wave * f.cap;
+  double *s_val = reinterpret_cast<double *>(smem + (size_t)(blockDim.x >> 6) * f.cap * sizeof(int)) +
+                  (size_t)wave * f.cap;
+  ilu_row(f, i, lane, s_col, s_val, kmax);
 }
 
 template <bool IC>
@@ -649,6 +704,22 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
       else hipLaunchKernelGGL((fac_tail_kernel<false>), dim3(1), dim3(64 * tw), shm, s, f, seg.l0, seg.l1);
     }
   }
+  BPF_CHECK_HIP(hipGetLastError());
+  prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
+}
+
+void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
+                           const int *diagpos, double *fac, int max_row_nnz, int *d_status) {
+  if (r1 <= r0) return;
+  const int cap = std::max(64, max_row_nnz);
+  const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
+  if (per_wave > 64 * 1024)
+    BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
+  const int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
+  FacDev f{rowptr, colidx, diagpos, nullptr, nullptr, fac, d_status, cap};
+  const int tok = prof::begin(s);
+  hipLaunchKernelGGL(fac_partial_kernel, dim3((r1 - r0 + wpb - 1) / wpb), dim3(64 * wpb), per_wave * wpb, s, f,
+                     r0, r1, kmax);
   BPF_CHECK_HIP(hipGetLastError());
   prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
 }

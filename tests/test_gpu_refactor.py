@@ -133,3 +133,32 @@ def test_rf_singular_raises(bpf):
     A = bpf.HipSparseMatrixCSR(M.indptr, M.indices, M.data)
     with pytest.raises(bpf.SingularException):
         bpf.CUSOLVERRF.CusolverRfLU(A)
+
+
+def test_rf_dense_tail_matches_oracle(bpf, gen, oracle):
+    # n >= 16384: the last rows of B = A[P, Q] are refactored and solved as a
+    # dense block (blocked LU without pivoting on matrix cores); the factor
+    # values must equal the fixed-pattern sparse refactorization's.
+    grid, S = gen.make_system(10000, 64, seed=46)
+    assert S.n >= 16384
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, nrhs=64)
+    P, Q, rp, ci, v = rf.factors()
+    ref = oracle.lu_refactor(S.indptr, S.indices, S.data, P, Q, rp, ci)
+    np.testing.assert_allclose(v, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+    B = bpf.HipMatrix.from_numpy(S.B)
+    X = bpf.HipMatrix(S.B.shape)
+    bpf.ldiv_(X, rf, B)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    np.testing.assert_allclose(X.to_numpy(), Xd, rtol=1e-8, atol=1e-10 * np.abs(Xd).max())
+    # Newton value set on the same pattern; FP32 solves + 2 FP64 refinements
+    vals = gen.newton_value_sets(grid, 1, seed=47)[0]
+    A.set_values(vals)
+    bpf.CUSOLVERRF.update_(rf, A)
+    _, _, _, _, v2 = rf.factors()
+    ref2 = oracle.lu_refactor(S.indptr, S.indices, vals, P, Q, rp, ci)
+    np.testing.assert_allclose(v2, ref2, rtol=1e-9, atol=1e-11 * np.abs(ref2).max())
+    Xr, berr = rf.solve_refined(A, B, refine=2)
+    assert berr.max() <= 1e-10
+    ref_x = oracle.direct_solve(S.indptr, S.indices, vals, S.B)
+    np.testing.assert_allclose(Xr.to_numpy(), ref_x, rtol=1e-7, atol=1e-9 * np.abs(ref_x).max())
