@@ -160,40 +160,115 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
     S[(size_t)r * lds + c] = G[r * kLd + c];
   }
   __syncthreads();
-  // U^{-1}, column c by thread c: x[c] = 1/U[c][c],
-  // x[r] = -(sum_{r<mm<=c} U[r][mm] x[mm]) / U[r][r]; x[r] (r < c) is kept at
-  // G[c][r] (the strict lower part, whose L is already saved in S).
-  if (tid < w) {
-    const int c = tid;
-    const double dc = 1.0 / G[c * kLd + c];
-    for (int r = c - 1; r >= 0; --r) {
-      double v = G[r * kLd + c] * dc;
+  // Inverses, blockwise on 16 x 16 blocks (I, J = block indices).
+  //  U^{-1} = X upper: X_II = U_II^{-1} (one thread per column), then
+  //    X_IJ = -X_II sum_{I<M<=J} U_IM X_MJ, block column J by wavefront J;
+  //    X is kept transposed in the strict lower part (L is saved in S) with
+  //    its diagonal in dg[].
+  //  L^{-1} = Z unit lower: L moves transposed into the strict upper part,
+  //    Z_II by substitution, then Z_IJ = -Z_II sum_{J<=M<I} L_IM Z_MJ, block
+  //    column J by wavefront J; Z kept in the strict lower part.
+  __shared__ double dg[kDluNb];
+  __shared__ double scr[kDluNb / 16][16][17];
+  const int nb = (w + 15) / 16;
+  for (int c = tid; c < w; c += 1024) dg[c] = 1.0 / G[c * kLd + c];
+  __syncthreads();
+  for (int c = tid; c < w; c += 1024) {
+    const int rb = (c / 16) * 16;
+    for (int r = c - 1; r >= rb; --r) {
+      double v = G[r * kLd + c] * dg[c];
       for (int mm = r + 1; mm < c; ++mm) v += G[r * kLd + mm] * G[c * kLd + mm];
-      G[c * kLd + r] = -v / G[r * kLd + r];
+      G[c * kLd + r] = -v * dg[r];
+    }
+  }
+  __syncthreads();
+  auto xu = [&](int I, int J, int k, int n) -> double {   // X_IJ[k][n], I <= J
+    const int r = I * 16 + k, c = J * 16 + n;
+    if (r >= w || c >= w) return 0.0;
+    return r < c ? G[c * kLd + r] : (r == c ? dg[r] : 0.0);
+  };
+  if (wave < nb) {
+    const int J = wave;
+    double (*sw)[17] = scr[wave];
+    for (int I = J - 1; I >= 0; --I) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int M = I + 1; M <= J; ++M)
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const int k = k0 + lr;
+          const int rr = I * 16 + lc, cc = M * 16 + k;
+          acc = mfma4((rr < w && cc < w) ? G[rr * kLd + cc] : 0.0, xu(M, J, k, lc), acc);
+        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = acc[q];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      dbl4 x4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) x4 = mfma4(xu(I, I, lc, k0 + lr), sw[k0 + lr][lc], x4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = I * 16 + lr + 4 * q, c = J * 16 + lc;
+        if (r < w && c < w) G[c * kLd + r] = -x4[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
   }
   __syncthreads();
   for (int e = tid; e < kDluNb * kDluNb; e += 1024) {
     const int r = e / kDluNb, c = e % kDluNb;
     double v = 0.0;
-    if (r < w && c < w) v = r < c ? G[c * kLd + r] : (r == c ? 1.0 / G[r * kLd + r] : 0.0);
+    if (r < w && c < w) v = r < c ? G[c * kLd + r] : (r == c ? dg[r] : 0.0);
     Uinv[e] = v;
   }
   __syncthreads();
-  // L^{-1} (unit lower): L moves transposed into the strict upper part
-  // (G[mm][r] = L[r][mm], r > mm), then column c by thread c: x[c] = 1,
-  // x[r] = -sum_{c<=mm<r} L[r][mm] x[mm], x[r] (r > c) kept at G[r][c].
   for (int e = tid; e < w * w; e += 1024) {
     const int r = e / w, mm = e % w;
     if (r > mm) G[mm * kLd + r] = S[(size_t)r * lds + mm];
   }
   __syncthreads();
-  if (tid < w) {
-    const int c = tid;
-    for (int r = c + 1; r < w; ++r) {
-      double v = G[c * kLd + r];   // mm = c term (x[c] = 1)
+  // Z_II: column c by thread c within its diagonal block
+  for (int c = tid; c < w; c += 1024) {
+    const int re = min(w, (c / 16) * 16 + 16);
+    for (int r = c + 1; r < re; ++r) {
+      double v = G[c * kLd + r];
       for (int mm = c + 1; mm < r; ++mm) v += G[mm * kLd + r] * G[mm * kLd + c];
       G[r * kLd + c] = -v;
+    }
+  }
+  __syncthreads();
+  auto zl = [&](int I, int J, int k, int n) -> double {   // Z_IJ[k][n], I >= J
+    const int r = I * 16 + k, c = J * 16 + n;
+    if (r >= w || c >= w) return 0.0;
+    return r > c ? G[r * kLd + c] : (r == c ? 1.0 : 0.0);
+  };
+  auto lt = [&](int I, int M, int k, int n) -> double {   // L_IM[k][n], I > M (stored transposed)
+    const int r = I * 16 + k, c = M * 16 + n;
+    return (r < w && c < w && r > c) ? G[c * kLd + r] : 0.0;
+  };
+  if (wave < nb) {
+    const int J = wave;
+    double (*sw)[17] = scr[wave];
+    for (int I = J + 1; I < nb; ++I) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int M = J; M < I; ++M)
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) acc = mfma4(lt(I, M, lc, k0 + lr), zl(M, J, k0 + lr, lc), acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = acc[q];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) z4 = mfma4(zl(I, I, lc, k0 + lr), sw[k0 + lr][lc], z4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = I * 16 + lr + 4 * q, c = J * 16 + lc;
+        if (r < w && c < w) G[r * kLd + c] = -z4[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
   }
   __syncthreads();
