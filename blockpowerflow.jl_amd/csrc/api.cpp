@@ -75,6 +75,7 @@ const char *bpf_version(void) { return "bpf_amd 0.1 (gfx950)"; }
 int bpf_set_option(int option, long long value) {
   return guard([&] {
     if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
+    else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }
@@ -83,6 +84,7 @@ int bpf_get_option(int option, long long *value) {
   return guard([&] {
     BPF_REQUIRE(value, "null argument");
     if (option == BPF_OPT_SYNCFREE) *value = g_syncfree ? 1 : 0;
+    else if (option == BPF_OPT_AUTOTUNE) *value = g_autotune ? 1 : 0;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }
@@ -289,6 +291,7 @@ int bpf_csr_destroy(bpf_csr A) {
   return guard([&] {
     if (!A) return;
     (void)hipStreamSynchronize(A->ctx->stream);
+    spmm_forget(A->rowptr.as<int>());
     delete A;
   });
 }

@@ -141,6 +141,10 @@ struct TriOp {
 // Process-wide switch between the sync-free single-launch triangular
 // solves / factorizations (default) and the level-by-level launches.
 extern bool g_syncfree;
+// Process-wide switch for the SpMM launch-shape autotuner (default on), and
+// removal of a matrix's cached shapes (its row-pointer array is released).
+extern bool g_autotune;
+void spmm_forget(const int *rowptr);
 // Reads the op's status word (blocking); throws BPF_E_HIP if a sync-free
 // dependency wait ran out, after switching to level-by-level launches.
 void triop_check(TriOp &op, hipStream_t s);

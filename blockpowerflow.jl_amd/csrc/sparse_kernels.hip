@@ -11,6 +11,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace bpf {
 
@@ -520,17 +523,15 @@ __global__ __launch_bounds__(kBlock) void tri_gather_kernel(int m, const int *ga
 }  // namespace
 
 // ------------------------------------------------------------------ hosts
+bool g_autotune = true;
+
+namespace {
+
 template <typename TV>
-void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *colidx, const void *val,
-                 bool val32, int p, const TV *X, int ldx, TV *Y, int ldy, double alpha,
-                 double beta) {
-  if (n <= 0 || p <= 0) return;
-  const int tok = prof::begin(s);
-  int glog, cpl;
-  lane_shape(p, glog, cpl);
+void spmm_shape(hipStream_t s, int n, const int *rowptr, const int *colidx, const void *val, bool val32,
+                int p, const TV *X, int ldx, TV *Y, int ldy, TV a, TV b, int glog, int cpl) {
   const int rpb = kBlock >> glog;
   dim3 grid((n + rpb - 1) / rpb);
-  const TV a = static_cast<TV>(alpha), b = static_cast<TV>(beta);
 #define BPF_SPMM(TA, C)                                                                 \
   hipLaunchKernelGGL((spmm_rm_kernel<TA, TV, C>), grid, dim3(kBlock), 0, s, n, rowptr,  \
                      colidx, static_cast<const TA *>(val), p, X, ldx, Y, ldy, a, b, glog)
@@ -540,6 +541,95 @@ void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *co
     if (cpl == 1) BPF_SPMM(double, 1); else if (cpl == 2) BPF_SPMM(double, 2); else BPF_SPMM(double, 4);
   }
 #undef BPF_SPMM
+}
+
+// SpMM launch-shape autotuner.  On the first product of a matrix (keyed by
+// its row-pointer array), width and element types, each candidate shape --
+// G lanes per row (a power of two <= 64) with CPL = 1, 2 or 4 columns per
+// lane -- runs on the actual operands into a scratch block and is timed with
+// HIP events; the fastest shape is cached for the process.
+struct SpmmKey {
+  const int *rowptr;
+  int n, p, ldx, tv, val32;
+  bool operator<(const SpmmKey &o) const {
+    return std::tie(rowptr, n, p, ldx, tv, val32) < std::tie(o.rowptr, o.n, o.p, o.ldx, o.tv, o.val32);
+  }
+};
+std::map<SpmmKey, std::pair<int, int>> g_spmm_shapes;
+std::mutex g_spmm_mu;
+
+template <typename TV>
+std::pair<int, int> spmm_autotune(hipStream_t s, int n, const int *rowptr, const int *colidx, const void *val,
+                                  bool val32, int p, const TV *X, int ldx) {
+  std::vector<std::pair<int, int>> cand;
+  for (int cpl : {1, 2, 4}) {
+    int g = next_pow2((p + cpl - 1) / cpl);
+    g = std::max(1, std::min(64, g));
+    if (cpl > 1 && g * (cpl / 2) >= p) continue;   // the narrower CPL already covers p
+    int glog = 0;
+    while ((1 << glog) < g) ++glog;
+    cand.push_back({glog, cpl});
+  }
+  if (cand.size() == 1) return cand[0];
+  DevBuf scratch;
+  scratch.ensure((size_t)n * p * sizeof(TV));
+  hipEvent_t e0, e1;
+  BPF_CHECK_HIP(hipEventCreate(&e0));
+  BPF_CHECK_HIP(hipEventCreate(&e1));
+  std::pair<int, int> best = cand[0];
+  float best_ms = 1e30f;
+  for (const auto &c : cand) {
+    spmm_shape<TV>(s, n, rowptr, colidx, val, val32, p, X, ldx, scratch.as<TV>(), p, TV(1), TV(0), c.first, c.second);
+    BPF_CHECK_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r)
+      spmm_shape<TV>(s, n, rowptr, colidx, val, val32, p, X, ldx, scratch.as<TV>(), p, TV(1), TV(0), c.first,
+                     c.second);
+    BPF_CHECK_HIP(hipEventRecord(e1, s));
+    BPF_CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    BPF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+}  // namespace
+
+void spmm_forget(const int *rowptr) {
+  std::lock_guard<std::mutex> lk(g_spmm_mu);
+  for (auto it = g_spmm_shapes.begin(); it != g_spmm_shapes.end();)
+    it = it->first.rowptr == rowptr ? g_spmm_shapes.erase(it) : std::next(it);
+}
+
+template <typename TV>
+void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *colidx, const void *val,
+                 bool val32, int p, const TV *X, int ldx, TV *Y, int ldy, double alpha,
+                 double beta) {
+  if (n <= 0 || p <= 0) return;
+  int glog, cpl;
+  lane_shape(p, glog, cpl);
+  if (g_autotune) {
+    const SpmmKey key{rowptr, n, p, ldx, (int)sizeof(TV), val32 ? 1 : 0};
+    std::pair<int, int> shape;
+    bool have = false;
+    {
+      std::lock_guard<std::mutex> lk(g_spmm_mu);
+      auto it = g_spmm_shapes.find(key);
+      if (it != g_spmm_shapes.end()) { shape = it->second; have = true; }
+    }
+    if (!have) {
+      shape = spmm_autotune<TV>(s, n, rowptr, colidx, val, val32, p, X, ldx);
+      std::lock_guard<std::mutex> lk(g_spmm_mu);
+      g_spmm_shapes[key] = shape;
+    }
+    glog = shape.first;
+    cpl = shape.second;
+  }
+  const int tok = prof::begin(s);
+  spmm_shape<TV>(s, n, rowptr, colidx, val, val32, p, X, ldx, Y, ldy, static_cast<TV>(alpha), static_cast<TV>(beta),
+                 glog, cpl);
   BPF_CHECK_HIP(hipGetLastError());
   const double wv = val32 ? 4.0 : 8.0, wx = sizeof(TV);
   prof::end(tok, s, prof::SPMM, nnz * (wv + 4.0) + 4.0 * (n + 1) + 2.0 * n * p * wx,

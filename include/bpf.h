@@ -58,8 +58,11 @@ const char *bpf_version(void);
  * dependency level and one single-workgroup launch for runs of narrow levels.
  * On MI355X the device-scope flag handshake costs more than the launches it
  * saves (each release/acquire writes back / invalidates the per-XCD L2), so
- * the sync-free form is slower there; it is kept as an option. */
-enum { BPF_OPT_SYNCFREE = 1 };
+ * the sync-free form is slower there; it is kept as an option.
+ * BPF_OPT_AUTOTUNE: 1 (default) = the first SpMM of a matrix at a given
+ * block width times each launch shape (lanes per row x columns per lane) on
+ * the actual operands and keeps the fastest; 0 = a fixed shape per width. */
+enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

@@ -162,3 +162,25 @@ def test_syncfree_matches_level_schedule(bpf, gen, oracle, p):
     F = oracle.ilu0(S.indptr, S.indices, S.data)
     ref = oracle.ilu0_apply(S.indptr, S.indices, F, S.B)
     np.testing.assert_allclose(out[1][1], ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("p", [8, 64, 128, 256])
+def test_spmm_autotuned_shape_is_exact(bpf, gen, oracle, p):
+    # Every SpMM launch shape sums each row's products in the same order, so
+    # the autotuner's choice cannot change the result.
+    _, S = _system(gen, 2000, p, 14)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    ctx = A.ctx
+    Xr = bpf.HipMatrix.from_numpy(np.asfortranarray(S.B.T))
+    out = {}
+    try:
+        for tune in (0, 1):
+            bpf.set_option("autotune", tune)
+            Yr = bpf.HipMatrix((p, S.n))
+            bpf._lib.check(bpf.L.bpf_spmm(ctx.handle, A.handle, 0, 1, p, Xr.ptr, p, Yr.ptr, p, 1.0, 0.0))
+            out[tune] = Yr.to_numpy()
+    finally:
+        bpf.set_option("autotune", 1)
+    np.testing.assert_array_equal(out[0], out[1])
+    ref = oracle.spmm(S.indptr, S.indices, S.data, S.B)
+    np.testing.assert_allclose(out[1].T, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
