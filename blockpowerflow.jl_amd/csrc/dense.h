@@ -164,6 +164,8 @@ void colmax_colmajor(hipStream_t s, int n, int p, const double *X, int ld, doubl
 template <typename T>
 void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out);
 void axpby(hipStream_t s, size_t m, double alpha, const double *x, double beta, double *y);
+// y[g*stride + e] += x[g*stride + e] for e < count, g < ng (one launch).
+void axpby_groups(hipStream_t s, int ng, size_t count, size_t stride, const double *x, double *y);
 template <typename TS, typename TD>
 void copy_cast(hipStream_t s, int n, int p, const TS *src, int lds, TD *dst, int ldd, double scale);
 

@@ -1108,6 +1108,14 @@ __global__ __launch_bounds__(kB) void axpby_kernel(size_t m, double alpha, const
   if (i < m) y[i] = alpha * x[i] + beta * y[i];
 }
 
+__global__ __launch_bounds__(kB) void add_groups_kernel(size_t count, size_t stride, const double *x,
+                                                       double *y) {
+  const size_t e = (size_t)blockIdx.x * kB + threadIdx.x;
+  if (e >= count) return;
+  const size_t o = (size_t)blockIdx.y * stride + e;
+  y[o] += x[o];
+}
+
 template <typename TS, typename TD>
 __global__ __launch_bounds__(kB) void copy_cast_kernel(int n, int p, const TS *src, int lds, TD *dst,
                                                       int ldd, double scale) {
@@ -1313,6 +1321,13 @@ void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode,
 }
 template void colstat_rowmajor<double>(hipStream_t, int, int, const double *, int, int, double *);
 template void colstat_rowmajor<float>(hipStream_t, int, int, const float *, int, int, double *);
+
+void axpby_groups(hipStream_t s, int ng, size_t count, size_t stride, const double *x, double *y) {
+  if (count == 0 || ng <= 0) return;
+  hipLaunchKernelGGL(add_groups_kernel, dim3((unsigned)((count + kB - 1) / kB), ng), dim3(kB), 0, s, count, stride,
+                     x, y);
+  BPF_CHECK_HIP(hipGetLastError());
+}
 
 void axpby(hipStream_t s, size_t m, double alpha, const double *x, double beta, double *y) {
   if (m == 0) return;

@@ -195,8 +195,7 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
     gb.psi = t;
     gb.rows_sum = rows_sum_;
     launch_block_gemm<TV, TV>(st_, gb, S_, ng_);
-    for (int g = 0; g < ng_; ++g)
-      axpby(st_, (size_t)(k + 1) * SS, 1.0, t + g * gstr, 1.0, col + g * gstr);
+    axpby_groups(st_, ng_, (size_t)(k + 1) * SS, gstr, t, col);
   }
 }
 

@@ -176,7 +176,7 @@ void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool all
   // the dense block width, up to 32 blocks, about n/16).
   if (allow_dense && n >= kDenseMinN) {
     const int nb = kDluNb;
-    rf->m = std::min(32 * nb, (n / 16) / nb * nb);
+    rf->m = std::min(16 * nb, (n / 16) / nb * nb);
   }
   if (!rf->dense()) {
     build_triop(rf->lo, n, lu.frow, lu.fcol, 0, s);
