@@ -46,6 +46,15 @@ BERR = 1e-10
 PEAK_HBM = 8.0e12
 METRIC = "solved RHS/sec at 1e-10 backward error: mixed+batched vs plain FP64, 1 GPU"
 FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "small"]
+PEAK_FP64_MFMA = 78.6e12
+# family -> (kernel, bound); "mfma" families are priced in FP64 flops.
+KERNELS = {
+    "spmm": ("spmm_rm_kernel", "hbm"),
+    "trsv": ("tri_level_kernel", "hbm"),
+    "factor": ("fac_level_kernel", "hbm"),
+    "proj": ("proj_partial_kernel", "mfma"),
+    "gemm": ("block_gemm_kernel", "mfma"),
+}
 
 
 def log(*a):
@@ -70,20 +79,36 @@ def log_families(fam):
                 f"{v['flops'] / max(v['ms'], 1e-9) / 1e9:8.2f} TFLOP/s")
 
 
-def roofline_from(fam, dt, families=("spmm", "trsv")):
-    """Roofline object for the dominant HBM-bound kernel family from the
-    in-process kernel timers (HIP events around every launch on the solver's
-    stream during the timed steps)."""
-    names = {"spmm": "spmm_rm_kernel", "trsv": "tri_level_kernel", "factor": "fac_level_kernel"}
-    dom = max(families, key=lambda k: fam[k]["ms"])
+def family_roofline(fam, name):
+    """(bound, achieved, peak, unit) of one kernel family: HBM families as
+    algorithmic bytes / time, matrix-core families as FP64 flops / time."""
+    d = fam[name]
+    sec = d["ms"] * 1e-3
+    if KERNELS[name][1] == "mfma":
+        return "mfma", (d["flops"] / sec / 1e12 if sec > 0 else 0.0), PEAK_FP64_MFMA / 1e12, "TFLOP/s"
+    return "hbm", (d["bytes"] / sec / 1e9 if sec > 0 else 0.0), PEAK_HBM / 1e9, "GB/s"
+
+
+def roofline_from(fam, dt, workload):
+    """Roofline object for the dominant kernel family (largest share of the
+    timed steps) from the in-process kernel timers (HIP events around every
+    launch on the solver's stream during the timed steps).  The other
+    families' figures are listed under "families"."""
+    timed_fams = [k for k in KERNELS if fam[k]["launches"]]
+    dom = max(timed_fams, key=lambda k: fam[k]["ms"])
     d = fam[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    bound, achieved, peak, unit = family_roofline(fam, dom)
+    others = {}
+    for k in timed_fams:
+        b, a, pk, u = family_roofline(fam, k)
+        others[k] = {"kernel": KERNELS[k][0], "bound": b, "achieved": round(a, 2), "unit": u,
+                     "frac": round(a / pk, 4), "share": round(fam[k]["ms"] / (dt * 1e3), 3)}
     return {
-        "bound": "hbm", "kernel": names[dom],
-        "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-        "frac": round(achieved / (PEAK_HBM / 1e9), 4), "traffic": None,
+        "bound": bound, "kernel": KERNELS[dom][0],
+        "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+        "frac": round(achieved / peak, 4), "traffic": measured_traffic(workload, KERNELS[dom][0]),
         "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(1, d["launches"]), 2),
-        "step_share": {k: round(v["ms"] / (dt * 1e3), 3) for k, v in fam.items() if v["launches"]},
+        "families": others,
     }
 
 
@@ -202,7 +227,7 @@ def bench_single(args, bpf, gen):
 
     def step(precision, bw):
         P.factor_(A)
-        X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, restart=False, itmax=2000,
+        X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=args.memory, restart=args.restart, itmax=2000,
                                 timemax=120.0, berr_tol=BERR, precision=precision, block_width=bw,
                                 max_refine=8, poly_degree=args.poly_degree if precision == "mixed" else 0,
                                 reorthogonalization=args.orth == "bcgs2")
@@ -216,7 +241,7 @@ def bench_single(args, bpf, gen):
     log(f"[bench] mixed: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * p} "
         f"its {st_last.niter} refinements {st_last.refine_steps} max berr {be_last.max():.2e}")
     log_families(fam)
-    roofline = roofline_from(fam, dt)
+    roofline = roofline_from(fam, dt, args.workload)
     plain = None
     k_plain = None
     if not args.no_plain:
@@ -285,7 +310,7 @@ def bench_batch(args, bpf, gen):
     log(f"[bench] batched: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nrhs} "
         f"max berr (rechecked) {be_max:.2e}")
     log_families(fam)
-    roofline = roofline_from(fam, dt)
+    roofline = roofline_from(fam, dt, "c4")
     plain = None
     if not args.no_plain:
         Ps = [bpf.ilu0(A) for A in As]
@@ -404,7 +429,7 @@ def bench_refactor(args, bpf, gen):
     log(f"[bench] refactor mixed: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nsets * p} "
         f"{out[-1][0]}")
     log_families(fam)
-    roofline = roofline_from(fam, dt, families=("spmm", "trsv", "factor"))
+    roofline = roofline_from(fam, dt, "c5")
     plain = None
     if not args.no_plain:
         dtp, outp, _ = timed(bpf, ctx, step_plain, 1, 1, "refactor+fp64 solve")
@@ -459,6 +484,34 @@ def cpu_baseline_refactor(S, vals, rf, p, budget_s=20.0):
     }
 
 
+def copy_probe(nbytes=1 << 30, copies=3):
+    """Known-size copies for calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters: each copy reads and writes nbytes of HBM."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(copies):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": copies}))
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (profiles/<workload>_<kernel>_traffic.md, line 'bytes per launch:
+    N'), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{kernel}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().lower().startswith("bytes per launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,12 +521,21 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block-width", type=int, default=0)
+    ap.add_argument("--memory", type=int, default=40,
+                    help="Krylov basis blocks kept (c2/c3); with --restart the restart length")
+    ap.add_argument("--restart", action="store_true", help="restarted block GMRES (c2/c3)")
     ap.add_argument("--poly-degree", type=int, default=None,
                     help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default=None,
                     help="block Gram-Schmidt: bmgs (the reference's loop) or bcgs2 (two classical "
                          "passes); default bmgs for c2/c3, bcgs2 for c4")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three device-to-device copies of 1 GiB (the known-size kernel that "
+                         "scales the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     batched = args.workload == "c4"
     if args.poly_degree is None:
         args.poly_degree = 0 if batched else 4

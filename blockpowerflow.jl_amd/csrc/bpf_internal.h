@@ -139,7 +139,7 @@ struct TriOp {
 };
 
 // Process-wide switch between the sync-free single-launch triangular
-// solves / factorizations (default) and the level-by-level launches.
+// solves / factorizations and the level-by-level launches (default).
 extern bool g_syncfree;
 // Process-wide switch for the SpMM launch-shape autotuner (default on), and
 // removal of a matrix's cached shapes (its row-pointer array is released).

@@ -37,6 +37,9 @@ struct ProjArgs {
   // counts, which sizes the chunks; empty chunks are skipped.  0 = column
   // groups over all n rows.
   int rows_sum = 0;
+  // Rows x width of the groups that take part (traffic accounting only;
+  // negative = all groups).
+  double cells = -1.0;
 };
 
 // Y[:, g] = beta*Y[:, g] + alpha * sum_{i<nb} V_{i0+i}[:, g] * Psi_{i,g}
@@ -56,6 +59,7 @@ struct GemmArgs {
   // part iff K > 0, over its first min(nb, K) blocks.
   int use_k = 0;
   int rows_sum = 0;    // stacked groups: sum of heights (traffic accounting)
+  double cells = -1.0; // rows x width of the groups that take part (accounting; negative = all)
 };
 
 struct CholArgs {

@@ -291,7 +291,9 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
       const int e = tid + q * NT;
       const int rr = e / KA, kk = e % KA;
       const int r = r0 + rr, k = k0 + kk;
-      rv[q] = (r < rend && k < gd.w) ? to_d(V[(size_t)r * a.ldv + k]) : 0.0;
+      // Last use of the basis block in this pass: streamed without displacing
+      // W from the caches (W is read again by the next projection).
+      rv[q] = (r < rend && k < gd.w) ? to_d(__builtin_nontemporal_load(&V[(size_t)r * a.ldv + k])) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < PE; ++q) {
@@ -1202,7 +1204,7 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   // Algorithmic traffic: each basis block and W read once; the s x s
   // products are 2 n w^2 flops per (block, group).
   // Stacked groups: rows_sum rows of width S; column groups: n rows of ng*S.
-  const double cells = a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * a.ng * S;
+  const double cells = a.cells >= 0 ? a.cells : a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * a.ng * S;
   prof::end(tok, s, prof::PROJ, cells * ((a.nb) * sizeof(TVV) + sizeof(TW)), 2.0 * cells * S * a.nb);
 }
 template void launch_proj<double, double>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
@@ -1226,7 +1228,7 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   BPF_CHECK_HIP(hipGetLastError());
-  const double cells = a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * ng * S;
+  const double cells = a.cells >= 0 ? a.cells : a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * ng * S;
   prof::end(tok, s, prof::GEMM,
             cells * (a.nb * sizeof(TVV) + (a.beta == 0.0 ? 1.0 : 2.0) * sizeof(TY)),
             2.0 * cells * S * a.nb);

@@ -172,20 +172,28 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
   const size_t SS = (size_t)S_ * S_;
   const size_t gstr = (size_t)(k + 2) * SS;
   const void *const *vt = vtab_.dev<const void>();
+  // work of the groups still iterating (kernel traffic accounting)
+  double cells = 0.0;
+  for (const auto &g : h_groups_)
+    if (g.active) cells += (double)(rows_sum_ > 0 ? g.nr : n) * S_;
   if (cfg_.orth == ORTH_BMGS) {
     for (int i = 0; i <= k; ++i) {
       ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
+      pa.cells = cells;
       launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, 0, gstr, false, part_);
       GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
       ga.rows_sum = rows_sum_;
+      ga.cells = cells;
       launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     }
     return;
   }
   ProjArgs pa{vt, nullptr, 0, k + 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
+  pa.cells = cells;
   launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
   GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
   ga.rows_sum = rows_sum_;
+  ga.cells = cells;
   launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
   if (cfg_.orth == ORTH_BCGS2) {
     tmpcol_.ensure(hcol_elems(k, ng_, S_) * sizeof(double));
