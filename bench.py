@@ -89,7 +89,7 @@ def family_roofline(fam, name):
     return "hbm", (d["bytes"] / sec / 1e9 if sec > 0 else 0.0), PEAK_HBM / 1e9, "GB/s"
 
 
-def roofline_from(fam, dt, workload):
+def roofline_from(fam, dt_prof, workload):
     """Roofline object for the dominant kernel family (largest share of the
     timed steps) from the in-process kernel timers (HIP events around every
     launch on the solver's stream during the timed steps).  The other
@@ -102,7 +102,7 @@ def roofline_from(fam, dt, workload):
     for k in timed_fams:
         b, a, pk, u = family_roofline(fam, k)
         others[k] = {"kernel": KERNELS[k][0], "bound": b, "achieved": round(a, 2), "unit": u,
-                     "frac": round(a / pk, 4), "share": round(fam[k]["ms"] / (dt * 1e3), 3)}
+                     "frac": round(a / pk, 4), "share": round(fam[k]["ms"] / (dt_prof * 1e3), 3)}
     return {
         "bound": bound, "kernel": KERNELS[dom][0],
         "achieved": round(achieved, 2), "peak": peak, "unit": unit,
@@ -112,26 +112,33 @@ def roofline_from(fam, dt, workload):
     }
 
 
-def timed(bpf, ctx, step, steps, warmup, label):
-    """warmup untimed steps, then `steps` timed steps with the kernel timers
-    on; returns (seconds, list of step results, kernel families)."""
+def timed(bpf, ctx, step, steps, warmup, label, profile=True):
+    """warmup untimed steps, then `steps` timed steps.  The kernel timers
+    (HIP events around every launch) are on during the last timed step only,
+    so their own cost touches one step (none with profile=False, as for the
+    baselines); returns (seconds of all timed steps,
+    list of step results, kernel families of the instrumented step, seconds
+    of the instrumented step)."""
     for w in range(warmup):
         r = step()
         log(f"[bench] warmup {label} {w}: {r[0]}")
     ctx.synchronize()
-    bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
     out = []
     try:
-        ctx.synchronize()
         t = time.perf_counter()
-        for _ in range(steps):
+        for i in range(steps):
+            if i == steps - 1:
+                ctx.synchronize()
+                if profile:
+                    bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
+                t_prof = time.perf_counter()
             out.append(step())
         ctx.synchronize()
-        dt = time.perf_counter() - t
-        fam = family_stats(bpf, ctx)
+        t_end = time.perf_counter()
+        fam = family_stats(bpf, ctx) if profile else None
     finally:
         bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 0))
-    return dt, out, fam
+    return t_end - t, out, fam, t_end - t_prof
 
 
 def result_line(args, value, dt, dtype, config, extra):
@@ -230,22 +237,23 @@ def bench_single(args, bpf, gen):
         X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=args.memory, restart=args.restart, itmax=2000,
                                 timemax=120.0, berr_tol=BERR, precision=precision, block_width=bw,
                                 max_refine=8, poly_degree=args.poly_degree if precision == "mixed" else 0,
+                                bmgs_blocks=args.bmgs_blocks if precision == "mixed" else 1,
                                 reorthogonalization=args.orth == "bcgs2")
         be = bpf.backward_error(A, X, B)
         return (f"its {st.niter} max berr {be.max():.2e}", st, be)
 
-    dt, out, fam = timed(bpf, ctx, lambda: step("mixed", s_mixed), args.steps, args.warmup, "mixed")
+    dt, out, fam, dt_prof = timed(bpf, ctx, lambda: step("mixed", s_mixed), args.steps, args.warmup, "mixed")
     _, st_last, be_last = out[-1]
     solved = sum(int(np.sum(be <= BERR)) for _, _, be in out)
     value = solved / dt
     log(f"[bench] mixed: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * p} "
         f"its {st_last.niter} refinements {st_last.refine_steps} max berr {be_last.max():.2e}")
     log_families(fam)
-    roofline = roofline_from(fam, dt, args.workload)
+    roofline = roofline_from(fam, dt_prof, args.workload)
     plain = None
     k_plain = None
     if not args.no_plain:
-        dtp, outp, _ = timed(bpf, ctx, lambda: step("fp64", 0), 1, 1, "fp64")
+        dtp, outp, _, _ = timed(bpf, ctx, lambda: step("fp64", 0), 1, 1, "fp64", profile=False)
         _, stp, bep = outp[-1]
         k_plain = stp.niter
         solvedp = int(np.sum(bep <= BERR))
@@ -259,10 +267,11 @@ def bench_single(args, bpf, gen):
     config = {
         "workload": (f"{args.workload}: {nbus}-bus synthetic polar NR Jacobian n={S.n} nnz={S.nnz}, "
                      f"{p} RHS ({p - nunit} N(0,1) + {nunit} unit injections); flexible block GMRES "
-                     f"(FP64 basis, {args.orth}, block width {s_mixed}) with an FP32 ILU0 + degree-{args.poly_degree} "
+                     f"(FP64 basis, {args.orth}" + (f" over {args.bmgs_blocks}-block steps" if args.orth == "bmgs" else "")
+                     + f", block width {s_mixed}) with an FP32 ILU0 + degree-{args.poly_degree} "
                      f"GMRES-polynomial preconditioner, to per-RHS backward error 1e-10"),
         "n": S.n, "nnz": S.nnz, "nrhs": p, "block_width": s_mixed, "poly_degree": args.poly_degree,
-        "orth": args.orth,
+        "orth": args.orth, "bmgs_blocks": args.bmgs_blocks,
         "iterations": st_last.niter, "refinement_steps": st_last.refine_steps,
         "max_backward_error": float(be_last.max()),
     }
@@ -301,7 +310,7 @@ def bench_batch(args, bpf, gen):
         nsolved = int(sum(int(ps[i]) for i in range(nsys) if solved[i]))
         return (f"block its {st.niter} solved RHS {nsolved}/{nrhs} max berr {st.max_berr:.2e}", st, nsolved)
 
-    dt, out, fam = timed(bpf, ctx, step, args.steps, args.warmup, "batched")
+    dt, out, fam, dt_prof = timed(bpf, ctx, step, args.steps, args.warmup, "batched")
     solved = sum(r[2] for r in out)
     value = solved / dt
     st_last = out[-1][1]
@@ -310,7 +319,7 @@ def bench_batch(args, bpf, gen):
     log(f"[bench] batched: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nrhs} "
         f"max berr (rechecked) {be_max:.2e}")
     log_families(fam)
-    roofline = roofline_from(fam, dt, "c4")
+    roofline = roofline_from(fam, dt_prof, "c4")
     plain = None
     if not args.no_plain:
         Ps = [bpf.ilu0(A) for A in As]
@@ -326,7 +335,7 @@ def bench_batch(args, bpf, gen):
                 its += st.niter
             return (f"block its {its} solved RHS {ns}/{nrhs}", its, ns)
 
-        dtp, outp, _ = timed(bpf, ctx, step_plain, 1, 0, "fp64 per system")
+        dtp, outp, _, _ = timed(bpf, ctx, step_plain, 1, 0, "fp64 per system", profile=False)
         plain = {"value": outp[-1][2] / dtp, "ms_per_step": dtp * 1e3, "niter": outp[-1][1],
                  "solved": outp[-1][2],
                  "form": "plain FP64 block GMRES + ILU0, one system after another"}
@@ -423,16 +432,16 @@ def bench_refactor(args, bpf, gen):
             worst = max(worst, float(be.max()))
         return (f"solved RHS {ns}/{nsets * p} max berr {worst:.2e}", worst, ns)
 
-    dt, out, fam = timed(bpf, ctx, step_mixed, args.steps, args.warmup, "refactor+fp32 solve+2 IR")
+    dt, out, fam, dt_prof = timed(bpf, ctx, step_mixed, args.steps, args.warmup, "refactor+fp32 solve+2 IR")
     solved = sum(r[2] for r in out)
     value = solved / dt
     log(f"[bench] refactor mixed: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nsets * p} "
         f"{out[-1][0]}")
     log_families(fam)
-    roofline = roofline_from(fam, dt, "c5")
+    roofline = roofline_from(fam, dt_prof, "c5")
     plain = None
     if not args.no_plain:
-        dtp, outp, _ = timed(bpf, ctx, step_plain, 1, 1, "refactor+fp64 solve")
+        dtp, outp, _, _ = timed(bpf, ctx, step_plain, 1, 1, "refactor+fp64 solve", profile=False)
         plain = {"value": outp[-1][2] / dtp, "ms_per_step": dtp * 1e3, "solved": outp[-1][2],
                  "max_berr": outp[-1][1], "form": "FP64 refactorization + FP64 triangular solves"}
         log(f"[bench] refactor plain fp64: {dtp:.3f}s {outp[-1][0]}")
@@ -524,6 +533,9 @@ def main():
     ap.add_argument("--memory", type=int, default=40,
                     help="Krylov basis blocks kept (c2/c3); with --restart the restart length")
     ap.add_argument("--restart", action="store_true", help="restarted block GMRES (c2/c3)")
+    ap.add_argument("--bmgs-blocks", type=int, default=4,
+                    help="basis blocks projected together per block-MGS step of the mixed form "
+                         "(c2/c3; the plain FP64 form keeps the reference's one-block loop)")
     ap.add_argument("--poly-degree", type=int, default=None,
                     help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default=None,

@@ -29,6 +29,7 @@ class SolverParams(C.Structure):
         ("berr_tol", C.c_double), ("precision", C.c_int), ("block_width", C.c_int),
         ("max_refine", C.c_int), ("inner_rtol", C.c_double), ("reorthogonalization", C.c_int),
         ("poly_degree", C.c_int),
+        ("bmgs_blocks", C.c_int),
     ]
 
 

@@ -117,6 +117,7 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
     cfg.flexible = mixed;
     cfg.prec32 = mixed;
     cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
+    cfg.bmgs_blocks = std::max(1, prm.bmgs_blocks);
     cfg.groups = b.groups;
     b.arn.reset();
     b.arn.reset(new BlockArnoldi(ctx, b.A, b.N, cfg));

@@ -717,14 +717,31 @@ __global__ __launch_bounds__(kWyThreads) void hess_apply_kernel(HessArgs a) {
   xfer(0, 0, true);
   xfer(1, 1, true);
   __syncthreads();
+  // Block i + 2 of the column is untouched until step i + 1: its loads are
+  // issued before step i's products and land in the window afterwards.
+  constexpr int NE = (S * 16 + kWyThreads - 1) / kWyThreads;
+  double nxt[NE];
+  auto prefetch = [&](int i) {
+    const double *blk = hblock(a.cols, k, g, i, S);
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = tid + q * kWyThreads;
+      nxt[q] = e < S * 16 ? blk[(size_t)(e >> 4) * S + b0 + (e & 15)] : 0.0;
+    }
+  };
   for (int i = 0; i < k; ++i) {
+    if (i + 2 <= k) prefetch(i + 2);
     wy_apply_tile<S>(hblock(a.cols, i, g, i, S), htmat(a.cols, i, g, a.ng, S), seg, m1, hs, 2 * S);
     if (i + 1 < k) {
       // block i is final; slide the window down by one block
       xfer(i, 0, false);
       for (int e = tid; e < S * 16; e += kWyThreads) seg[e >> 4][e & 15] = seg[S + (e >> 4)][e & 15];
       __syncthreads();
-      xfer(i + 2, 1, true);
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const int e = tid + q * kWyThreads;
+        if (e < S * 16) seg[S + (e >> 4)][e & 15] = nxt[q];
+      }
       __syncthreads();
     }
   }

@@ -177,11 +177,15 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
   for (const auto &g : h_groups_)
     if (g.active) cells += (double)(rows_sum_ > 0 ? g.nr : n) * S_;
   if (cfg_.orth == ORTH_BMGS) {
-    for (int i = 0; i <= k; ++i) {
-      ProjArgs pa{vt, nullptr, i, 1, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
+    // Block MGS over steps of g consecutive basis blocks (g = 1: the
+    // reference's loop); within a step the blocks are projected together.
+    const int g = std::max(1, cfg_.bmgs_blocks);
+    for (int i = 0; i <= k; i += g) {
+      const int nb = std::min(g, k + 1 - i);
+      ProjArgs pa{vt, nullptr, i, nb, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
       pa.cells = cells;
-      launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, 0, gstr, false, part_);
-      GemmArgs ga{vt, nullptr, i, 1, p, col + (size_t)i * SS, 0, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
+      launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, SS, gstr, false, part_);
+      GemmArgs ga{vt, nullptr, i, nb, p, col + (size_t)i * SS, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
       ga.rows_sum = rows_sum_;
       ga.cells = cells;
       launch_block_gemm<TV, TV>(st_, ga, S_, ng_);

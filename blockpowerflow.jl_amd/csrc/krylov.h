@@ -65,6 +65,7 @@ struct ArnoldiConfig {
   bool f32 = false;            // basis / operator precision
   bool flexible = false;       // store Z_k = N^{-1} V_k
   int orth = 0;                // ORTH_BMGS / ORTH_BCGS / ORTH_BCGS2
+  int bmgs_blocks = 1;         // BMGS: basis blocks projected together per step
   bool prec32 = false;         // apply the preconditioner in FP32 (FP64 basis)
   // Explicit groups (row ranges, widths, slots) for many systems stacked in
   // one block-diagonal operator; empty = column slices of width s.

@@ -135,6 +135,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   // reorthogonalization=true two classical passes.  (A single classical pass
   // loses orthogonality on these Jacobians and stalls the solve.)
   cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
+  cfg.bmgs_blocks = std::max(1, prm.bmgs_blocks);
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
   Poly poly;
@@ -319,6 +320,7 @@ void bpf_solver_params_default(bpf_solver_params *prm) {
   prm->inner_rtol = 1e-6;
   prm->reorthogonalization = 0;
   prm->poly_degree = 0;
+  prm->bmgs_blocks = 1;
 }
 
 int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *d_B, int ldb,

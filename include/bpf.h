@@ -171,6 +171,10 @@ typedef struct {
   int reorthogonalization; /* second block Gram-Schmidt pass (reference kwarg) */
   int poly_degree;     /* mixed: degree of the GMRES polynomial applied on top of
                           the ILU(0) preconditioner (0 = ILU(0) alone)          */
+  int bmgs_blocks;     /* block MGS (src/block_gmres.jl:301-304) projects against
+                          this many consecutive basis blocks per step (one
+                          projection and one update launch for them; <= 1 =
+                          one block per step, the reference's loop)             */
 } bpf_solver_params;
 
 typedef struct {

@@ -69,6 +69,27 @@ def test_block_gmres_mixed(bpf, gen, oracle, nbus, p, s, prec, deg):
     assert np.all(_rel(Xh, Xd) < 1e-5)
 
 
+@pytest.mark.parametrize("nbus,p,s,prec,g", [(2000, 64, 64, "fp64", 2), (9000, 128, 0, "mixed", 4),
+                                              (2000, 128, 64, "fp64", 3), (300, 48, 0, "mixed", 8)])
+def test_grouped_bmgs_matches_one_block_steps(bpf, gen, oracle, nbus, p, s, prec, g):
+    # Projecting against g basis blocks per block-MGS step is the same
+    # Gram-Schmidt in exact arithmetic (the blocks are orthonormal): the
+    # residual histories agree to rounding, and so do the solutions.
+    S, A, B, Xd = _setup(bpf, gen, oracle, nbus, p, 35)
+    P = bpf.ilu0(A, p)
+    out = {}
+    for gg in (1, g):
+        X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-9, itmax=40,
+                                history=True, block_width=s, precision=prec, bmgs_blocks=gg)
+        out[gg] = (X.to_numpy(), np.array(st.residuals), st.niter)
+    assert out[1][2] == out[g][2]
+    # residual norms agree to rounding relative to the initial residual
+    np.testing.assert_allclose(out[g][1], out[1][1], rtol=1e-6, atol=1e-12 * out[1][1][0])
+    np.testing.assert_allclose(out[g][0], out[1][0], rtol=1e-6, atol=1e-8 * np.abs(out[1][0]).max())
+    be = oracle.backward_error(S.indptr, S.indices, S.data, out[g][0], S.B)
+    assert be.max() <= 1e-6
+
+
 def test_block_gmres_restart_and_warm_start(bpf, gen, oracle):
     S, A, B, Xd = _setup(bpf, gen, oracle, 300, 8, 35)
     P = bpf.ilu0(A)
