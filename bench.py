@@ -306,7 +306,7 @@ def bench_batch(args, bpf, gen):
     def step():
         _, st, solved = batch.solve(Bs, Xs, precision="mixed", berr_tol=BERR, memory=40, itmax=1000,
                                     timemax=120.0, poly_degree=args.poly_degree,
-                                    reorthogonalization=args.orth == "bcgs2")
+                                    reorthogonalization=args.orth == "bcgs2", bmgs_blocks=args.bmgs_blocks)
         nsolved = int(sum(int(ps[i]) for i in range(nsys) if solved[i]))
         return (f"block its {st.niter} solved RHS {nsolved}/{nrhs} max berr {st.max_berr:.2e}", st, nsolved)
 

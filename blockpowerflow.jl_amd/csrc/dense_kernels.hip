@@ -605,10 +605,11 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
       if (e < CR * S) hs[e / S][e % S] = pre[q];
     }
   };
-  // m1 = Y^T seg over rows r < rmax
-  dbl4 acc[NU];
+  // m1 = Y^T seg over rows r < rmax.  Two accumulators per tile (even and
+  // odd 4-row steps) keep two independent matrix-core chains in flight.
+  dbl4 acc[NU], acc2[NU];
 #pragma unroll
-  for (int u = 0; u < NU; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < NU; ++u) acc[u] = acc2[u] = dbl4{0.0, 0.0, 0.0, 0.0};
   fetch(H, 0);
   for (int r0 = 0; r0 < rmax; r0 += CR) {
     __syncthreads();
@@ -621,14 +622,17 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
       if (jt < NT && jt * 16 < r0 + CR) {
         const int j = jt * 16 + lc;
 #pragma unroll 4
-        for (int kk = 0; kk < CR; kk += 4) {
-          const int r = r0 + kk + lr;
-          const double h = hs[kk + lr][j];
+        for (int kk = 0; kk < CR; kk += 8) {
+          const int r = r0 + kk + lr, r2 = r + 4;
+          const double h = hs[kk + lr][j], h2 = hs[kk + 4 + lr][j];
           acc[u] = mfma4(r == j ? 1.0 : (r > j ? h : 0.0), seg[r][lc], acc[u]);
+          acc2[u] = mfma4(r2 == j ? 1.0 : (r2 > j ? h2 : 0.0), seg[r2][lc], acc2[u]);
         }
       }
     }
   }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) acc[u] += acc2[u];
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     const int jt = wave + NW * u;
@@ -638,7 +642,7 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
   }
   // m2 = T^T m1: M2[j][c] = sum_{q <= j} T[q][j] M1[q][c], T streamed by rows q
 #pragma unroll
-  for (int u = 0; u < NU; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < NU; ++u) acc[u] = acc2[u] = dbl4{0.0, 0.0, 0.0, 0.0};
   fetch(T, 0);
   for (int q0 = 0; q0 < S; q0 += CR) {
     __syncthreads();
@@ -651,13 +655,16 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
       if (jt < NT && q0 < jt * 16 + 16) {
         const int j = jt * 16 + lc;
 #pragma unroll 4
-        for (int kk = 0; kk < CR; kk += 4) {
-          const int q = q0 + kk + lr;
+        for (int kk = 0; kk < CR; kk += 8) {
+          const int q = q0 + kk + lr, q2 = q + 4;
           acc[u] = mfma4(q <= j ? hs[kk + lr][j] : 0.0, m1[q][lc], acc[u]);
+          acc2[u] = mfma4(q2 <= j ? hs[kk + 4 + lr][j] : 0.0, m1[q2][lc], acc2[u]);
         }
       }
     }
   }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) acc[u] += acc2[u];
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
@@ -676,15 +683,17 @@ __device__ __forceinline__ void wy_apply_tile(const double *__restrict__ H, cons
     if (r0 + CR < 2 * S) fetch(H, r0 + CR);
     if (wave < CR / 16) {
       const int rt = r0 / 16 + wave;
-      dbl4 a4 = {0.0, 0.0, 0.0, 0.0};
+      dbl4 a4 = {0.0, 0.0, 0.0, 0.0}, b4 = {0.0, 0.0, 0.0, 0.0};
       const int r = rt * 16 + lc;
-      const int jend = min(S, rt * 16 + 16);
+      const int jend = min(S, rt * 16 + 16);   // a multiple of 16
 #pragma unroll 4
-      for (int j0 = 0; j0 < jend; j0 += 4) {
-        const int j = j0 + lr;
-        const double h = hs[r - r0][j];
+      for (int j0 = 0; j0 < jend; j0 += 8) {
+        const int j = j0 + lr, j2 = j + 4;
+        const double h = hs[r - r0][j], h2 = hs[r - r0][j2];
         a4 = mfma4(r == j ? 1.0 : (r > j ? h : 0.0), m1[j][lc], a4);
+        b4 = mfma4(r == j2 ? 1.0 : (r > j2 ? h2 : 0.0), m1[j2][lc], b4);
       }
+      a4 += b4;
 #pragma unroll
       for (int q = 0; q < 4; ++q) seg[rt * 16 + lr + 4 * q][lc] -= a4[q];
     }

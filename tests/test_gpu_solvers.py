@@ -83,9 +83,12 @@ def test_grouped_bmgs_matches_one_block_steps(bpf, gen, oracle, nbus, p, s, prec
                                 history=True, block_width=s, precision=prec, bmgs_blocks=gg)
         out[gg] = (X.to_numpy(), np.array(st.residuals), st.niter)
     assert out[1][2] == out[g][2]
-    # residual norms agree to rounding relative to the initial residual
-    np.testing.assert_allclose(out[g][1], out[1][1], rtol=1e-6, atol=1e-12 * out[1][1][0])
-    np.testing.assert_allclose(out[g][0], out[1][0], rtol=1e-6, atol=1e-8 * np.abs(out[1][0]).max())
+    # Residual norms agree to rounding relative to the initial residual.  In
+    # the mixed form the FP32 preconditioner sees the rounding difference of
+    # its FP64 input, so the histories agree to FP32 level there.
+    rt = 1e-6 if prec == "fp64" else 1e-4
+    np.testing.assert_allclose(out[g][1], out[1][1], rtol=rt, atol=1e-12 * out[1][1][0])
+    np.testing.assert_allclose(out[g][0], out[1][0], rtol=rt, atol=1e-8 * np.abs(out[1][0]).max())
     be = oracle.backward_error(S.indptr, S.indices, S.data, out[g][0], S.B)
     assert be.max() <= 1e-6
 
