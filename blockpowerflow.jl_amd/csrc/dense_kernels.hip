@@ -190,6 +190,63 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
   }
 }
 
+// Psi_i = V_i^T W for width-16 groups (S = 16).  The matrix-core fragments
+// are read straight from global memory: lane l holds row r + l/16, column
+// l%16 of both V_i and W (16 consecutive values per row), so there is no LDS
+// staging and no barrier.  A workgroup owns one row chunk of one group and
+// all nb blocks: wavefront w takes blocks w, w + 4, ..., kNarrowBatch at a
+// time, so each wavefront reads the chunk of W once per batch.  Partial sums
+// have the layout of proj_partial_kernel's.
+constexpr int kNarrowBatch = 8;
+
+template <typename TVV, typename TW>
+__global__ __launch_bounds__(kB) void proj16_kernel(ProjArgs a) {
+  constexpr int NW = kB / 64, NB = kNarrowBatch, RU = 4;   // RU 4-row steps per round
+  const int chunk = blockIdx.x, g = blockIdx.y;
+  const GroupDesc gd = a.groups[g];
+  const int r0 = gd.r0 + chunk * a.chunk_rows;
+  const int r1 = gd.active ? min(gd.r0 + gd.nr, r0 + a.chunk_rows) : r0;
+  if (a.rows_sum > 0 && r0 >= r1) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane >> 4, lc = lane & 15;
+  const bool colok = lc < gd.w;
+  const TW *W = static_cast<const TW *>(a.W) + gd.c0;
+  for (int ib = wave; ib < a.nb; ib += NW * NB) {
+    const TVV *V[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = ib + u * NW;
+      V[u] = i < a.nb ? static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0 : nullptr;
+    }
+    dbl4 acc[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int rr = r0; rr < r1; rr += 4 * RU) {
+      double bw[RU], av[RU][NB];
+#pragma unroll
+      for (int t = 0; t < RU; ++t) {
+        const int r = rr + 4 * t + lr;
+        const bool ok = colok && r < r1;
+        bw[t] = ok ? to_d(W[(size_t)r * a.ldw + lc]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) av[t][u] = (ok && V[u]) ? to_d(V[u][(size_t)r * a.ldv + lc]) : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < RU; ++t)
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+          if (V[u]) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][u], bw[t], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!V[u]) continue;
+      double *out = a.part + ((size_t)(chunk * a.nb + ib + u * NW) * a.ng + g) * 256;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[(lr + 4 * q) * 16 + lc] = acc[u][q];
+    }
+  }
+}
+
 // out[i*o_istride + g*o_gstride + e] (+)= sum_chunk part[...]: each
 // workgroup owns 64 consecutive outputs; its 4 wavefronts split the chunks
 // and combine through LDS.
@@ -1196,7 +1253,12 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
                  bool accumulate, DevBuf &part) {
   if (a.nb <= 0 || a.ng <= 0) return;
   int nchunk;
-  if (a.rows_sum > 0) {
+  if (S == 16) {
+    // One workgroup per row chunk and group (all blocks): ~2048 of them.
+    const long rows = a.rows_sum > 0 ? a.rows_sum : (long)a.n * a.ng;
+    const int cr = (int)std::max<long>(64, (rows + 2047) / 2048);
+    a.chunk_rows = ((cr + 15) / 16) * 16;
+  } else if (a.rows_sum > 0) {
     // Stacked groups of uneven height: ~1024 non-empty workgroups over all
     // groups' rows (a.n is the tallest group; shorter ones leave chunks empty).
     const long target = std::max(64, 1024 / std::max(1, a.nb));
@@ -1216,7 +1278,7 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   dim3 grid(a.nb, nchunk, a.ng);
   const int tok = prof::begin(s);
   switch (S) {
-    case 16: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 16>), grid, dim3(kB), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((proj16_kernel<TVV, TW>), dim3(nchunk, a.ng), dim3(kB), 0, s, a); break;
     case 32: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 32>), grid, dim3(kB), 0, s, a); break;
     case 64: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 64>), grid, dim3(kB), 0, s, a); break;
     case 128: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 128>), grid, dim3(kB), 0, s, a); break;
