@@ -78,14 +78,17 @@ __global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double 
 constexpr int kLd = kDluNb + 1;
 
 // LU without pivoting of the w x w block at S (ld lds), w <= kDluNb, in LDS:
-// 16-column sub-panels; the sub-panel's 16 x 16 diagonal block by one
-// wavefront, its L column block and U row block by substitution (one thread
-// per row / column), the trailing update on matrix cores.  Then U^{-1} and
-// L^{-1} (unit) by column-oriented substitution, one thread per column.
+// 16-column sub-panels.  One wavefront factors the sub-panel's 16 x 16
+// diagonal block and inverts both of its triangles (lanes 0-15 the columns
+// of U_PP^{-1}, lanes 16-31 those of L_PP^{-1}); the L column block
+// (G_rP U_PP^{-1}), the U row block (L_PP^{-1} G_Pc) and the trailing update
+// then run on matrix cores.  The 16 x 16 inverses are also the diagonal
+// blocks of U^{-1} and L^{-1}, which are completed blockwise at the end.
 // A zero pivot at block row r sets *status = min(*status, base + r).
 __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int w, double *Linv, double *Uinv,
                                                         int *status, int base) {
   __shared__ double G[kDluNb * kLd];
+  __shared__ double sU[16][17], sL[16][17];   // current sub-panel's U_PP^{-1}, L_PP^{-1}
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane >> 4, lc = lane & 15;
   for (int e = tid; e < kDluNb * kLd; e += 1024) {
@@ -116,23 +119,79 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
     }
-    __syncthreads();
-    // L column block: rows r >= p0 + pw, G[r][p0:p0+pw] <- G[r][p0:p0+pw] U_PP^{-1}
-    for (int r = p0 + pw + tid; r < w; r += 1024) {
-      for (int kk = 0; kk < pw; ++kk) {
-        const int k = p0 + kk;
-        double v = G[r * kLd + k];
-        for (int mm = 0; mm < kk; ++mm) v -= G[r * kLd + p0 + mm] * G[(p0 + mm) * kLd + k];
-        G[r * kLd + k] = v / G[k * kLd + k];
+    // 16 x 16 triangular inverses (identity outside the pw x pw block).
+    if (wave == 0 && lane < 32) {
+      const int j = lane & 15;
+      double x[16];
+      if (lane < 16) {
+        // column j of U_PP^{-1}: x_i = (delta_ij - sum_{i<m<=j} U_im x_m) / U_ii
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+          double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+          for (int m = i + 1; m < 16; ++m)
+            if (m <= j && m < pw) v -= G[(p0 + i) * kLd + p0 + m] * x[m];
+          x[i] = (i <= j && i < pw) ? v / G[(p0 + i) * kLd + p0 + i] : (i == j ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sU[i][j] = x[i];
+      } else {
+        // column j of L_PP^{-1} (unit): x_i = delta_ij - sum_{j<=m<i} L_im x_m
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+          for (int m = 0; m < i; ++m)
+            if (m >= j && i < pw) v -= G[(p0 + i) * kLd + p0 + m] * x[m];
+          x[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sL[i][j] = x[i];
       }
     }
-    // U row block: columns c >= p0 + pw, G[p0:p0+pw][c] <- L_PP^{-1} G[p0:p0+pw][c]
-    for (int c = p0 + pw + tid; c < w; c += 1024) {
-      for (int kk = 0; kk < pw; ++kk) {
-        const int k = p0 + kk;
-        double v = G[k * kLd + c];
-        for (int mm = 0; mm < kk; ++mm) v -= G[k * kLd + p0 + mm] * G[(p0 + mm) * kLd + c];
-        G[k * kLd + c] = v;
+    __syncthreads();
+    // L column block G[r][P] <- G[r][P] U_PP^{-1} and U row block
+    // G[P][c] <- L_PP^{-1} G[P][c], one 16 x 16 tile per wavefront turn; each
+    // wavefront reads its whole tile before overwriting it.
+    {
+      const int t0 = p0 + 16;
+      const int nt = (w - t0 + 15) / 16;
+      for (int tt = wave; tt < 2 * nt; tt += 16) {
+        const int t = tt >> 1;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        if ((tt & 1) == 0) {
+          const int rb = t0 + t * 16;
+#pragma unroll
+          for (int k0 = 0; k0 < 16; k0 += 4) {
+            const int k = k0 + lr;
+            acc = mfma4(rb + lc < w ? G[(rb + lc) * kLd + p0 + k] : 0.0, sU[k][lc], acc);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = rb + lr + 4 * q;
+            if (r < w && lc < pw) G[r * kLd + p0 + lc] = acc[q];
+          }
+        } else {
+          const int cb = t0 + t * 16;
+#pragma unroll
+          for (int k0 = 0; k0 < 16; k0 += 4) {
+            const int k = k0 + lr;
+            acc = mfma4(sL[lc][k], cb + lc < w ? G[(p0 + k) * kLd + cb + lc] : 0.0, acc);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = lr + 4 * q, c = cb + lc;
+            if (r < pw && c < w) G[(p0 + r) * kLd + c] = acc[q];
+          }
+        }
+      }
+    }
+    // The diagonal blocks of U^{-1} and L^{-1} (completed below).
+    for (int e = tid; e < 256; e += 1024) {
+      const int i = e >> 4, j = e & 15;
+      if (i < pw && j < pw) {
+        Uinv[(size_t)(p0 + i) * kDluNb + p0 + j] = sU[i][j];
+        Linv[(size_t)(p0 + i) * kDluNb + p0 + j] = sL[i][j];
       }
     }
     __syncthreads();
@@ -171,15 +230,11 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
   __shared__ double dg[kDluNb];
   __shared__ double scr[kDluNb / 16][16][17];
   const int nb = (w + 15) / 16;
-  for (int c = tid; c < w; c += 1024) dg[c] = 1.0 / G[c * kLd + c];
-  __syncthreads();
-  for (int c = tid; c < w; c += 1024) {
-    const int rb = (c / 16) * 16;
-    for (int r = c - 1; r >= rb; --r) {
-      double v = G[r * kLd + c] * dg[c];
-      for (int mm = r + 1; mm < c; ++mm) v += G[r * kLd + mm] * G[c * kLd + mm];
-      G[c * kLd + r] = -v * dg[r];
-    }
+  // X_II: the sub-panel inverses stored during the factorization
+  for (int e = tid; e < w * 16; e += 1024) {
+    const int c = e >> 4, r = (c / 16) * 16 + (e & 15);
+    if (r == c) dg[c] = Uinv[(size_t)c * kDluNb + c];
+    else if (r < c) G[c * kLd + r] = Uinv[(size_t)r * kDluNb + c];
   }
   __syncthreads();
   auto xu = [&](int I, int J, int k, int n) -> double {   // X_IJ[k][n], I <= J
@@ -228,14 +283,10 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
     if (r > mm) G[mm * kLd + r] = S[(size_t)r * lds + mm];
   }
   __syncthreads();
-  // Z_II: column c by thread c within its diagonal block
-  for (int c = tid; c < w; c += 1024) {
-    const int re = min(w, (c / 16) * 16 + 16);
-    for (int r = c + 1; r < re; ++r) {
-      double v = G[c * kLd + r];
-      for (int mm = c + 1; mm < r; ++mm) v += G[mm * kLd + r] * G[mm * kLd + c];
-      G[r * kLd + c] = -v;
-    }
+  // Z_II: the sub-panel inverses stored during the factorization
+  for (int e = tid; e < w * 16; e += 1024) {
+    const int c = e >> 4, r = (c / 16) * 16 + (e & 15);
+    if (r > c && r < w) G[r * kLd + c] = Linv[(size_t)r * kDluNb + c];
   }
   __syncthreads();
   auto zl = [&](int I, int J, int k, int n) -> double {   // Z_IJ[k][n], I >= J
