@@ -81,14 +81,16 @@ def default_context() -> Context:
     return _DEFAULT
 
 
-_OPTIONS = {"syncfree": 1, "autotune": 2}
+_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3}
 
 
 def set_option(name: str, value) -> None:
     """Process-wide execution option: "syncfree" (default False) runs each
     triangular solve / factorization as one sync-free launch instead of one
     launch per wide dependency level; "autotune" (default True) times the
-    SpMM launch shapes on the first product of each matrix and width."""
+    SpMM launch shapes on the first product of each matrix and width;
+    "overlap" (default True) runs block GMRES's Hessenberg update of each
+    iteration on a second stream beside the next iteration."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -76,6 +76,7 @@ int bpf_set_option(int option, long long value) {
   return guard([&] {
     if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
     else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
+    else if (option == BPF_OPT_OVERLAP) g_overlap_hess = value != 0;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }
@@ -85,6 +86,7 @@ int bpf_get_option(int option, long long *value) {
     BPF_REQUIRE(value, "null argument");
     if (option == BPF_OPT_SYNCFREE) *value = g_syncfree ? 1 : 0;
     else if (option == BPF_OPT_AUTOTUNE) *value = g_autotune ? 1 : 0;
+    else if (option == BPF_OPT_OVERLAP) *value = g_overlap_hess ? 1 : 0;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }

@@ -64,9 +64,10 @@ void rethrow(int rc) {
 
 // Per-slot host copies of segmented column statistics.
 void group_stats(bpf_ctx ctx, Bin &b, const double *X, int mode, std::vector<double> &out) {
-  b.stat.ensure((size_t)std::max(1, b.nslots) * sizeof(double));
+  const size_t ns = std::max(1, b.nslots);
+  b.stat.ensure((1 + (size_t)colstat_groups_blocks(b.gmax)) * ns * sizeof(double));
   colstat_groups<double>(ctx->stream, b.d_groups.as<GroupDesc>(), (int)b.groups.size(), b.gmax, b.p, X,
-                         b.p, mode, b.stat.as<double>(), b.nslots);
+                         b.p, mode, b.stat.as<double>(), b.nslots, b.stat.as<double>() + ns);
   out.resize(b.nslots);
   BPF_CHECK_HIP(hipMemcpyAsync(out.data(), b.stat.ptr, b.nslots * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));

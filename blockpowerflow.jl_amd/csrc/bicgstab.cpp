@@ -40,7 +40,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     const size_t SS = (size_t)S * S;
     for (DevBuf *b : {&K, &D, &al, &be}) b->ensure(SS * sizeof(double));
     piv.ensure(S * sizeof(int));
-    dots.ensure(4 * sizeof(double));
+    dots.ensure((4 + kDotBlocks) * sizeof(double));
     std::vector<GroupDesc> hg{{0, p, 0, n, 0, 1 << 30, 1, 0}};
     upload(grp, hg, s);
     const GroupDesc *dg = grp.as<GroupDesc>();
@@ -55,7 +55,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     BPF_CHECK_HIP(hipMemcpyAsync(R.ptr, B.ptr, mb, hipMemcpyDeviceToDevice, s));   // :215
     double hd[2];
     auto dot = [&](const double *x, const double *y) {
-      launch_dot<double>(s, m, x, y, dots.as<double>());
+      launch_dot<double>(s, m, x, y, dots.as<double>(), dots.as<double>() + 4);
       BPF_CHECK_HIP(hipMemcpyAsync(hd, dots.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
       BPF_CHECK_HIP(hipStreamSynchronize(s));
       return hd[0];

@@ -141,6 +141,9 @@ struct TriOp {
 // Process-wide switch between the sync-free single-launch triangular
 // solves / factorizations and the level-by-level launches (default).
 extern bool g_syncfree;
+// Process-wide switch: block-Hessenberg updates on a second stream, beside
+// the next iteration (default on).
+extern bool g_overlap_hess;
 // Process-wide switch for the SpMM launch-shape autotuner (default on), and
 // removal of a matrix's cached shapes (its row-pointer array is released).
 extern bool g_autotune;

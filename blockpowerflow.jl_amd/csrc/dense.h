@@ -144,10 +144,13 @@ void launch_scale_ycols(hipStream_t s, double *Y, int K, int ng, int S, const Gr
 void launch_scale_cols(hipStream_t s, int n, int p, double *X, int ld, const double *scale, bool invert);
 // Per-group forms for systems stacked by rows (block-diagonal operator):
 // out[slot + c] = max_r |X[r, c0 + c]| (mode 0) or sum_r X[r, c0 + c]^2
-// (mode 1) over the group's rows; out holds nslots entries.
+// (mode 1) over the group's rows; out holds nslots entries.  Mode 1 needs
+// part with colstat_groups_blocks(gmax_rows) * nslots doubles (sums are
+// formed from per-block partials in a fixed order: reproducible).
+int colstat_groups_blocks(int gmax_rows);
 template <typename T>
 void colstat_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
-                    const T *X, int ld, int mode, double *out, int nslots);
+                    const T *X, int ld, int mode, double *out, int nslots, double *part);
 // X[r, c0 + c] /= scale[slot + c] (invert) or *= over each group's rows.
 void launch_scale_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
                          double *X, int ld, const double *scale, bool invert);
@@ -160,13 +163,19 @@ void launch_pack_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax
 void launch_lu_small(hipStream_t s, double *A, int *piv, int S, const GroupDesc *groups, int ng);
 void launch_lu_small_solve(hipStream_t s, const double *A, const int *piv, double *B, int S,
                            const GroupDesc *groups, int ng);
-// out = sum over the n x p row-major block of x .* y (FP64), device scalar.
+// out = sum over the n x p row-major block of x .* y (FP64), device scalar,
+// from kDotBlocks-or-fewer per-block partials in part, added in a fixed order.
+constexpr int kDotBlocks = 2048;
 template <typename T>
-void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out);
+void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out, double *part);
 
 void colmax_colmajor(hipStream_t s, int n, int p, const double *X, int ld, double *out);
+// Per-column inf-norms (mode 0) or sums of squares (mode 1) of a row-major
+// n x p block into out[p]; mode 1 needs part with colstat_blocks(n) * p doubles.
+int colstat_blocks(int n);
 template <typename T>
-void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out);
+void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out,
+                      double *part = nullptr);
 void axpby(hipStream_t s, size_t m, double alpha, const double *x, double beta, double *y);
 // y[g*stride + e] += x[g*stride + e] for e < count, g < ng (one launch).
 void axpby_groups(hipStream_t s, int ng, size_t count, size_t stride, const double *x, double *y);

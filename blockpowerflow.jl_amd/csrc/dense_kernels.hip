@@ -431,7 +431,7 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   __shared__ double dinv[128];
   __shared__ double scr[8][16][17];
   __shared__ int s_fail;
-  __shared__ double s_norm;
+  __shared__ double s_norm, s_part[16];
   const int g = blockIdx.x;
   if (!a.groups[g].active) return;
   const int w = a.groups[g].w;
@@ -449,7 +449,13 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
     loc += v * v;
   }
   for (int off = 32; off > 0; off >>= 1) loc += __shfl_xor(loc, off, 64);
-  if (lane == 0) atomicAdd(&s_norm, loc);
+  if (lane == 0) s_part[wave] = loc;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += s_part[q];   // fixed order
+    s_norm = t;
+  }
   __syncthreads();
   const double gnorm = sqrt(s_norm);
   const int nb = (w + 15) / 16;
@@ -1168,11 +1174,15 @@ __global__ __launch_bounds__(kB) void colmax_cm_kernel(int n, const T *X, int ld
   if (threadIdx.x == 0) out[c] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// Row-major n x p: per-column inf-norm (mode 0) or sum of squares (mode 1),
-// accumulated with atomics into out[p] (must be zeroed by the caller).
+// Row-major n x p: per-column inf-norm (mode 0: atomic max into out[p],
+// zeroed by the caller, exact in any order) or sum of squares (mode 1: the
+// block's sum, its 4 wavefronts combined in a fixed order, into
+// part[blockIdx.x * p + c]; sum_parts_kernel adds the blocks in a fixed
+// order, so the result does not depend on scheduling).
 template <typename T>
 __global__ __launch_bounds__(kB) void colstat_rm_kernel(int n, int p, const T *X, int ld, int mode,
-                                                        double *out) {
+                                                        double *out, double *part) {
+  __shared__ double red[kB / 64][64];
   const int c = blockIdx.y * 64 + (threadIdx.x & 63);
   const int rstride = (kB / 64) * gridDim.x;
   double m = 0.0;
@@ -1182,8 +1192,27 @@ __global__ __launch_bounds__(kB) void colstat_rm_kernel(int n, int p, const T *X
       m = mode == 0 ? fmax(m, fabs(v)) : m + v * v;
     }
     if (mode == 0) atomic_max_nonneg(out + c, m);
-    else atomicAdd(out + c, m);
   }
+  if (mode == 0) return;
+  red[threadIdx.x >> 6][threadIdx.x & 63] = m;
+  __syncthreads();
+  if (threadIdx.x < 64 && c < p)
+    part[(size_t)blockIdx.x * p + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                                       (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// out[c] = sum_b part[b * stride + c] for b < nparts, c < ncol: one block
+// per column, a fixed split over its threads and a fixed reduction tree.
+__global__ __launch_bounds__(kB) void sum_parts_kernel(const double *part, int nparts, size_t stride,
+                                                       double *out) {
+  __shared__ double red[kB / 64];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += kB) s += part[(size_t)b * stride + c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[c] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // R = B - A*X for row-major FP64 blocks (residual in working precision).
@@ -1399,18 +1428,25 @@ void colmax_colmajor(hipStream_t s, int n, int p, const double *X, int ld, doubl
   BPF_CHECK_HIP(hipGetLastError());
 }
 
+int colstat_blocks(int n) { return std::max(1, std::min(1024, (n + 255) / 256)); }
+
 template <typename T>
-void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out) {
+void colstat_rowmajor(hipStream_t s, int n, int p, const T *X, int ld, int mode, double *out, double *part) {
   if (p <= 0) return;
   fill_double(s, out, 0.0, p);
   if (n <= 0) return;
-  const int gx = std::max(1, std::min(1024, (n + 255) / 256));
+  BPF_REQUIRE(mode == 0 || part, "column sums need a partial-sum buffer");
+  const int gx = colstat_blocks(n);
   dim3 grid(gx, (p + 63) / 64);
-  hipLaunchKernelGGL((colstat_rm_kernel<T>), grid, dim3(kB), 0, s, n, p, X, ld, mode, out);
+  hipLaunchKernelGGL((colstat_rm_kernel<T>), grid, dim3(kB), 0, s, n, p, X, ld, mode, out, part);
   BPF_CHECK_HIP(hipGetLastError());
+  if (mode == 1) {
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(p), dim3(kB), 0, s, part, gx, (size_t)p, out);
+    BPF_CHECK_HIP(hipGetLastError());
+  }
 }
-template void colstat_rowmajor<double>(hipStream_t, int, int, const double *, int, int, double *);
-template void colstat_rowmajor<float>(hipStream_t, int, int, const float *, int, int, double *);
+template void colstat_rowmajor<double>(hipStream_t, int, int, const double *, int, int, double *, double *);
+template void colstat_rowmajor<float>(hipStream_t, int, int, const float *, int, int, double *, double *);
 
 void axpby_groups(hipStream_t s, int ng, size_t count, size_t stride, const double *x, double *y) {
   if (count == 0 || ng <= 0) return;
@@ -1463,21 +1499,30 @@ __global__ __launch_bounds__(256) void scale_cols_kernel(int n, int p, double *X
 }
 
 // Grid (row chunks, group, 64-column slice); threads: 64 columns x 4 rows.
+// Mode 1 writes the block's sum (wavefronts combined in a fixed order) to
+// part[blockIdx.x * nslots + slot], reduced by sum_parts_kernel.
 template <typename T>
 __global__ __launch_bounds__(kB) void colstat_groups_kernel(const GroupDesc *groups, const T *X, int ld,
-                                                            int mode, double *out) {
+                                                            int mode, double *out, double *part, int nslots) {
+  __shared__ double red[kB / 64][64];
   const GroupDesc gd = groups[blockIdx.y];
   const int c = blockIdx.z * 64 + (threadIdx.x & 63);
-  if (c >= gd.w) return;
   const int rend = gd.r0 + gd.nr;
   const int rstride = (kB / 64) * gridDim.x;
   double m = 0.0;
-  for (int r = gd.r0 + blockIdx.x * (kB / 64) + (threadIdx.x >> 6); r < rend; r += rstride) {
-    const double v = to_d(X[(size_t)r * ld + gd.c0 + c]);
-    m = mode == 0 ? fmax(m, fabs(v)) : m + v * v;
+  if (c < gd.w) {
+    for (int r = gd.r0 + blockIdx.x * (kB / 64) + (threadIdx.x >> 6); r < rend; r += rstride) {
+      const double v = to_d(X[(size_t)r * ld + gd.c0 + c]);
+      m = mode == 0 ? fmax(m, fabs(v)) : m + v * v;
+    }
+    if (mode == 0) atomic_max_nonneg(out + gd.slot + c, m);
   }
-  if (mode == 0) atomic_max_nonneg(out + gd.slot + c, m);
-  else atomicAdd(out + gd.slot + c, m);
+  if (mode == 0) return;
+  red[threadIdx.x >> 6][threadIdx.x & 63] = m;
+  __syncthreads();
+  if (threadIdx.x < 64 && c < gd.w)
+    part[(size_t)blockIdx.x * nslots + gd.slot + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                                                      (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kB) void scale_groups_kernel(const GroupDesc *groups, double *X, int ld,
@@ -1594,13 +1639,17 @@ __global__ __launch_bounds__(256) void lu_small_solve_kernel(const double *A0, c
   }
 }
 
+// Per-block partial dot products (fixed reduction tree) into part[blockIdx.x].
 template <typename T>
-__global__ __launch_bounds__(256) void dot_kernel(size_t m, const T *x, const T *y, double *out) {
+__global__ __launch_bounds__(256) void dot_kernel(size_t m, const T *x, const T *y, double *part) {
+  __shared__ double red[4];
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (size_t)gridDim.x * 256)
     s += static_cast<double>(x[i]) * static_cast<double>(y[i]);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 }  // namespace
@@ -1629,18 +1678,27 @@ dim3 group_grid(int ng, int gmax_rows, int wmax, int rows_per_wg) {
 }
 }  // namespace
 
+int colstat_groups_blocks(int gmax_rows) { return group_grid(1, gmax_rows, 1, 256).x; }
+
 template <typename T>
 void colstat_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
-                    const T *X, int ld, int mode, double *out, int nslots) {
+                    const T *X, int ld, int mode, double *out, int nslots, double *part) {
   if (nslots <= 0) return;
   fill_double(s, out, 0.0, nslots);
   if (ng <= 0 || gmax_rows <= 0 || wmax <= 0) return;
-  hipLaunchKernelGGL((colstat_groups_kernel<T>), group_grid(ng, gmax_rows, wmax, 256), dim3(kB), 0, s,
-                     groups, X, ld, mode, out);
+  BPF_REQUIRE(mode == 0 || part, "column sums need a partial-sum buffer");
+  const dim3 grid = group_grid(ng, gmax_rows, wmax, 256);
+  if (mode == 1) fill_double(s, part, 0.0, (size_t)grid.x * nslots);   // slots of empty columns
+  hipLaunchKernelGGL((colstat_groups_kernel<T>), grid, dim3(kB), 0, s, groups, X, ld, mode, out, part,
+                     nslots);
   BPF_CHECK_HIP(hipGetLastError());
+  if (mode == 1) {
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(nslots), dim3(kB), 0, s, part, (int)grid.x, (size_t)nslots, out);
+    BPF_CHECK_HIP(hipGetLastError());
+  }
 }
 template void colstat_groups<double>(hipStream_t, const GroupDesc *, int, int, int, const double *, int,
-                                     int, double *, int);
+                                     int, double *, int, double *);
 
 void launch_scale_groups(hipStream_t s, const GroupDesc *groups, int ng, int gmax_rows, int wmax,
                          double *X, int ld, const double *scale, bool invert) {
@@ -1673,15 +1731,17 @@ void launch_lu_small_solve(hipStream_t s, const double *A, const int *piv, doubl
 }
 
 template <typename T>
-void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out) {
+void launch_dot(hipStream_t s, size_t m, const T *x, const T *y, double *out, double *part) {
   fill_double(s, out, 0.0, 1);
   if (m == 0) return;
-  const unsigned nb = (unsigned)std::min<size_t>(2048, (m + 255) / 256);
-  hipLaunchKernelGGL((dot_kernel<T>), dim3(nb), dim3(256), 0, s, m, x, y, out);
+  const unsigned nb = (unsigned)std::min<size_t>(kDotBlocks, (m + 255) / 256);
+  hipLaunchKernelGGL((dot_kernel<T>), dim3(nb), dim3(256), 0, s, m, x, y, part);
+  BPF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(kB), 0, s, part, (int)nb, (size_t)1, out);
   BPF_CHECK_HIP(hipGetLastError());
 }
-template void launch_dot<double>(hipStream_t, size_t, const double *, const double *, double *);
-template void launch_dot<float>(hipStream_t, size_t, const float *, const float *, double *);
+template void launch_dot<double>(hipStream_t, size_t, const double *, const double *, double *, double *);
+template void launch_dot<float>(hipStream_t, size_t, const float *, const float *, double *, double *);
 
 }  // namespace bpf
 

@@ -19,6 +19,8 @@
 
 namespace bpf {
 
+bool g_overlap_hess = true;
+
 void *BlockArena::block(int i) {
   while (i >= (int)ptrs.size()) {
     DevBuf b;
@@ -56,6 +58,10 @@ void PtrTable::set(int i, void *ptr, hipStream_t s) {
   if (h[i] == ptr) return;
   h[i] = ptr;
   BPF_CHECK_HIP(hipMemcpyAsync(d.as<void *>() + i, h + i, sizeof(void *), hipMemcpyHostToDevice, s));
+}
+
+void PtrTable::reserve(int n, hipStream_t s) {
+  if (n > cap) set(n - 1, nullptr, s);   // grows the table; the new entry stays null
 }
 
 BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConfig &cfg)
@@ -100,6 +106,10 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   rnorm_.ensure(std::max(1, nslots_) * sizeof(double));
   flags_.ensure(16);
   ctx->ensure_small(std::max(nslots_, 16));
+  BPF_CHECK_HIP(hipStreamCreateWithFlags(&hs_, hipStreamNonBlocking));
+  BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
+  BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_hess_, hipEventDisableTiming));
+  BPF_CHECK_HIP(hipHostMalloc((void **)&h_rn_, std::max(1, nslots_) * sizeof(double), hipHostMallocDefault));
 }
 
 void BlockArnoldi::set_active(int g, bool active, int K) {
@@ -109,7 +119,13 @@ void BlockArnoldi::set_active(int g, bool active, int K) {
 
 void BlockArnoldi::upload_groups() { upload(d_groups_, h_groups_, st_); }
 
-BlockArnoldi::~BlockArnoldi() = default;
+BlockArnoldi::~BlockArnoldi() {
+  if (hs_) (void)hipStreamSynchronize(hs_);
+  if (h_rn_) (void)hipHostFree(h_rn_);
+  if (ev_main_) (void)hipEventDestroy(ev_main_);
+  if (ev_hess_) (void)hipEventDestroy(ev_hess_);
+  if (hs_) (void)hipStreamDestroy(hs_);
+}
 
 double *BlockArnoldi::col_region(int k) {
   while ((int)cols_.size() <= k) {
@@ -282,10 +298,16 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   vtab_.set(0, V_.block(0), st_);
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
   cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
-  double *h_rn = ctx_->h_small;
-  int it = 0;
-  while (it < maxit) {
-    const int k = it;
+  double *h_rn = h_rn_;
+  const bool ovl = g_overlap_hess;
+  if (ovl) {
+    // no table growth (and device-table replacement) while the second stream runs
+    coltab_.reserve(maxit + 2, st_);
+    vtab_.reserve(maxit + 2, st_);
+    if (flex) ztab_.reserve(maxit + 2, st_);
+  }
+  // Preconditioner, SpMM, Gram-Schmidt and CholQR2 of iteration k (st_).
+  auto main_work = [&](int k) {
     // P = N^{-1} V_k
     TV *Vk = static_cast<TV *>(V_.block(k));
     TV *Pk = Vk;
@@ -323,17 +345,51 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     // V_{k+1} C = W; C into block k+1 of each group's column.
     cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr);
     vtab_.set(k + 1, V_.block(k + 1), st_);
-    // QR update of the block Hessenberg matrix.
+  };
+  // QR update of the block Hessenberg matrix for iteration k and the copy
+  // of the residual estimates; ev_hess_ marks its end.
+  auto hess_work = [&](int k) {
+    hipStream_t hs = st_;
+    if (ovl) {
+      hs = hs_;
+      BPF_CHECK_HIP(hipEventRecord(ev_main_, st_));   // after everything queued so far
+      BPF_CHECK_HIP(hipStreamWaitEvent(hs, ev_main_, 0));
+    }
     HessArgs ha{coltab_.dev<double>(), k, S_, ng_, dg};
-    launch_hess_apply(st_, ha, ng_);
+    launch_hess_apply(hs, ha, ng_);
     HessQrArgs qa{coltab_.dev<double>(), k, S_, ng_, E_.as<double>(), dg, rnorm_.as<double>()};
-    launch_hess_qr(st_, qa, ng_);
-    BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)nslots_ * sizeof(double), hipMemcpyDeviceToHost, st_));
-    BPF_CHECK_HIP(hipStreamSynchronize(st_));
-    ++it;
-    k_ = it;
-    if (stop(h_rn, it)) break;
+    launch_hess_qr(hs, qa, ng_);
+    BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)nslots_ * sizeof(double), hipMemcpyDeviceToHost, hs));
+    BPF_CHECK_HIP(hipEventRecord(ev_hess_, hs));
+  };
+  int it = 0;
+  for (;;) {
+    const bool more = it < maxit;
+    if (ovl) {
+      // Iteration `it` is queued before iteration it-1 is tested, so the
+      // update of it-1 (second stream) runs beside it; if it-1 stops the
+      // solve, its work is dropped (iterations past k_ are never used).
+      if (more) main_work(it);
+      if (it > 0) {
+        BPF_CHECK_HIP(hipEventSynchronize(ev_hess_));
+        k_ = it;
+        if (stop(h_rn, it)) break;
+      }
+      if (!more) break;
+      hess_work(it);
+      ++it;
+    } else {
+      if (!more) break;
+      main_work(it);
+      hess_work(it);
+      BPF_CHECK_HIP(hipEventSynchronize(ev_hess_));
+      ++it;
+      k_ = it;
+      if (stop(h_rn, it)) break;
+    }
   }
+  // the solution assembly below reads the last update's results
+  BPF_CHECK_HIP(hipStreamWaitEvent(st_, ev_hess_, 0));
   if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)nslots_ * sizeof(double));
   int fl = 0;
   BPF_CHECK_HIP(hipMemcpy(&fl, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost));

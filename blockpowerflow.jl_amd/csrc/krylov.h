@@ -40,6 +40,9 @@ struct PtrTable {
   PtrTable &operator=(const PtrTable &) = delete;
   ~PtrTable();
   void set(int i, void *ptr, hipStream_t s);
+  // Capacity for entries 0..n-1, so that later set() calls never replace the
+  // device table (work on another stream may be reading it).
+  void reserve(int n, hipStream_t s);
   template <typename T> T *const *dev() const { return d.as<T *>(); }
 };
 
@@ -124,6 +127,11 @@ class BlockArnoldi {
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
   hipStream_t st_;
+  // Second stream: the Hessenberg update of iteration k runs there while
+  // iteration k+1's preconditioner, SpMM and Gram-Schmidt are queued on st_.
+  hipStream_t hs_ = nullptr;
+  hipEvent_t ev_main_ = nullptr, ev_hess_ = nullptr;
+  double *h_rn_ = nullptr;     // pinned: residual estimates of the last update
   std::vector<GroupDesc> h_groups_;
   DevBuf d_groups_;
   BlockArena V_, Z_;           // basis and (flexible) preconditioned basis

@@ -165,6 +165,8 @@ void Poly::build(bpf_ctx ctx, bpf_csr A, bpf_prec N, int degree) {
   std::vector<GroupDesc> hg{{0, 1, 0, n, 0, 1 << 30, 1, 0}};
   upload(grp, hg, s);
   hcoef.ensure((size_t)(d + 1) * 16 * 16 * sizeof(double));
+  DevBuf dpart;
+  dpart.ensure(kDotBlocks * sizeof(double));
   tab.ensure((size_t)(d + 1) * sizeof(void *));
   std::vector<const void *> ptrs(d + 1);
   for (int i = 0; i <= d; ++i) ptrs[i] = Vp + (size_t)i * n;
@@ -192,7 +194,7 @@ void Poly::build(bpf_ctx ctx, bpf_csr A, bpf_prec N, int degree) {
       BPF_CHECK_HIP(hipStreamSynchronize(s));
       for (int i = 0; i <= j; ++i) H[(size_t)i * d + j] += hs[i];
     }
-    launch_dot<double>(s, n, t, t, hcoef.as<double>());
+    launch_dot<double>(s, n, t, t, hcoef.as<double>(), dpart.as<double>());
     BPF_CHECK_HIP(hipMemcpyAsync(hs, hcoef.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     BPF_CHECK_HIP(hipStreamSynchronize(s));
     const double h = std::sqrt(hs[0]);

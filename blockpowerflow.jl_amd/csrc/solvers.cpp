@@ -37,8 +37,9 @@ struct Blocks {
 };
 
 void col_norms2(bpf_ctx ctx, int n, int p, const double *R, DevBuf &stat, std::vector<double> &out) {
-  stat.ensure(3 * (size_t)std::max(1, p) * sizeof(double));
-  colstat_rowmajor<double>(ctx->stream, n, p, R, p, 1, stat.as<double>());
+  const size_t pp = std::max(1, p);
+  stat.ensure((3 + (size_t)colstat_blocks(n)) * pp * sizeof(double));
+  colstat_rowmajor<double>(ctx->stream, n, p, R, p, 1, stat.as<double>(), stat.as<double>() + 3 * pp);
   out.resize(p);
   BPF_CHECK_HIP(hipMemcpyAsync(out.data(), stat.ptr, p * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   BPF_CHECK_HIP(hipStreamSynchronize(ctx->stream));

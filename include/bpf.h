@@ -61,8 +61,13 @@ const char *bpf_version(void);
  * the sync-free form is slower there; it is kept as an option.
  * BPF_OPT_AUTOTUNE: 1 (default) = the first SpMM of a matrix at a given
  * block width times each launch shape (lanes per row x columns per lane) on
- * the actual operands and keeps the fastest; 0 = a fixed shape per width. */
-enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2 };
+ * the actual operands and keeps the fastest; 0 = a fixed shape per width.
+ * BPF_OPT_OVERLAP: 1 (default) = block GMRES runs the block-Hessenberg QR
+ * update of iteration k (src/block_gmres.jl:318-342) on a second stream
+ * while iteration k+1's preconditioner, SpMM and Gram-Schmidt run, testing
+ * convergence of k meanwhile (one iteration's work is dropped at the end);
+ * 0 = one iteration after another.  Results are identical either way. */
+enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

@@ -75,6 +75,28 @@ def test_batch_uneven_with_restart_and_zero_rhs(bpf, gen, oracle):
         batch.destroy()
 
 
+def test_batch_overlapped_hessenberg_update_is_exact(bpf, gen):
+    # Groups stop at different iterations; with the Hessenberg updates on a
+    # second stream (and one speculative iteration queued) the results are
+    # bitwise those of the one-iteration-after-another order.
+    systems = _systems(gen, 9, 83, ps=[8, 16, 32])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    out = {}
+    try:
+        for ovl in (0, 1):
+            bpf.set_option("overlap", ovl)
+            Xs, st, solved = bpf.block_gmres_batched(As, Bs, precision="mixed", berr_tol=1e-10, memory=40,
+                                                     itmax=600)
+            out[ovl] = ([X.to_numpy() for X in Xs], st.niter, solved.copy())
+    finally:
+        bpf.set_option("overlap", 1)
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for a, b in zip(out[0][0], out[1][0]):
+        np.testing.assert_array_equal(b, a)
+
+
 def test_batch_errors(bpf, gen):
     _, S = gen.make_system(30, 4, seed=74)
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)

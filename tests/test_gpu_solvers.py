@@ -93,6 +93,29 @@ def test_grouped_bmgs_matches_one_block_steps(bpf, gen, oracle, nbus, p, s, prec
     assert be.max() <= 1e-6
 
 
+@pytest.mark.parametrize("nbus,p,s,prec,restart", [(2000, 64, 32, "fp64", False), (9000, 128, 0, "mixed", False),
+                                                   (300, 16, 0, "fp64", True)])
+def test_overlapped_hessenberg_update_is_exact(bpf, gen, oracle, nbus, p, s, prec, restart):
+    # The Hessenberg update on the second stream runs the same kernels on the
+    # same data as the one-iteration-after-another order: bitwise equal
+    # solutions, histories and iteration counts.
+    S, A, B, Xd = _setup(bpf, gen, oracle, nbus, p, 36)
+    P = bpf.ilu0(A, p)
+    out = {}
+    try:
+        for ovl in (0, 1):
+            bpf.set_option("overlap", ovl)
+            X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=12 if restart else 30, restart=restart,
+                                    atol=0.0, rtol=1e-9, itmax=60, history=True, block_width=s,
+                                    precision=prec, berr_tol=1e-10 if prec == "mixed" else 0.0)
+            out[ovl] = (X.to_numpy(), np.array(st.residuals), st.niter, st.solved)
+    finally:
+        bpf.set_option("overlap", 1)
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+
+
 def test_block_gmres_restart_and_warm_start(bpf, gen, oracle):
     S, A, B, Xd = _setup(bpf, gen, oracle, 300, 8, 35)
     P = bpf.ilu0(A)
