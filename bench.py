@@ -197,15 +197,16 @@ def cpu_baseline_single(S, p, k_target, budget_s=20.0):
         if dt > budget_s / 3 or m >= k_target:
             break
         m *= 2
-    # Fit total(m) = c0 + c1 m + c2 m^2 on the probes (MGS cost grows linearly per iteration).
+    # Iteration j costs a + b j (preconditioner and SpMM, plus MGS against j
+    # blocks), so total(m) = c1 m + c2 m^2 with c1, c2 >= 0: non-negative
+    # least squares on the probes.  The per-iteration cost never falls, so
+    # the last probe's average cost per iteration times k_target is a floor.
+    from scipy.optimize import nnls
     ms = np.array([t[0] for t in times], dtype=float)
     ts = np.array([t[1] for t in times])
-    if len(ms) >= 3:
-        c = np.polyfit(ms, ts, 2)
-        total = float(np.polyval(c, k_target))
-    else:
-        total = float(ts[-1] / ms[-1] * k_target)
-    total = max(total, float(ts[-1]))
+    coef, _ = nnls(np.stack([ms, ms * ms], axis=1), ts)
+    total = float(coef[0] * k_target + coef[1] * k_target * k_target)
+    total = max(total, float(ts[-1] / ms[-1] * k_target))
     return {
         "value": p / (t_ilu + total),
         "unit": "RHS/s",

@@ -173,10 +173,13 @@ void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool all
   upload(rf->amap, amap, s);
   rf->fac.ensure(std::max(1, rf->nnzF) * sizeof(double));
   // Dense tail for large single systems: the last m rows (m a multiple of
-  // the dense block width, up to 32 blocks, about n/16).
+  // the dense block width, at most 12 blocks and about n/16).  On the
+  // 70,000-bus pattern 12 blocks (1,536 rows) balance the single-workgroup
+  // sparse tail against the serial diagonal-block LUs best (8, 16, 24 and 32
+  // blocks measured slower).
   if (allow_dense && n >= kDenseMinN) {
     const int nb = kDluNb;
-    rf->m = std::min(16 * nb, (n / 16) / nb * nb);
+    rf->m = std::min(12 * nb, (n / 16) / nb * nb);
   }
   if (!rf->dense()) {
     build_triop(rf->lo, n, lu.frow, lu.fcol, 0, s);
