@@ -176,11 +176,12 @@ def _oracle():
 
 
 # ----------------------------------------------------------------- c2 / c3
-def cpu_baseline_single(S, p, k_target, budget_s=20.0):
+def cpu_baseline_single(S, p, k_target, budget_s=20.0, k_known=True):
     """Reference algorithm (src/block_gmres.jl restated on CPU arrays: MGS,
     LAPACK Householder, ILU0 solves) timed on a bounded sample: the first
-    block iterations until ~budget_s, then the per-iteration cost (linear in
-    the iteration index, as MGS is) extrapolated to k_target iterations."""
+    block iterations until ~budget_s, plus one timed MGS projection and
+    update, extrapolated to k_target iterations (MGS cost grows linearly with
+    the iteration index)."""
     oracle = _oracle()
     A = S.scipy()
     t0 = time.perf_counter()
@@ -197,15 +198,23 @@ def cpu_baseline_single(S, p, k_target, budget_s=20.0):
         if dt > budget_s / 3 or m >= k_target:
             break
         m *= 2
-    # Iteration j costs a + b j (preconditioner and SpMM, plus MGS against j
-    # blocks), so total(m) = c1 m + c2 m^2 with c1, c2 >= 0: non-negative
-    # least squares on the probes.  The per-iteration cost never falls, so
-    # the last probe's average cost per iteration times k_target is a floor.
-    from scipy.optimize import nnls
+    # Iteration j (1-based) costs a + b j: the preconditioner, SpMM and
+    # Householder step (a) plus MGS against j basis blocks (b per block).
+    # b is timed directly on one n x p projection and update (the probes are
+    # too short to resolve it); a from the probes with b's share removed.
     ms = np.array([t[0] for t in times], dtype=float)
     ts = np.array([t[1] for t in times])
-    coef, _ = nnls(np.stack([ms, ms * ms], axis=1), ts)
-    total = float(coef[0] * k_target + coef[1] * k_target * k_target)
+    rng = np.random.default_rng(0)
+    Vb = rng.standard_normal((S.n, p))
+    Wb = rng.standard_normal((S.n, p))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Hb = Vb.T @ Wb
+        Wb -= Vb @ Hb
+    b_pair = (time.perf_counter() - t0) / reps
+    a_iter = max(0.0, float(np.mean((ts - b_pair * ms * (ms + 1) / 2) / ms)))
+    total = a_iter * k_target + b_pair * k_target * (k_target + 1) / 2
     total = max(total, float(ts[-1] / ms[-1] * k_target))
     return {
         "value": p / (t_ilu + total),
@@ -214,8 +223,10 @@ def cpu_baseline_single(S, p, k_target, budget_s=20.0):
         "kind": "port",
         "sample": (f"reference algorithm restated (oracle/oracle.py block_gmres: block MGS, LAPACK "
                    f"Householder, ILU0 solves) on the same {S.n}-unknown system with {p} RHS; timed "
-                   f"for {', '.join(str(int(x)) for x in ms)} block iterations ({ts.sum():.1f} s), "
-                   f"extrapolated to the {k_target} iterations the FP64 GPU run needed"),
+                   f"for {', '.join(str(int(x)) for x in ms)} block iterations ({ts.sum():.1f} s) and one "
+                   f"MGS projection + update ({b_pair:.3f} s), "
+                   + (f"extrapolated to the {k_target} iterations the FP64 GPU run needed" if k_known else
+                      f"extrapolated to {k_target} iterations (the FP64 GPU run was skipped)")),
     }
 
 
@@ -263,7 +274,7 @@ def bench_single(args, bpf, gen):
         log(f"[bench] plain fp64: {dtp:.3f}s solved {solvedp}/{p} its {stp.niter} berr {bep.max():.2e}")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_single(S, p, k_plain or 60)
+        cpu = cpu_baseline_single(S, p, k_plain or 60, k_known=k_plain is not None)
         log(f"[bench] cpu baseline: {cpu['value']:.3f} RHS/s ({cpu['sample']})")
     config = {
         "workload": (f"{args.workload}: {nbus}-bus synthetic polar NR Jacobian n={S.n} nnz={S.nnz}, "
