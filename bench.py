@@ -545,7 +545,7 @@ def main():
     ap.add_argument("--memory", type=int, default=40,
                     help="Krylov basis blocks kept (c2/c3); with --restart the restart length")
     ap.add_argument("--restart", action="store_true", help="restarted block GMRES (c2/c3)")
-    ap.add_argument("--bmgs-blocks", type=int, default=4,
+    ap.add_argument("--bmgs-blocks", type=int, default=8,
                     help="basis blocks projected together per block-MGS step of the mixed form "
                          "(c2/c3; the plain FP64 form keeps the reference's one-block loop)")
     ap.add_argument("--poly-degree", type=int, default=None,
