@@ -247,6 +247,25 @@ __global__ __launch_bounds__(kB) void proj16_kernel(ProjArgs a) {
   }
 }
 
+// Stacked groups hold only a few chunks each: one thread per output sums
+// its group's chunks in chunk order.
+__global__ __launch_bounds__(kB) void proj_reduce_stacked_kernel(const double *part, int nchunk, int nb, int ng,
+                                                                 int SS, double *out, size_t o_istride,
+                                                                 size_t o_gstride, int accumulate,
+                                                                 const GroupDesc *groups, int stacked_rows) {
+  const size_t total = (size_t)nb * ng * SS;
+  const size_t e = (size_t)blockIdx.x * kB + threadIdx.x;
+  if (e >= total) return;
+  const GroupDesc gd = groups[(e / SS) % ng];
+  const int nc = gd.active ? min(nchunk, (gd.nr + stacked_rows - 1) / stacked_rows) : 0;
+  double s = 0.0;
+  for (int c = 0; c < nc; ++c) s += part[(size_t)c * total + e];
+  const size_t ig = e / SS, el = e % SS;
+  const size_t i = ig / ng, g = ig % ng;
+  double *o = out + i * o_istride + g * o_gstride + el;
+  *o = accumulate ? *o + s : s;
+}
+
 // out[i*o_istride + g*o_gstride + e] (+)= sum_chunk part[...]: each
 // workgroup owns 64 consecutive outputs; its 4 wavefronts split the chunks
 // and combine through LDS.
@@ -1314,9 +1333,14 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
     default: BPF_THROW(BPF_E_UNSUPPORTED, "block width %d", S);
   }
   const size_t total = (size_t)a.nb * a.ng * S * S;
-  hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64 * kRedWaves), 0, s,
-                     a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
-                     accumulate ? 1 : 0, a.groups, a.rows_sum > 0 ? a.chunk_rows : 0);
+  if (a.rows_sum > 0)
+    hipLaunchKernelGGL(proj_reduce_stacked_kernel, dim3((unsigned)((total + kB - 1) / kB)), dim3(kB), 0, s,
+                       a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride, accumulate ? 1 : 0,
+                       a.groups, a.chunk_rows);
+  else
+    hipLaunchKernelGGL(proj_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64 * kRedWaves), 0, s,
+                       a.part, nchunk, a.nb, a.ng, S * S, out, o_istride, o_gstride,
+                       accumulate ? 1 : 0, a.groups, 0);
   BPF_CHECK_HIP(hipGetLastError());
   // Algorithmic traffic: each basis block and W read once; the s x s
   // products are 2 n w^2 flops per (block, group).
