@@ -2,6 +2,8 @@
 // device buffers, triangular-solve schedules and kernel launchers.
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -140,13 +142,13 @@ struct TriOp {
 
 // Process-wide switch between the sync-free single-launch triangular
 // solves / factorizations and the level-by-level launches (default).
-extern bool g_syncfree;
+extern std::atomic<bool> g_syncfree;
 // Process-wide switch: block-Hessenberg updates on a second stream, beside
 // the next iteration (default on).
-extern bool g_overlap_hess;
+extern std::atomic<bool> g_overlap_hess;
 // Process-wide switch for the SpMM launch-shape autotuner (default on), and
 // removal of a matrix's cached shapes (its row-pointer array is released).
-extern bool g_autotune;
+extern std::atomic<bool> g_autotune;
 void spmm_forget(const int *rowptr);
 // Reads the op's status word (blocking); throws BPF_E_HIP if a sync-free
 // dependency wait ran out, after switching to level-by-level launches.

@@ -19,7 +19,8 @@
 
 namespace bpf {
 
-bool g_overlap_hess = true;
+std::atomic<bool> g_overlap_hess{true};
+constexpr int kOverlapMaxIt = 1 << 16;
 
 void *BlockArena::block(int i) {
   while (i >= (int)ptrs.size()) {
@@ -299,7 +300,9 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
   cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
   double *h_rn = h_rn_;
-  const bool ovl = g_overlap_hess;
+  // (runs longer than kOverlapMaxIt iterations keep the serial order: the
+  // pointer tables are sized for the whole run up front)
+  const bool ovl = g_overlap_hess && maxit <= kOverlapMaxIt;
   if (ovl) {
     // no table growth (and device-table replacement) while the second stream runs
     coltab_.reserve(maxit + 2, st_);

@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void tri_gather_kernel(int m, const int *ga
 }  // namespace
 
 // ------------------------------------------------------------------ hosts
-bool g_autotune = true;
+std::atomic<bool> g_autotune{true};
 
 namespace {
 
@@ -662,7 +662,7 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
   BPF_CHECK_HIP(hipGetLastError());
 }
 
-bool g_syncfree = false;
+std::atomic<bool> g_syncfree{false};
 
 void triop_check(TriOp &op, hipStream_t s) {
   if (op.n <= 0) return;
