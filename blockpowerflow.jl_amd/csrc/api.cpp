@@ -18,8 +18,58 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
+void *BufPool::take(size_t nbytes, size_t *got) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = free.lower_bound(nbytes);
+    if (it != free.end() && it->first <= 2 * nbytes) {
+      void *p = it->second;
+      *got = it->first;
+      cached -= it->first;
+      free.erase(it);
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    // out of memory with blocks cached: release them and retry once
+    trim();
+    e = hipMalloc(&p, nbytes);
+  }
+  if (e != hipSuccess) BPF_THROW(BPF_E_NOMEM, "hipMalloc(%zu) failed: %s", nbytes, hipGetErrorString(e));
+  *got = nbytes;
+  return p;
+}
+
+void BufPool::give(void *p, size_t nbytes) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (cap == 0) {
+    size_t fr = 0, tot = 0;
+    cap = hipMemGetInfo(&fr, &tot) == hipSuccess && tot > 0 ? tot / 4 : ((size_t)8 << 30);
+  }
+  free.emplace(nbytes, p);
+  cached += nbytes;
+  while (cached > cap && !free.empty()) {
+    auto it = std::prev(free.end());
+    (void)hipFree(it->second);
+    cached -= it->first;
+    free.erase(it);
+  }
+}
+
+void BufPool::trim() {
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto &kv : free) (void)hipFree(kv.second);
+  free.clear();
+  cached = 0;
+}
+
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) {
+    if (pool) pool->give(ptr, bytes);
+    else (void)hipFree(ptr);
+  }
   ptr = nullptr;
   bytes = 0;
 }
@@ -28,6 +78,12 @@ void DevBuf::ensure(size_t nbytes) {
   if (nbytes <= bytes && ptr) return;
   release();
   if (nbytes == 0) nbytes = 16;
+  if (pool) {
+    size_t got = 0;
+    ptr = pool->take(nbytes, &got);
+    bytes = got;
+    return;
+  }
   hipError_t e = hipMalloc(&ptr, nbytes);
   if (e != hipSuccess) {
     ptr = nullptr;

@@ -3,6 +3,8 @@
 #pragma once
 
 #include <atomic>
+#include <mutex>
+#include <map>
 
 #include <hip/hip_runtime.h>
 
@@ -59,16 +61,42 @@ int guard(F &&f) {
 }
 
 // ---------------------------------------------------------------- buffers
-// Owning device buffer (hipMalloc); grows, never shrinks.
+// A context's cache of device allocations: blocks given back are handed out
+// again for requests of up to their size (and at least half of it), so a
+// solver that runs again with the same shapes makes no hipMalloc / hipFree
+// (which cost ~0.1-0.3 ms each and synchronize).  Blocks are only reused by
+// work of the same context, whose kernels run in stream order, so a block
+// given back while kernels still use it is safe to hand out again.
+struct BufPool {
+  std::multimap<size_t, void *> free;
+  size_t cached = 0;
+  // Cache limit: a quarter of the device's memory (set on first use); a
+  // block given back beyond it is freed, largest cached blocks first.
+  size_t cap = 0;
+  std::mutex mu;
+  void *take(size_t nbytes, size_t *got);   // cached block or hipMalloc
+  void give(void *p, size_t nbytes);
+  void trim();                              // hipFree every cached block
+  ~BufPool() { trim(); }
+};
+
+// Owning device buffer (hipMalloc, or a context's BufPool when `pool` is
+// set); grows, never shrinks.
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  BufPool *pool = nullptr;
   DevBuf() = default;
+  explicit DevBuf(BufPool *p) : pool(p) {}
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), pool(o.pool) { o.ptr = nullptr; o.bytes = 0; }
   DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      ptr = o.ptr; bytes = o.bytes; pool = o.pool;
+      o.ptr = nullptr; o.bytes = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
@@ -101,6 +129,7 @@ struct bpf_ctx_s {
   double *h_small = nullptr;  // pinned host staging for small D2H reads
   size_t h_small_cap = 0;
   void ensure_small(size_t ndouble);
+  bpf::BufPool pool;          // solver work buffers, reused across calls
 };
 
 struct bpf_csr_s {

@@ -24,7 +24,7 @@ constexpr int kOverlapMaxIt = 1 << 16;
 
 void *BlockArena::block(int i) {
   while (i >= (int)ptrs.size()) {
-    DevBuf b;
+    DevBuf b(pool);
     b.ensure(block_elems * esize * chunk);
     for (int q = 0; q < chunk; ++q) ptrs.push_back(static_cast<char *>(b.ptr) + q * block_elems * esize);
     chunks.push_back(std::move(b));
@@ -94,6 +94,11 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
     gmax_rows_ = cfg.n;
   }
   ng_ = (int)h_groups_.size();
+  // per-solve work buffers come from the context's pool
+  for (DevBuf *b : {&d_groups_, &W_, &P_, &E_, &rnorm_, &Y_, &Zb_, &Rinvb_, &Xtmp_, &Vf_, &Zf_, &tmpcol_,
+                    &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.flags, &cq_.part})
+    b->pool = &ctx->pool;
+  V_.pool = Z_.pool = &ctx->pool;
   upload(d_groups_, h_groups_, st_);
   const size_t es = cfg.f32 ? sizeof(float) : sizeof(double);
   const size_t blk = (size_t)cfg.n * cfg.p;
@@ -131,7 +136,7 @@ BlockArnoldi::~BlockArnoldi() {
 double *BlockArnoldi::col_region(int k) {
   while ((int)cols_.size() <= k) {
     const int kk = (int)cols_.size();
-    DevBuf b;
+    DevBuf b(&ctx_->pool);
     b.ensure(hcol_elems(kk, ng_, S_) * sizeof(double));
     coltab_.set(kk, b.ptr, st_);
     cols_.push_back(std::move(b));

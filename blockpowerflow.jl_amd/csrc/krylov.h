@@ -18,6 +18,7 @@ struct BlockArena {
   size_t block_elems = 0;
   size_t esize = 0;
   int chunk = 8;
+  BufPool *pool = nullptr;     // chunks come from / go back to this pool
   std::vector<DevBuf> chunks;
   std::vector<void *> ptrs;
   void init(size_t elems, size_t es, int chunk_blocks = 8) {

@@ -145,6 +145,7 @@ static std::vector<cplx> leja_order(std::vector<cplx> z) {
 
 void Poly::build(bpf_ctx ctx, bpf_csr A, bpf_prec N, int degree) {
   roots.clear();
+  for (DevBuf *b : {&V, &Wb, &hcoef, &tab, &part, &grp, &prod, &q, &t}) b->pool = &ctx->pool;
   if (degree <= 0 || !N || A->n < 2 * degree) return;
   const int n = A->n;
   const int d = degree;

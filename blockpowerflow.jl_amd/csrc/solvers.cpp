@@ -29,7 +29,8 @@ struct HistoryWriter {
 // Row-major FP64 copies of the caller's column-major blocks plus scratch.
 struct Blocks {
   DevBuf B, X, R, Xt, scale, stat;
-  void init(int n, int p) {
+  void init(int n, int p, BufPool *pool) {
+    for (DevBuf *b : {&B, &X, &R, &Xt, &scale, &stat}) b->pool = pool;
     const size_t m = (size_t)n * p * sizeof(double);
     B.ensure(m); X.ensure(m); R.ensure(m); Xt.ensure(m);
     scale.ensure(std::max(1, p) * sizeof(double));
@@ -94,7 +95,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   const int n = A->n;
   hipStream_t s = ctx->stream;
   Blocks blk;
-  blk.init(n, p);
+  blk.init(n, p, &ctx->pool);
   launch_cm_to_rm(s, n, p, dB, ldb, blk.B.ptr, p, false);
   if (dX0) {
     launch_cm_to_rm(s, n, p, dX0, ldx0, blk.X.ptr, p, false);
@@ -222,7 +223,7 @@ void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   const int n = A->n;
   hipStream_t s = ctx->stream;
   Blocks blk;
-  blk.init(n, p);
+  blk.init(n, p, &ctx->pool);
   launch_cm_to_rm(s, n, p, dB, ldb, blk.B.ptr, p, false);
   if (dX0) launch_cm_to_rm(s, n, p, dX0, ldx0, blk.X.ptr, p, false);
   else fill_double(s, blk.X.as<double>(), 0.0, (size_t)n * p);
@@ -362,7 +363,7 @@ int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx
     BPF_REQUIRE(ldx >= A->n && ldb >= A->n, "leading dimension < n");
     const int n = A->n;
     Blocks blk;
-    blk.init(n, p);
+    blk.init(n, p, &ctx->pool);
     launch_cm_to_rm(ctx->stream, n, p, d_B, ldb, blk.B.ptr, p, false);
     launch_cm_to_rm(ctx->stream, n, p, d_X, ldx, blk.X.ptr, p, false);
     residual_rm(ctx->stream, A, p, blk.B.as<double>(), blk.X.as<double>(), blk.R.as<double>());
