@@ -111,6 +111,18 @@ void bpf_ctx_s::ensure_small(size_t ndouble) {
   h_small_cap = ndouble;
 }
 
+void bpf_ctx_s::ensure_stream2(size_t ndouble) {
+  if (!stream2) BPF_CHECK_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  if (!ev_a) BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming));
+  if (!ev_b) BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_b, hipEventDisableTiming));
+  if (ndouble > h_rn_cap) {
+    if (h_rn) (void)hipHostFree(h_rn);
+    h_rn = nullptr;
+    BPF_CHECK_HIP(hipHostMalloc((void **)&h_rn, ndouble * sizeof(double), hipHostMallocDefault));
+    h_rn_cap = ndouble;
+  }
+}
+
 void bpf_csr_s::refresh_derived() {
   ctx->scratch.ensure(64);
   csr_norm_and_cast(ctx->stream, n, rowptr.as<int>(), val.as<double>(), val32.as<float>(), nnz,
@@ -182,7 +194,12 @@ int bpf_ctx_destroy(bpf_ctx ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+    if (ctx->h_rn) (void)hipHostFree(ctx->h_rn);
+    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     (void)hipStreamDestroy(ctx->stream);

@@ -130,6 +130,13 @@ struct bpf_ctx_s {
   size_t h_small_cap = 0;
   void ensure_small(size_t ndouble);
   bpf::BufPool pool;          // solver work buffers, reused across calls
+  // Second stream, its two events and pinned host buffer for the block
+  // GMRES Hessenberg updates (created on first use, kept for the context).
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double *h_rn = nullptr;
+  size_t h_rn_cap = 0;
+  void ensure_stream2(size_t ndouble);
 };
 
 struct bpf_csr_s {

@@ -112,10 +112,11 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   rnorm_.ensure(std::max(1, nslots_) * sizeof(double));
   flags_.ensure(16);
   ctx->ensure_small(std::max(nslots_, 16));
-  BPF_CHECK_HIP(hipStreamCreateWithFlags(&hs_, hipStreamNonBlocking));
-  BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
-  BPF_CHECK_HIP(hipEventCreateWithFlags(&ev_hess_, hipEventDisableTiming));
-  BPF_CHECK_HIP(hipHostMalloc((void **)&h_rn_, std::max(1, nslots_) * sizeof(double), hipHostMallocDefault));
+  ctx->ensure_stream2(std::max(1, nslots_));
+  hs_ = ctx->stream2;
+  ev_main_ = ctx->ev_a;
+  ev_hess_ = ctx->ev_b;
+  h_rn_ = ctx->h_rn;
 }
 
 void BlockArnoldi::set_active(int g, bool active, int K) {
@@ -126,11 +127,8 @@ void BlockArnoldi::set_active(int g, bool active, int K) {
 void BlockArnoldi::upload_groups() { upload(d_groups_, h_groups_, st_); }
 
 BlockArnoldi::~BlockArnoldi() {
+  // the second stream, its events and h_rn_ belong to the context
   if (hs_) (void)hipStreamSynchronize(hs_);
-  if (h_rn_) (void)hipHostFree(h_rn_);
-  if (ev_main_) (void)hipEventDestroy(ev_main_);
-  if (ev_hess_) (void)hipEventDestroy(ev_hess_);
-  if (hs_) (void)hipStreamDestroy(hs_);
 }
 
 double *BlockArnoldi::col_region(int k) {

@@ -128,8 +128,9 @@ class BlockArnoldi {
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
   hipStream_t st_;
-  // Second stream: the Hessenberg update of iteration k runs there while
-  // iteration k+1's preconditioner, SpMM and Gram-Schmidt are queued on st_.
+  // Second stream (the context's): the Hessenberg update of iteration k runs
+  // there while iteration k+1's preconditioner, SpMM and Gram-Schmidt are
+  // queued on st_.
   hipStream_t hs_ = nullptr;
   hipEvent_t ev_main_ = nullptr, ev_hess_ = nullptr;
   double *h_rn_ = nullptr;     // pinned: residual estimates of the last update
