@@ -116,7 +116,6 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   hs_ = ctx->stream2;
   ev_main_ = ctx->ev_a;
   ev_hess_ = ctx->ev_b;
-  h_rn_ = ctx->h_rn;
 }
 
 void BlockArnoldi::set_active(int g, bool active, int K) {
@@ -302,7 +301,10 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   vtab_.set(0, V_.block(0), st_);
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
   cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
-  double *h_rn = h_rn_;
+  // (the context's pinned buffer may have been regrown for another solver
+  // since this one was built)
+  ctx_->ensure_stream2(std::max(1, nslots_));
+  double *h_rn = ctx_->h_rn;
   // (runs longer than kOverlapMaxIt iterations keep the serial order: the
   // pointer tables are sized for the whole run up front)
   const bool ovl = g_overlap_hess && maxit <= kOverlapMaxIt;

@@ -133,7 +133,6 @@ class BlockArnoldi {
   // queued on st_.
   hipStream_t hs_ = nullptr;
   hipEvent_t ev_main_ = nullptr, ev_hess_ = nullptr;
-  double *h_rn_ = nullptr;     // pinned: residual estimates of the last update
   std::vector<GroupDesc> h_groups_;
   DevBuf d_groups_;
   BlockArena V_, Z_;           // basis and (flexible) preconditioned basis

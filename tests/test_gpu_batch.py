@@ -97,6 +97,34 @@ def test_batch_overlapped_hessenberg_update_is_exact(bpf, gen):
         np.testing.assert_array_equal(b, a)
 
 
+def test_batch_handle_solves_again_identically(bpf, gen, oracle):
+    # A BatchedSystems handle keeps one Krylov engine per RHS-count bin across
+    # solves; bins of different widths share the context's buffers.  Solving
+    # twice with the same handle gives bitwise the same answer, and a third
+    # solve with other right-hand sides is still correct.
+    systems = _systems(gen, 10, 91, ps=[4, 8, 16, 32, 64])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    batch = bpf.BatchedSystems(As, [S.B.shape[1] for S in systems])
+    try:
+        runs = []
+        for _ in range(2):
+            Xs, st, solved = batch.solve(Bs, precision="mixed", berr_tol=1e-10, memory=40, itmax=600)
+            assert solved.all()
+            runs.append([X.to_numpy() for X in Xs])
+        for a, b in zip(runs[0], runs[1]):
+            np.testing.assert_array_equal(b, a)
+        rng = np.random.default_rng(5)
+        B2 = [rng.standard_normal(S.B.shape) for S in systems]
+        Xs, st, solved = batch.solve([bpf.HipMatrix.from_numpy(b) for b in B2], precision="mixed",
+                                     berr_tol=1e-10, memory=40, itmax=600)
+        assert solved.all()
+        for S, X, b in zip(systems, Xs, B2):
+            assert oracle.backward_error(S.indptr, S.indices, S.data, X.to_numpy(), b).max() <= 1e-10
+    finally:
+        batch.destroy()
+
+
 def test_batch_errors(bpf, gen):
     _, S = gen.make_system(30, 4, seed=74)
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
