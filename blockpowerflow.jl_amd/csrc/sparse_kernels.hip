@@ -124,6 +124,53 @@ __device__ __forceinline__ void tri_row(const TriDev &a, int t, int lane, int G,
       acc[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
     }
     int k = k0;
+    for (; k + 7 < k1; k += 8) {        // eight non-zeros per step: 8 * CPL loads in flight
+      int j[8];
+      TV v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        j[u] = a.col[k + u];
+        v[u] = static_cast<TV>(val[k + u]);
+      }
+      TV av[8][CPL];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const TV *yu = y + (size_t)j[u] * ldy;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+          const int c = cbase + lane + m * G;
+          av[u][m] = c < p ? yu[c] : TV(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc[m] = fma(-v[u], av[u][m], acc[m]);
+    }
+    for (; k + 3 < k1; k += 4) {        // four non-zeros per step: 4 * CPL loads in flight
+      const int j0 = a.col[k], j1 = a.col[k + 1], j2 = a.col[k + 2], j3 = a.col[k + 3];
+      const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + 1]);
+      const TV v2 = static_cast<TV>(val[k + 2]), v3 = static_cast<TV>(val[k + 3]);
+      const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
+      const TV *y2 = y + (size_t)j2 * ldy, *y3 = y + (size_t)j3 * ldy;
+      TV a0[CPL], a1[CPL], a2[CPL], a3[CPL];
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = cbase + lane + m * G;
+        const bool ok = c < p;
+        a0[m] = ok ? y0[c] : TV(0);
+        a1[m] = ok ? y1[c] : TV(0);
+        a2[m] = ok ? y2[c] : TV(0);
+        a3[m] = ok ? y3[c] : TV(0);
+      }
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        acc[m] = fma(-v0, a0[m], acc[m]);
+        acc[m] = fma(-v1, a1[m], acc[m]);
+        acc[m] = fma(-v2, a2[m], acc[m]);
+        acc[m] = fma(-v3, a3[m], acc[m]);
+      }
+    }
     for (; k + 1 < k1; k += 2) {        // two non-zeros per step (loads overlap)
       const int j0 = a.col[k], j1 = a.col[k + 1];
       const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + 1]);
