@@ -64,7 +64,34 @@ __global__ __launch_bounds__(kBlock) void spmm_rm_kernel(
     TV acc[CPL];
 #pragma unroll
     for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
-    for (int k = k0; k < k1; ++k) {
+    int k = k0;
+    // Eight non-zeros per step: their column indices and values first, then
+    // all 8 * CPL gathers in flight; products are added in k order, so the
+    // result does not depend on the unrolling or the launch shape.
+    for (; k + 7 < k1; k += 8) {
+      int j[8];
+      TV a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        j[u] = colidx[k + u];
+        a[u] = static_cast<TV>(val[k + u]);
+      }
+      TV xv[8][CPL];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const TV *xr = X + (size_t)j[u] * ldx;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+          const int c = cbase + lane + m * G;
+          xv[u][m] = c < p ? xr[c] : TV(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc[m] = fma(a[u], xv[u][m], acc[m]);
+    }
+    for (; k < k1; ++k) {
       const int j = colidx[k];
       const TV a = static_cast<TV>(val[k]);
       const TV *xr = X + (size_t)j * ldx;
