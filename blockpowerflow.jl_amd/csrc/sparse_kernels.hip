@@ -423,16 +423,42 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
     s_val[q] = f.fac[base + q];
   }
   wave_sync();
+  // Pivot rows are final, so pivot kk+1's diagonal, extent and first 64
+  // entries are loaded while pivot kk is applied (the loads of one step no
+  // longer wait on each other); the arithmetic and its order are unchanged.
+  struct Piv {
+    int k, j0, j1, col;
+    double ukk, val;
+  };
+  auto fetch = [&](int kk) {
+    Piv q;
+    q.k = s_col[kk];
+    const int d = f.diagpos[q.k];
+    q.ukk = f.fac[d];
+    q.j0 = d + 1;
+    q.j1 = f.rowptr[q.k + 1];
+    const int jj = q.j0 + lane;
+    q.col = jj < q.j1 ? f.colidx[jj] : -1;
+    q.val = jj < q.j1 ? f.fac[jj] : 0.0;
+    return q;
+  };
+  Piv cur = dloc > 0 ? fetch(0) : Piv{0, 0, 0, -1, 1.0, 0.0};
   for (int kk = 0; kk < dloc; ++kk) {
-    const int k = s_col[kk];
+    const int k = cur.k;
     if (k >= kmax) break;
-    const double ukk = f.fac[f.diagpos[k]];
+    const Piv nxt = kk + 1 < dloc ? fetch(kk + 1) : cur;
+    const double ukk = cur.ukk;
     const double l = s_val[kk] / ukk;
-    const int j0 = f.diagpos[k] + 1, j1 = f.rowptr[k + 1];
-    for (int jj = j0 + lane; jj < j1; jj += 64) {
+    const int j0 = cur.j0, j1 = cur.j1;
+    if (cur.col >= 0) {
+      const int pos = lds_find(s_col, kk + 1, len, cur.col);
+      if (pos >= 0) s_val[pos] -= l * cur.val;
+    }
+    for (int jj = j0 + 64 + lane; jj < j1; jj += 64) {
       const int pos = lds_find(s_col, kk + 1, len, f.colidx[jj]);
       if (pos >= 0) s_val[pos] -= l * f.fac[jj];
     }
+    cur = nxt;
     wave_sync();
     if (lane == 0) {
       s_val[kk] = l;
