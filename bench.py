@@ -359,10 +359,13 @@ def bench_batch(args, bpf, gen):
     config = {
         "workload": (f"c4: {nsys} independent synthetic NR Jacobians, n log-uniform in [500, 20000] "
                      f"(sum n={ntot}, sum nnz={nnz}), RHS counts in {{8,16,32,64}} (sum {nrhs}); batched "
-                     f"flexible block GMRES, FP64 basis ({args.orth}), FP32 ILU0 solves"
+                     f"flexible block GMRES, FP64 basis ({args.orth}"
+                     + (f" over {args.bmgs_blocks}-block steps" if args.orth == "bmgs" else "")
+                     + "), FP32 ILU0 solves"
                      f"{f' in a degree-{args.poly_degree} GMRES polynomial' if args.poly_degree else ''}, "
                      f"per-system stopping, to per-RHS backward error 1e-10"),
         "nsys": nsys, "sum_n": ntot, "sum_nnz": nnz, "nrhs": nrhs, "orth": args.orth,
+        "bmgs_blocks": args.bmgs_blocks,
         "poly_degree": args.poly_degree,
         "block_iterations": st_last.niter, "max_backward_error": be_max,
     }
@@ -545,14 +548,15 @@ def main():
     ap.add_argument("--memory", type=int, default=40,
                     help="Krylov basis blocks kept (c2/c3); with --restart the restart length")
     ap.add_argument("--restart", action="store_true", help="restarted block GMRES (c2/c3)")
-    ap.add_argument("--bmgs-blocks", type=int, default=8,
-                    help="basis blocks projected together per block-MGS step of the mixed form "
-                         "(c2/c3; the plain FP64 form keeps the reference's one-block loop)")
+    ap.add_argument("--bmgs-blocks", type=int, default=None,
+                    help="basis blocks projected together per block-MGS step of the chosen form "
+                         "(default 8 for c2/c3, 16 for c4; the plain FP64 form keeps the reference's "
+                         "one-block loop)")
     ap.add_argument("--poly-degree", type=int, default=None,
                     help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
-    ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default=None,
-                    help="block Gram-Schmidt: bmgs (the reference's loop) or bcgs2 (two classical "
-                         "passes); default bmgs for c2/c3, bcgs2 for c4")
+    ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
+                    help="block Gram-Schmidt: bmgs (the reference's loop, over --bmgs-blocks steps) or "
+                         "bcgs2 (two classical passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three device-to-device copies of 1 GiB (the known-size kernel that "
                          "scales the HBM byte counters) and exit")
@@ -563,8 +567,8 @@ def main():
     batched = args.workload == "c4"
     if args.poly_degree is None:
         args.poly_degree = 0 if batched else 4
-    if args.orth is None:
-        args.orth = "bcgs2" if batched else "bmgs"
+    if args.bmgs_blocks is None:
+        args.bmgs_blocks = 16 if batched else 8
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
