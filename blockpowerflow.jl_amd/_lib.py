@@ -101,7 +101,7 @@ SIGNATURES = {
     "bpf_rf_create": [vp, vp, i, d, C.POINTER(vp)],
     "bpf_rf_refactor": [vp, vp],
     "bpf_rf_solve": [vp, i, i, vp, i, vp, i],
-    "bpf_rf_solve_refined": [vp, vp, i, vp, i, vp, i, i, dp],
+    "bpf_rf_solve_refined": [vp, vp, i, vp, i, vp, i, i, d, dp, C.POINTER(C.c_int)],
     "bpf_rf_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bpf_rf_get_factor": [vp, ip, ip, ip, ip, dp],
     "bpf_rf_destroy": [vp],

@@ -306,11 +306,13 @@ int bpf_rf_solve(bpf_rf rf, int dtype, int nrhs, const double *d_B, int ldb, dou
 }
 
 int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int ldb, double *d_X,
-                         int ldx, int refine, double *h_berr) {
+                         int ldx, int refine, double berr_tol, double *h_berr, int *h_steps) {
   return guard([&] {
     BPF_REQUIRE(rf && A && d_B && d_X, "null argument");
     check_pattern(rf, A);
     BPF_REQUIRE(nrhs >= 0 && ldb >= rf->n && ldx >= rf->n && refine >= 0, "bad sizes");
+    BPF_REQUIRE(berr_tol >= 0.0, "berr_tol must be >= 0");
+    if (h_steps) *h_steps = 0;
     if (nrhs == 0 || rf->n == 0) return;
     hipStream_t s = rf->ctx->stream;
     const int n = rf->n, p = nrhs;
@@ -321,22 +323,34 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
     double *B = rf->w1.as<double>(), *X = rf->w2.as<double>(), *R = rf->w3.as<double>();
     float *z = reinterpret_cast<float *>(R + m);
     launch_cm_to_rm(s, n, p, d_B, ldb, B, p, false);
-    // X0 = FP32 solve of B; then `refine` steps X += solve32(B - A X).
+    // X0 = FP32 solve of B; then up to `refine` steps X += solve32(B - A X).
+    // With berr_tol > 0 the backward error of every column is checked on each
+    // residual, and the loop stops once all of them are <= berr_tol.
     launch_perm_rows_rm<double, float>(s, n, p, B, rf->dP.as<int>(), z, false);
     tri_solve<float>(rf, p, z);
     launch_perm_rows_rm<float, double>(s, n, p, z, rf->dQ.as<int>(), X, true);
-    for (int it = 0; it < refine; ++it) {
+    const bool adaptive = berr_tol > 0.0;
+    std::vector<double> be(p, 0.0);
+    int steps = 0;
+    for (int it = 0; it <= refine; ++it) {
+      const bool last = it == refine;
+      const bool want_berr = adaptive || (last && h_berr);
+      if (last && !want_berr) break;
       residual_rm(s, A, p, B, X, R);
+      if (want_berr) {
+        backward_error_rm(rf->ctx, A, p, R, X, B, be.data(), rf->stat);
+        if (adaptive && std::all_of(be.begin(), be.end(), [&](double e) { return e <= berr_tol; })) break;
+      }
+      if (last) break;
       launch_perm_rows_rm<double, float>(s, n, p, R, rf->dP.as<int>(), z, false);
       tri_solve<float>(rf, p, z);
       // X[Q[k]] += z[k]: scatter into R (free now), then add.
       launch_perm_rows_rm<float, double>(s, n, p, z, rf->dQ.as<int>(), R, true);
       axpby(s, m, 1.0, R, 1.0, X);
+      ++steps;
     }
-    if (h_berr) {
-      residual_rm(s, A, p, B, X, R);
-      backward_error_rm(rf->ctx, A, p, R, X, B, h_berr, rf->stat);
-    }
+    if (h_berr) std::copy(be.begin(), be.end(), h_berr);
+    if (h_steps) *h_steps = steps;
     launch_rm_to_cm(s, n, p, X, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));
     check_ops(rf);

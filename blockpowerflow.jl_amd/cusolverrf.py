@@ -91,18 +91,24 @@ class CusolverRfLU(_RfBase):
         h = C.c_void_p()
         check(L.bpf_rf_create(self.ctx.handle, A.handle, _ordering(ordering), float(tol), C.byref(h)))
         self.handle = h
+        self.refine_steps = 0
 
-    def solve_refined(self, A, B, refine=2, X=None):
-        """FP32 triangular solves + `refine` FP64 refinement steps; returns
-        (X, per-RHS backward error).  X (optional) receives the solution."""
+    def solve_refined(self, A, B, refine=2, X=None, berr_tol=0.0):
+        """FP32 triangular solves + at most `refine` FP64 refinement steps;
+        with berr_tol > 0 refinement stops once every RHS's backward error is
+        <= berr_tol.  Returns (X, per-RHS backward error); X (optional)
+        receives the solution and `self.refine_steps` the steps taken."""
         from . import HipMatrix
         if X is None:
             X = HipMatrix(B.shape, np.float64, B.ctx)
         elif X.shape != B.shape or X.dtype != np.float64:
             raise ValueError("X must be a float64 matrix of B's shape")
         berr = np.zeros(B.ncols)
+        steps = C.c_int(0)
         check(L.bpf_rf_solve_refined(self.handle, A.handle, B.ncols, B.ptr, B.nrows, X.ptr, X.nrows,
-                                     int(refine), berr.ctypes.data_as(_lib.dp)))
+                                     int(refine), float(berr_tol), berr.ctypes.data_as(_lib.dp),
+                                     C.byref(steps)))
+        self.refine_steps = steps.value
         return X, berr
 
 

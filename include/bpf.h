@@ -269,10 +269,15 @@ int bpf_rf_refactor(bpf_rf rf, bpf_csr A);
  * FP32 triangular solves (BPF_F32, followed by nothing else). d_X may alias d_B. */
 int bpf_rf_solve(bpf_rf rf, int dtype, int nrhs, const double *d_B, int ldb,
                  double *d_X, int ldx);
-/* FP32 triangular solves plus `refine` FP64 iterative-refinement steps with
- * the true residual B - A X (no reference counterpart). */
+/* FP32 triangular solves plus at most `refine` FP64 iterative-refinement
+ * steps with the true residual B - A X (no reference counterpart).  With
+ * berr_tol > 0 the per-RHS backward error is checked on every residual and
+ * refinement stops once all nrhs values are <= berr_tol; berr_tol = 0 runs
+ * all `refine` steps.  h_berr (nrhs, optional) receives the final backward
+ * errors, h_steps (optional) the refinement steps taken. */
 int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int ldb,
-                         double *d_X, int ldx, int refine, double *h_berr);
+                         double *d_X, int ldx, int refine, double berr_tol, double *h_berr,
+                         int *h_steps);
 /* Host copies of the analysis: permutations P, Q (n each, B = A[P,Q]) and the
  * factor pattern / values (combined L\U CSR of B). */
 int bpf_rf_info(bpf_rf rf, int *n, int *nnz_factor, int *nlevels_lower, int *nlevels_upper);

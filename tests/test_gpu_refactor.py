@@ -81,6 +81,7 @@ def test_rf_refined_newton_sequence(bpf, gen, oracle):
         bpf.CUSOLVERRF.update_(rf, A)
         X, berr = rf.solve_refined(A, B, refine=2)
         assert berr.max() <= 1e-10, berr.max()
+        assert rf.refine_steps == 2
         ref = oracle.backward_error(S.indptr, S.indices, vals, X.to_numpy(), S.B)
         # both at rounding level: compare against the target, not each other
         assert ref.max() <= 1e-10
@@ -162,3 +163,30 @@ def test_rf_dense_tail_matches_oracle(bpf, gen, oracle):
     assert berr.max() <= 1e-10
     ref_x = oracle.direct_solve(S.indptr, S.indices, vals, S.B)
     np.testing.assert_allclose(Xr.to_numpy(), ref_x, rtol=1e-7, atol=1e-9 * np.abs(ref_x).max())
+
+
+def test_rf_refined_stops_at_tolerance(bpf, gen, oracle):
+    # refinement with a backward-error target: stops at the first step whose
+    # residual meets it, and X is bitwise the fixed-step result of that many steps
+    grid, S = gen.make_system(2000, 24, seed=48)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    for vals in gen.newton_value_sets(grid, 3, seed=49):
+        A.set_values(vals)
+        bpf.CUSOLVERRF.update_(rf, A)
+        X, berr = rf.solve_refined(A, B, refine=4, berr_tol=1e-10)
+        k = rf.refine_steps
+        assert 1 <= k <= 4 and berr.max() <= 1e-10, (k, berr.max())
+        ref = oracle.backward_error(S.indptr, S.indices, vals, X.to_numpy(), S.B)
+        np.testing.assert_allclose(berr, ref, rtol=0, atol=1e-14)
+        Xk, berr_k = rf.solve_refined(A, B, refine=k)
+        assert rf.refine_steps == k
+        np.testing.assert_array_equal(X.to_numpy(), Xk.to_numpy())
+        np.testing.assert_array_equal(berr, berr_k)
+        if k > 1:
+            _, berr_less = rf.solve_refined(A, B, refine=k - 1)
+            assert berr_less.max() > 1e-10
+        # a tolerance no column meets runs every step
+        _, berr_all = rf.solve_refined(A, B, refine=3, berr_tol=1e-300)
+        assert rf.refine_steps == 3
