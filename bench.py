@@ -46,15 +46,18 @@ WORKLOADS = {
 BERR = 1e-10
 PEAK_HBM = 8.0e12
 METRIC = "solved RHS/sec at 1e-10 backward error: mixed+batched vs plain FP64, 1 GPU"
-FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "small"]
+FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "dense_lu"]
 PEAK_FP64_MFMA = 78.6e12
-# family -> (kernel, bound); "mfma" families are priced in FP64 flops.
+# family -> (kernels its timers cover, bound, stem of its counter summary
+# profiles/<workload>_<stem>_traffic.md); "mfma" families are priced in FP64
+# flops, "hbm" families in compulsory bytes.
 KERNELS = {
-    "spmm": ("spmm_rm_kernel", "hbm"),
-    "trsv": ("tri_level_kernel", "hbm"),
-    "factor": ("fac_level_kernel", "hbm"),
-    "proj": ("proj_partial_kernel", "mfma"),
-    "gemm": ("block_gemm_kernel", "mfma"),
+    "spmm": ("spmm_rm_kernel", "hbm", "spmm_rm_kernel"),
+    "trsv": ("tri_level_kernel, tri_tail_kernel", "hbm", "tri_level_kernel"),
+    "factor": ("fac_level_kernel, fac_tail_kernel, fac_partial_kernel", "hbm", "fac_level_kernel"),
+    "dense_lu": ("dlu_diag_kernel, dgemm_kernel, copy_block_kernel", "mfma", "dense_lu"),
+    "proj": ("proj_partial_kernel or proj16_kernel, then proj_reduce_kernel", "mfma", "projection"),
+    "gemm": ("block_gemm_kernel", "mfma", "block_gemm_kernel"),
 }
 
 
@@ -107,7 +110,7 @@ def roofline_from(fam, dt_prof, workload):
     return {
         "bound": bound, "kernel": KERNELS[dom][0],
         "achieved": round(achieved, 2), "peak": peak, "unit": unit,
-        "frac": round(achieved / peak, 4), "traffic": measured_traffic(workload, KERNELS[dom][0]),
+        "frac": round(achieved / peak, 4), "traffic": measured_traffic(workload, KERNELS[dom][2]),
         "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(1, d["launches"]), 2),
         "families": others,
     }
@@ -525,11 +528,11 @@ def copy_probe(nbytes=1 << 30, copies=3):
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": copies}))
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    summary (profiles/<workload>_<kernel>_traffic.md, line 'bytes per launch:
-    N'), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_{kernel}_traffic.md")
+def measured_traffic(workload, stem):
+    """HBM bytes per timed launch of the dominant family from the committed
+    counter summary (profiles/<workload>_<stem>_traffic.md, line 'bytes per
+    launch: N'), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{stem}_traffic.md")
     try:
         with open(path) as f:
             for line in f:

@@ -415,12 +415,16 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A) {
     int *d_status = P->ctx->scratch.as<int>();
     const int init = P->n;
     BPF_CHECK_HIP(hipMemcpyAsync(d_status, &init, sizeof(int), hipMemcpyHostToDevice, s));
-    if (P->kind == BPF_PREC_ILU0)
+    if (P->kind == BPF_PREC_ILU0) {
       launch_ilu_factor(s, P->lo, P->n, P->rowptr.as<int>(), P->colidx.as<int>(), P->diagpos.as<int>(),
-                        P->fac.as<double>(), P->max_row_nnz, d_status);
-    else
+                        P->fac.as<double>(), P->max_row_nnz, d_status, factor_bytes(P->n, P->nnz));
+    } else {
+      // IC(0) eliminates the strictly lower part and the diagonal of each row
+      double entries = P->n;
+      for (int i = 0; i < P->n; ++i) entries += P->h_diagpos[i] - P->h_rowptr[i];
       launch_ic_factor(s, P->lo, P->n, P->rowptr.as<int>(), P->colidx.as<int>(), P->diagpos.as<int>(),
-                       P->fac.as<double>(), P->max_row_nnz, d_status);
+                       P->fac.as<double>(), P->max_row_nnz, d_status, factor_bytes(P->n, entries));
+    }
     int status = 0;
     BPF_CHECK_HIP(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, s));
     BPF_CHECK_HIP(hipStreamSynchronize(s));

@@ -248,16 +248,24 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
 // Numeric IKJ factorization on a combined pattern (ILU0 when the pattern is
 // A's, exact LU when it is the full fill pattern).  The values must already
 // hold the (permuted) matrix entries.  Uses the lower TriOp's levels.
+// `bytes` is the launch's compulsory HBM traffic for the kernel timers: each
+// eliminated row's values read and written once, its column indices, and its
+// row pointer / diagonal position / order entries (factor_bytes()).
 // Rows [r0, r1), independent of each other, eliminated against the (final)
 // pivot rows k < kmax only.
 void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
-                           const int *diagpos, double *fac, int max_row_nnz, int *d_status);
+                           const int *diagpos, double *fac, int max_row_nnz, int *d_status,
+                           double bytes);
 void launch_ilu_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                        const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                       int *d_status);
+                       int *d_status, double bytes);
 void launch_ic_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                      int *d_status);
+                      int *d_status, double bytes);
+// Compulsory bytes of eliminating `rows` rows holding `entries` stored values.
+inline double factor_bytes(double rows, double entries) {
+  return entries * (2 * sizeof(double) + sizeof(int)) + rows * 3 * sizeof(int);
+}
 
 // Layout conversions (column-major <-> row-major) with type conversion.
 void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst,

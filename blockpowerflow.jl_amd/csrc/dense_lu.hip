@@ -17,6 +17,7 @@
 //   solves (m x p right-hand sides, FP64): Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b,
 //   then Z_b = U_bb^{-1} Y_b, Y_{<b} -= U_{<b,b} Z_b.
 #include "dense_lu.h"
+#include "profile.h"
 
 #include <algorithm>
 
@@ -375,9 +376,13 @@ void dense_tail_move(hipStream_t s, int cnt, const int *pos, const int *didx, do
 void dense_lu_factor(hipStream_t s, int m, double *S, double *Linv, double *Uinv, double *P, double *Q,
                      int *status, int base) {
   const int nb = kDluNb;
+  const int tok = prof::begin(s);
+  double flops = 0.0;
   for (int j0 = 0, b = 0; j0 < m; j0 += nb, ++b) {
     const int w = std::min(nb, m - j0);
     const int rest = m - j0 - w;
+    // diagonal-block LU (2/3 w^3) and the inverses of its two triangles (2/3 w^3)
+    flops += 4.0 / 3.0 * w * w * w + 4.0 * rest * w * w + 2.0 * rest * rest * w;
     double *S11 = S + (size_t)j0 * m + j0;
     double *Li = Linv + (size_t)b * nb * nb, *Ui = Uinv + (size_t)b * nb * nb;
     hipLaunchKernelGGL(dlu_diag_kernel, dim3(1), dim3(1024), 0, s, S11, m, w, Li, Ui, status, base + j0);
@@ -391,6 +396,8 @@ void dense_lu_factor(hipStream_t s, int m, double *S, double *Linv, double *Uinv
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
+  // compulsory traffic: S read and written once
+  prof::end(tok, s, prof::DENSE_LU, 2.0 * m * (double)m * sizeof(double), flops);
 }
 
 void dense_lu_solve(hipStream_t s, int m, int p, const double *S, const double *Linv, const double *Uinv,
@@ -398,6 +405,7 @@ void dense_lu_solve(hipStream_t s, int m, int p, const double *S, const double *
   // Forward: Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b.
   const int nb = kDluNb;
   const int nblk = (m + nb - 1) / nb;
+  const int tok = prof::begin(s);
   for (int b = 0; b < nblk; ++b) {
     const int j0 = b * nb, w = std::min(nb, m - j0), rest = m - j0 - w;
     gemm(s, false, w, p, w, 1.0, Linv + (size_t)b * nb * nb, nb, X + (size_t)j0 * p, p, Y + (size_t)j0 * p, p);
@@ -413,6 +421,11 @@ void dense_lu_solve(hipStream_t s, int m, int p, const double *S, const double *
       gemm(s, true, j0, p, w, -1.0, S + j0, m, X + (size_t)j0 * p, p, Y, p);
   }
   BPF_CHECK_HIP(hipGetLastError());
+  // two sweeps of m x m triangles plus the full diagonal-block inverses:
+  // 2 m^2 p + 2 nb m p flops; compulsory traffic S once, X read and written
+  const double mp = (double)m * p;
+  prof::end(tok, s, prof::DENSE_LU, (double)m * m * sizeof(double) + 2.0 * mp * sizeof(double),
+            2.0 * m * mp + 2.0 * nb * mp);
 }
 
 }  // namespace bpf

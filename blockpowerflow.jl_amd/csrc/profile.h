@@ -7,7 +7,7 @@
 namespace bpf {
 namespace prof {
 
-enum Family { SPMM = 0, TRSV = 1, PROJ = 2, GEMM = 3, FACTOR = 4, SMALL = 5, NFAM = 6 };
+enum Family { SPMM = 0, TRSV = 1, PROJ = 2, GEMM = 3, FACTOR = 4, DENSE_LU = 5, NFAM = 6 };
 
 bool enabled();
 // Records a start event on s when profiling is on; returns a token for end().

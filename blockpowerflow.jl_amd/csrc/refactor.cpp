@@ -76,11 +76,14 @@ void refactor_into(bpf_rf rf, const double *d_aval, double *fac) {
   launch_scatter_values(s, rf->nnzA, rf->amap.as<int>(), d_aval, fac);
   rf->status.ensure(16);
   fill_int(s, rf->status.as<int>(), rf->n, 1);
+  const int head = rf->dense() ? rf->t : rf->n;
+  const std::vector<int> &frow = rf->lu.frow;
   launch_ilu_factor(s, rf->lop(), rf->n, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(), fac,
-                    rf->max_row, rf->status.as<int>());
+                    rf->max_row, rf->status.as<int>(), factor_bytes(head, frow[head] - frow[0]));
   if (rf->dense()) {
     launch_partial_factor(s, rf->t, rf->n, rf->t, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(),
-                          fac, rf->max_row, rf->status.as<int>());
+                          fac, rf->max_row, rf->status.as<int>(),
+                          factor_bytes(rf->n - rf->t, frow[rf->n] - frow[rf->t]));
     double *S = rf->S.as<double>();
     fill_double(s, S, 0.0, (size_t)rf->m * rf->m);
     dense_tail_move(s, rf->ntail, rf->tpos.as<int>(), rf->tdidx.as<int>(), fac, S, true);

@@ -853,7 +853,7 @@ template void trisolve<float>(hipStream_t, TriOp &, int, const float *, int, flo
 
 static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowptr,
                           const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                          int *d_status) {
+                          int *d_status, double bytes) {
   const int cap = std::max(64, max_row_nnz);
   const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
   if (per_wave > 64 * 1024)
@@ -874,7 +874,7 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
                          lower.ptr.as<int>(), lower.col.as<int>(), lower.flags.as<int>(), epoch,
                          lower.status.as<int>());
     BPF_CHECK_HIP(hipGetLastError());
-    prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
+    prof::end(tok, s, prof::FACTOR, bytes, 0.0);
     return;
   }
   const auto plan = make_plan(lower, 32);
@@ -895,11 +895,12 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
-  prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
+  prof::end(tok, s, prof::FACTOR, bytes, 0.0);
 }
 
 void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
-                           const int *diagpos, double *fac, int max_row_nnz, int *d_status) {
+                           const int *diagpos, double *fac, int max_row_nnz, int *d_status,
+                           double bytes) {
   if (r1 <= r0) return;
   const int cap = std::max(64, max_row_nnz);
   const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
@@ -911,19 +912,19 @@ void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *r
   hipLaunchKernelGGL(fac_partial_kernel, dim3((r1 - r0 + wpb - 1) / wpb), dim3(64 * wpb), per_wave * wpb, s, f,
                      r0, r1, kmax);
   BPF_CHECK_HIP(hipGetLastError());
-  prof::end(tok, s, prof::FACTOR, 0.0, 0.0);
+  prof::end(tok, s, prof::FACTOR, bytes, 0.0);
 }
 
 void launch_ilu_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
                        const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                       int *d_status) {
-  launch_factor(false, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);
+                       int *d_status, double bytes) {
+  launch_factor(false, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status, bytes);
 }
 
 void launch_ic_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                      int *d_status) {
-  launch_factor(true, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status);
+                      int *d_status, double bytes) {
+  launch_factor(true, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status, bytes);
 }
 
 void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst, int ldd,

@@ -94,9 +94,10 @@ int bpf_timer_stop(bpf_ctx ctx, double *ms);
  * launches on any context are bracketed by HIP events on their stream and
  * summed per family with their algorithmic bytes and flops:
  *   0 SpMM, 1 triangular-solve level launch, 2 block projection (V^T W),
- *   3 block update (V Psi), 4 factorization, 5 small dense kernels.
+ *   3 block update (V Psi), 4 sparse factorization, 5 dense-tail LU of the
+ *   refactorization (its blocked factorization and its solves).
  * Enabling (or disabling) resets the counters. */
-enum { BPF_K_SPMM = 0, BPF_K_TRSV = 1, BPF_K_PROJ = 2, BPF_K_GEMM = 3, BPF_K_FACTOR = 4, BPF_K_SMALL = 5 };
+enum { BPF_K_SPMM = 0, BPF_K_TRSV = 1, BPF_K_PROJ = 2, BPF_K_GEMM = 3, BPF_K_FACTOR = 4, BPF_K_DENSE_LU = 5 };
 int bpf_profile_enable(bpf_ctx ctx, int on);
 int bpf_profile_read(bpf_ctx ctx, int family, long long *launches, double *ms, double *bytes,
                      double *flops);

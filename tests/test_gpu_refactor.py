@@ -190,3 +190,43 @@ def test_rf_refined_stops_at_tolerance(bpf, gen, oracle):
         # a tolerance no column meets runs every step
         _, berr_all = rf.solve_refined(A, B, refine=3, berr_tol=1e-300)
         assert rf.refine_steps == 3
+
+
+def _families(bpf, ctx):
+    import ctypes as C
+    out = []
+    for f in range(6):
+        n = C.c_longlong()
+        ms, by, fl = C.c_double(), C.c_double(), C.c_double()
+        bpf._lib.check(bpf.L.bpf_profile_read(ctx.handle, f, C.byref(n), C.byref(ms), C.byref(by), C.byref(fl)))
+        out.append((n.value, ms.value, by.value, fl.value))
+    return out
+
+
+def test_kernel_timers_cover_factor_and_dense_tail(bpf, gen):
+    # the in-process timers count one sparse-factor launch for the head rows
+    # and one for the tail rows, and the dense-tail LU's factorization and
+    # solves, with their compulsory bytes / flops
+    grid, S = gen.make_system(10000, 16, seed=51)
+    assert S.n >= 16384
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, nrhs=16)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    ctx = A.ctx
+    bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
+    try:
+        bpf.CUSOLVERRF.update_(rf, A)
+        _, berr = rf.solve_refined(A, B, refine=2)
+        fam = _families(bpf, ctx)
+    finally:
+        bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 0))
+    assert berr.max() <= 1e-10
+    P, Q, rp, ci, v = rf.factors()
+    n_fac, ms_fac, by_fac, _ = fam[4]
+    assert n_fac == 2 and ms_fac > 0
+    # every factor entry read and written once (8 + 8 bytes) plus its column index
+    # (4), and three 4-byte row entries per row
+    assert by_fac == pytest.approx(20.0 * len(v) + 12.0 * S.n)
+    n_dlu, ms_dlu, by_dlu, fl_dlu = fam[5]
+    assert n_dlu == 1 + 3 and ms_dlu > 0 and by_dlu > 0 and fl_dlu > 0   # factor + three solves
+    assert fam[1][0] > 0   # level-scheduled triangular solves
