@@ -194,52 +194,91 @@ __global__ __launch_bounds__(kB) void proj_partial_kernel(ProjArgs a) {
 // are read straight from global memory: lane l holds row r + l/16, column
 // l%16 of both V_i and W (16 consecutive values per row), so there is no LDS
 // staging and no barrier.  A workgroup owns one row chunk of one group and
-// all nb blocks: wavefront w takes blocks w, w + 4, ..., kNarrowBatch at a
-// time, so each wavefront reads the chunk of W once per batch.  Partial sums
-// have the layout of proj_partial_kernel's.
-constexpr int kNarrowBatch = 8;
+// all nb blocks: wavefront w takes blocks w, w + 4, w + 8, ... up to
+// kNarrowBatch at a time, so each wavefront reads the chunk of W once per
+// batch.  Loads are unconditional (rows past the chunk re-read its last row,
+// and their W values are zeroed), the block pointers are wavefront-uniform
+// scalars, and the number of live blocks is a template parameter, so the
+// loop holds no per-load branches.  Partial sums have the layout of
+// proj_partial_kernel's.
+constexpr int kNarrowBatch = 4;
 
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const void *p) {
+  // The table entries are wavefront-uniform: keep them in scalar registers.
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
+  return reinterpret_cast<const __attribute__((address_space(1))) T *>(
+      (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+template <int NV, typename TVV, typename TW>
+__device__ __forceinline__ void proj16_rows(const __attribute__((address_space(1))) TVV *const (&V)[kNarrowBatch],
+                                            const __attribute__((address_space(1))) TW *W, int ldv, int ldw,
+                                            int r0, int r1, int lr, int lc, bool colok, dbl4 (&acc)[kNarrowBatch]) {
+  constexpr int RU = 4;   // 4-row MFMA steps per round
+  for (int rr = r0; rr < r1; rr += 4 * RU) {
+    double bw[RU], av[RU][NV];
+#pragma unroll
+    for (int t = 0; t < RU; ++t) {
+      const int r = rr + 4 * t + lr;
+      const int rc = min(r, r1 - 1);
+      // 32-bit element offsets (blocks hold fewer than 2^32 values): scalar
+      // base + vector offset addressing
+      const unsigned ow = static_cast<unsigned>(rc) * static_cast<unsigned>(ldw) + lc;
+      const unsigned ov = static_cast<unsigned>(rc) * static_cast<unsigned>(ldv) + lc;
+      const double w = to_d(W[ow]);
+      bw[t] = (colok && r < r1) ? w : 0.0;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) {
+        const double v = to_d(V[u][ov]);
+        av[t][u] = colok ? v : 0.0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < RU; ++t)
+#pragma unroll
+      for (int u = 0; u < NV; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][u], bw[t], acc[u], 0, 0, 0);
+  }
+}
+
+// The occupancy hint keeps the kernel at 61-67 VGPRs and no AGPRs (7-8
+// wavefronts per SIMD); without it the compiler parks the accumulators in
+// AGPRs and settles at 4.
 template <typename TVV, typename TW>
-__global__ __launch_bounds__(kB) void proj16_kernel(ProjArgs a) {
-  constexpr int NW = kB / 64, NB = kNarrowBatch, RU = 4;   // RU 4-row steps per round
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(6))) void proj16_kernel(ProjArgs a) {
+  constexpr int NW = kB / 64, NB = kNarrowBatch;
   const int chunk = blockIdx.x, g = blockIdx.y;
   const GroupDesc gd = a.groups[g];
   const int r0 = gd.r0 + chunk * a.chunk_rows;
   const int r1 = gd.active ? min(gd.r0 + gd.nr, r0 + a.chunk_rows) : r0;
   if (a.rows_sum > 0 && r0 >= r1) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane >> 4, lc = lane & 15;
   const bool colok = lc < gd.w;
-  const TW *W = static_cast<const TW *>(a.W) + gd.c0;
+  const auto *W = global_ptr<TW>(static_cast<const TW *>(a.W) + gd.c0);
   for (int ib = wave; ib < a.nb; ib += NW * NB) {
-    const TVV *V[NB];
+    const int nv = min(NB, (a.nb - ib + NW - 1) / NW);   // live blocks ib, ib + NW, ...
+    const __attribute__((address_space(1))) TVV *V[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int i = ib + u * NW;
-      V[u] = i < a.nb ? static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0 : nullptr;
+      const int i = min(ib + u * NW, a.nb - 1);
+      V[u] = global_ptr<TVV>(static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0);
     }
     dbl4 acc[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int rr = r0; rr < r1; rr += 4 * RU) {
-      double bw[RU], av[RU][NB];
-#pragma unroll
-      for (int t = 0; t < RU; ++t) {
-        const int r = rr + 4 * t + lr;
-        const bool ok = colok && r < r1;
-        bw[t] = ok ? to_d(W[(size_t)r * a.ldw + lc]) : 0.0;
-#pragma unroll
-        for (int u = 0; u < NB; ++u) av[t][u] = (ok && V[u]) ? to_d(V[u][(size_t)r * a.ldv + lc]) : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < RU; ++t)
-#pragma unroll
-        for (int u = 0; u < NB; ++u)
-          if (V[u]) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][u], bw[t], acc[u], 0, 0, 0);
+    switch (nv) {
+      case 4: proj16_rows<4>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+      case 3: proj16_rows<3>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+      case 2: proj16_rows<2>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+      default: proj16_rows<1>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      if (!V[u]) continue;
+      if (u >= nv) break;
       double *out = a.part + ((size_t)(chunk * a.nb + ib + u * NW) * a.ng + g) * 256;
 #pragma unroll
       for (int q = 0; q < 4; ++q) out[(lr + 4 * q) * 16 + lc] = acc[u][q];
@@ -258,8 +297,18 @@ __global__ __launch_bounds__(kB) void proj_reduce_stacked_kernel(const double *p
   if (e >= total) return;
   const GroupDesc gd = groups[(e / SS) % ng];
   const int nc = gd.active ? min(nchunk, (gd.nr + stacked_rows - 1) / stacked_rows) : 0;
+  // Eight loads in flight, then the additions in chunk order: the same sum,
+  // bit for bit, as one chunk at a time.
   double s = 0.0;
-  for (int c = 0; c < nc; ++c) s += part[(size_t)c * total + e];
+  int c = 0;
+  for (; c + 8 <= nc; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = part[(size_t)(c + q) * total + e];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += v[q];
+  }
+  for (; c < nc; ++c) s += part[(size_t)c * total + e];
   const size_t ig = e / SS, el = e % SS;
   const size_t i = ig / ng, g = ig % ng;
   double *o = out + i * o_istride + g * o_gstride + el;
