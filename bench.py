@@ -11,7 +11,10 @@ Workloads (--workload):
   c2            9,000-bus Jacobian, 64 N(0,1) RHS, same step.
   c4            256 independent systems, n log-uniform in [500, 20000], RHS
                 counts in {8, 16, 32, 64}: one step = one batched solve of all
-                of them (ILU(0) factorizations, Krylov, backward-error check).
+                of them (ILU(0) factorizations, Krylov, backward-error check),
+                its four RHS-count bins solved at the same time.  The kernel
+                timers come from one extra step with the bins one after
+                another.
   c5            the 70,000-bus pattern with 32 Newton value sets and 256 RHS:
                 one step = for each value set, device LU refactorization with
                 fixed pivots + FP32 triangular solves + up to 2 FP64
@@ -326,14 +329,23 @@ def bench_batch(args, bpf, gen):
         nsolved = int(sum(int(ps[i]) for i in range(nsys) if solved[i]))
         return (f"block its {st.niter} solved RHS {nsolved}/{nrhs} max berr {st.max_berr:.2e}", st, nsolved)
 
-    dt, out, fam, dt_prof = timed(bpf, ctx, step, args.steps, args.warmup, "batched")
+    bpf.set_option("batch_threads", args.batch_threads)
+    dt, out, _, _ = timed(bpf, ctx, step, args.steps, args.warmup, "batched", profile=False)
     solved = sum(r[2] for r in out)
     value = solved / dt
     st_last = out[-1][1]
     # Independent check of the last step's solutions.
     be_max = max(float(bpf.backward_error(A, X, B).max()) for A, X, B in zip(As, Xs, Bs))
     log(f"[bench] batched: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nrhs} "
-        f"max berr (rechecked) {be_max:.2e}")
+        f"max berr (rechecked) {be_max:.2e} ({args.batch_threads} bins at a time)")
+    # Kernel timers: one more, untimed step with the bins one after another.
+    # Bins solved at the same time interleave their kernels, so an event pair
+    # around one launch would also enclose other bins' work.
+    bpf.set_option("batch_threads", 1)
+    try:
+        _, _, fam, dt_prof = timed(bpf, ctx, step, 1, 0, "batched, bins one after another")
+    finally:
+        bpf.set_option("batch_threads", args.batch_threads)
     log_families(fam)
     roofline = roofline_from(fam, dt_prof, "c4")
     plain = None
@@ -367,9 +379,10 @@ def bench_batch(args, bpf, gen):
                      + (f" over {args.bmgs_blocks}-block steps" if args.orth == "bmgs" else "")
                      + "), FP32 ILU0 solves"
                      f"{f' in a degree-{args.poly_degree} GMRES polynomial' if args.poly_degree else ''}, "
-                     f"per-system stopping, to per-RHS backward error 1e-10"),
+                     f"per-system stopping, to per-RHS backward error 1e-10; the four RHS-count bins "
+                     f"solved {args.batch_threads} at a time from separate host threads"),
         "nsys": nsys, "sum_n": ntot, "sum_nnz": nnz, "nrhs": nrhs, "orth": args.orth,
-        "bmgs_blocks": args.bmgs_blocks,
+        "bmgs_blocks": args.bmgs_blocks, "batch_threads": args.batch_threads,
         "poly_degree": args.poly_degree,
         "block_iterations": st_last.niter, "max_backward_error": be_max,
     }
@@ -561,6 +574,8 @@ def main():
                          "one-block loop)")
     ap.add_argument("--poly-degree", type=int, default=None,
                     help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
+    ap.add_argument("--batch-threads", type=int, default=4,
+                    help="c4: RHS-count bins solved at the same time, each from its own host thread")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
                     help="block Gram-Schmidt: bmgs (the reference's loop, over --bmgs-blocks steps) or "
                          "bcgs2 (two classical passes)")

@@ -81,7 +81,7 @@ def default_context() -> Context:
     return _DEFAULT
 
 
-_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3}
+_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4}
 
 
 def set_option(name: str, value) -> None:
@@ -90,7 +90,9 @@ def set_option(name: str, value) -> None:
     launch per wide dependency level; "autotune" (default True) times the
     SpMM launch shapes on the first product of each matrix and width;
     "overlap" (default True) runs block GMRES's Hessenberg update of each
-    iteration on a second stream beside the next iteration."""
+    iteration on a second stream beside the next iteration; "batch_threads"
+    (default 4) is the number of RHS-count bins of a batched solve solved at
+    the same time, each from its own host thread on its own streams."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -145,7 +145,10 @@ int bpf_set_option(int option, long long value) {
     if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
     else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
     else if (option == BPF_OPT_OVERLAP) g_overlap_hess = value != 0;
-    else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
+    else if (option == BPF_OPT_BATCH_THREADS) {
+      BPF_REQUIRE(value >= 1 && value <= 64, "batch threads must be in 1..64");
+      g_batch_threads = (int)value;
+    } else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }
 
@@ -155,6 +158,7 @@ int bpf_get_option(int option, long long *value) {
     if (option == BPF_OPT_SYNCFREE) *value = g_syncfree ? 1 : 0;
     else if (option == BPF_OPT_AUTOTUNE) *value = g_autotune ? 1 : 0;
     else if (option == BPF_OPT_OVERLAP) *value = g_overlap_hess ? 1 : 0;
+    else if (option == BPF_OPT_BATCH_THREADS) *value = g_batch_threads;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }

@@ -11,15 +11,21 @@
 // (src/block_gmres.jl:170-411); this is an addition with per-system results
 // identical in meaning to solving each system alone.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <thread>
 
 #include "krylov.h"
 
 using namespace bpf;
+
+namespace bpf {
+std::atomic<int> g_batch_threads{4};
+}
 
 namespace {
 
@@ -29,6 +35,7 @@ double now_s() {
 }
 
 struct Bin {
+  bpf_ctx cctx = nullptr;               // the bin's own streams, buffer pool and scratch
   int p = 0;
   std::vector<int> sys;                 // member systems, in group order
   std::vector<GroupDesc> groups;        // row range and slot of each member
@@ -43,6 +50,8 @@ struct Bin {
     arn.reset();
     if (N) bpf_prec_destroy(N);
     if (A) bpf_csr_destroy(A);
+    for (DevBuf *x : {&d_groups, &B, &X, &R, &Xt, &scale, &stat, &cmtab, &ldtab}) x->release();
+    if (cctx) bpf_ctx_destroy(cctx);
   }
 };
 
@@ -352,9 +361,13 @@ int bpf_batch_create(bpf_ctx ctx, int nsys, const int32_t *n, const int32_t *nnz
       }
       b->rows = r0;
       b->nslots = (int)b->sys.size() * b->p;
-      rethrow(bpf_csr_create(ctx, b->rows, (int)ci.size(), rp.data(), ci.data(), va.data(), &b->A));
-      rethrow(bpf_prec_create(ctx, b->A, BPF_PREC_ILU0, &b->N));
-      upload(b->d_groups, b->groups, ctx->stream);
+      // Each bin runs on a context of its own, so bins can be solved at the
+      // same time from separate host threads.
+      rethrow(bpf_ctx_create(ctx->device, &b->cctx));
+      rethrow(bpf_csr_create(b->cctx, b->rows, (int)ci.size(), rp.data(), ci.data(), va.data(), &b->A));
+      rethrow(bpf_prec_create(b->cctx, b->A, BPF_PREC_ILU0, &b->N));
+      upload(b->d_groups, b->groups, b->cctx->stream);
+      BPF_CHECK_HIP(hipStreamSynchronize(b->cctx->stream));
       bt->bins.push_back(std::move(b));
     }
     *out = bt.release();
@@ -374,11 +387,50 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
     bpf_solver_stats st{};
     bool overtimed = false;
     int max_cycles = 0;
-    for (auto &b : bt->bins) {
+    // The bins' streams start after the work queued on the caller's stream
+    // (which may have produced the right-hand sides).
+    hipEvent_t ready = nullptr;
+    BPF_CHECK_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    struct EventGuard {
+      hipEvent_t e;
+      ~EventGuard() { (void)hipEventDestroy(e); }
+    } ready_guard{ready};
+    BPF_CHECK_HIP(hipEventRecord(ready, bt->ctx->stream));
+    const int nbin = (int)bt->bins.size();
+    std::vector<BinResult> res(nbin);
+    auto run = [&](int i) {
+      Bin &b = *bt->bins[i];
+      BPF_CHECK_HIP(hipStreamWaitEvent(b.cctx->stream, ready, 0));
       // Refresh the factor: the caller may have changed nothing, but a bin is
       // factored once per solve so repeated solves time the same work.
-      rethrow(bpf_prec_factor(b->N, b->A));
-      BinResult r = solve_bin(bt->ctx, *b, d_B, d_X, *prm, t0, done, berr);
+      rethrow(bpf_prec_factor(b.N, b.A));
+      res[i] = solve_bin(b.cctx, b, d_B, d_X, *prm, t0, done, berr);
+      BPF_CHECK_HIP(hipStreamSynchronize(b.cctx->stream));
+      if (b.cctx->stream2) BPF_CHECK_HIP(hipStreamSynchronize(b.cctx->stream2));
+    };
+    // Bins write disjoint systems' entries of done / berr and their own
+    // blocks, so they run on up to g_batch_threads host threads at once.
+    const int nthr = std::max(1, std::min<int>(nbin, g_batch_threads));
+    if (nthr == 1) {
+      for (int i = 0; i < nbin; ++i) run(i);
+    } else {
+      std::atomic<int> next{0};
+      std::vector<std::exception_ptr> err(nthr);
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nthr; ++t)
+        pool.emplace_back([&, t] {
+          try {
+            BPF_CHECK_HIP(hipSetDevice(bt->ctx->device));
+            for (int i; (i = next.fetch_add(1)) < nbin;) run(i);
+          } catch (...) {
+            err[t] = std::current_exception();
+          }
+        });
+      for (auto &th : pool) th.join();
+      for (auto &e : err)
+        if (e) std::rethrow_exception(e);
+    }
+    for (const BinResult &r : res) {
       st.niter += r.iters;
       max_cycles = std::max(max_cycles, r.cycles);
       overtimed = overtimed || r.overtimed;

@@ -182,6 +182,8 @@ extern std::atomic<bool> g_syncfree;
 // Process-wide switch: block-Hessenberg updates on a second stream, beside
 // the next iteration (default on).
 extern std::atomic<bool> g_overlap_hess;
+// Process-wide number of host threads a batched solve runs its bins on.
+extern std::atomic<int> g_batch_threads;
 // Process-wide switch for the SpMM launch-shape autotuner (default on), and
 // removal of a matrix's cached shapes (its row-pointer array is released).
 extern std::atomic<bool> g_autotune;

@@ -66,8 +66,12 @@ const char *bpf_version(void);
  * update of iteration k (src/block_gmres.jl:318-342) on a second stream
  * while iteration k+1's preconditioner, SpMM and Gram-Schmidt run, testing
  * convergence of k meanwhile (one iteration's work is dropped at the end);
- * 0 = one iteration after another.  Results are identical either way. */
-enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3 };
+ * 0 = one iteration after another.  Results are identical either way.
+ * BPF_OPT_BATCH_THREADS: host threads that solve the bins of a batched solve
+ * (systems grouped by RHS count) at the same time, each bin on its own pair
+ * of streams (default 4; 1 = one bin after another).  Bins share nothing, so
+ * results are identical for every value. */
+enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

@@ -97,6 +97,33 @@ def test_batch_overlapped_hessenberg_update_is_exact(bpf, gen):
         np.testing.assert_array_equal(b, a)
 
 
+def test_batch_bins_solved_concurrently_are_exact(bpf, gen, oracle):
+    # Bins (systems grouped by RHS count) solved from several host threads at
+    # once, each on its own streams, give bitwise the one-bin-at-a-time result.
+    systems = _systems(gen, 12, 97, nmin=300, ps=[4, 8, 16, 32, 64])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    out = {}
+    try:
+        for nthr in (1, 4, 2):
+            bpf.set_option("batch_threads", nthr)
+            assert bpf.get_option("batch_threads") == nthr
+            Xs, st, solved = bpf.block_gmres_batched(As, Bs, precision="mixed", berr_tol=1e-10, memory=40,
+                                                     itmax=600)
+            out[nthr] = ([X.to_numpy() for X in Xs], st.niter, solved.copy())
+    finally:
+        bpf.set_option("batch_threads", 4)
+    assert out[1][2].all()
+    for nthr in (4, 2):
+        assert out[nthr][1] == out[1][1]
+        np.testing.assert_array_equal(out[nthr][2], out[1][2])
+        for a, b in zip(out[1][0], out[nthr][0]):
+            np.testing.assert_array_equal(b, a)
+    for S, X in zip(systems, out[4][0]):
+        be = oracle.backward_error(S.indptr, S.indices, S.data, X, S.B)
+        assert be.max() <= 1e-10
+
+
 def test_batch_handle_solves_again_identically(bpf, gen, oracle):
     # A BatchedSystems handle keeps one Krylov engine per RHS-count bin across
     # solves; bins of different widths share the context's buffers.  Solving
