@@ -218,6 +218,7 @@ __device__ __forceinline__ void proj16_rows(const __attribute__((address_space(1
                                             const __attribute__((address_space(1))) TW *W, int ldv, int ldw,
                                             int r0, int r1, int lr, int lc, bool colok, dbl4 (&acc)[kNarrowBatch]) {
   constexpr int RU = 4;   // 4-row MFMA steps per round
+  const int lcl = colok ? lc : 0;
   for (int rr = r0; rr < r1; rr += 4 * RU) {
     double bw[RU], av[RU][NV];
 #pragma unroll
@@ -225,9 +226,10 @@ __device__ __forceinline__ void proj16_rows(const __attribute__((address_space(1
       const int r = rr + 4 * t + lr;
       const int rc = min(r, r1 - 1);
       // 32-bit element offsets (blocks hold fewer than 2^32 values): scalar
-      // base + vector offset addressing
-      const unsigned ow = static_cast<unsigned>(rc) * static_cast<unsigned>(ldw) + lc;
-      const unsigned ov = static_cast<unsigned>(rc) * static_cast<unsigned>(ldv) + lc;
+      // base + vector offset addressing.  Lanes past the group's width (a
+      // narrower group padded to 16) read column 0 and are zeroed below.
+      const unsigned ow = static_cast<unsigned>(rc) * static_cast<unsigned>(ldw) + lcl;
+      const unsigned ov = static_cast<unsigned>(rc) * static_cast<unsigned>(ldv) + lcl;
       const double w = to_d(W[ow]);
       bw[t] = (colok && r < r1) ? w : 0.0;
 #pragma unroll
