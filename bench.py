@@ -20,9 +20,10 @@ Workloads (--workload):
                 fixed pivots + FP32 triangular solves + up to 2 FP64
                 refinement steps, each residual checked for backward error
                 <= 1e-10 on every RHS (refinement stops once all are).
-Inputs are resident in HBM before timing.  Beside each: the plain FP64 GPU
-form on the same input, and the reference algorithm on CPU arrays (the
-oracle) on a bounded sample.
+Inputs are resident in HBM before timing.  The timed steps run with the
+in-process kernel timers off; the roofline comes from one more, untimed step
+with them on.  Beside each: the plain FP64 GPU form on the same input, and
+the reference algorithm on CPU arrays (the oracle) on a bounded sample.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload c3|c2|c4|c5]
 Prints one JSON line.
@@ -120,32 +121,35 @@ def roofline_from(fam, dt_prof, workload):
 
 
 def timed(bpf, ctx, step, steps, warmup, label, profile=True):
-    """warmup untimed steps, then `steps` timed steps.  The kernel timers
-    (HIP events around every launch) are on during the last timed step only,
-    so their own cost touches one step (none with profile=False, as for the
-    baselines); returns (seconds of all timed steps,
-    list of step results, kernel families of the instrumented step, seconds
-    of the instrumented step)."""
+    """warmup untimed steps, then `steps` timed steps with the kernel timers
+    off; with profile=True one more, untimed step runs with the kernel timers
+    (HIP events around every instrumented launch) on, so their own cost never
+    touches the timed steps.  Returns (seconds of the timed steps, list of the
+    timed steps' results, kernel families of the instrumented step or None,
+    seconds of the instrumented step or None)."""
     for w in range(warmup):
         r = step()
         log(f"[bench] warmup {label} {w}: {r[0]}")
     ctx.synchronize()
     out = []
+    t = time.perf_counter()
+    for i in range(steps):
+        out.append(step())
+    ctx.synchronize()
+    dt = time.perf_counter() - t
+    if not profile:
+        return dt, out, None, None
     try:
-        t = time.perf_counter()
-        for i in range(steps):
-            if i == steps - 1:
-                ctx.synchronize()
-                if profile:
-                    bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
-                t_prof = time.perf_counter()
-            out.append(step())
+        bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
+        t_prof = time.perf_counter()
+        r = step()
         ctx.synchronize()
-        t_end = time.perf_counter()
-        fam = family_stats(bpf, ctx) if profile else None
+        dt_prof = time.perf_counter() - t_prof
+        fam = family_stats(bpf, ctx)
     finally:
         bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 0))
-    return t_end - t, out, fam, t_end - t_prof
+    log(f"[bench] kernel-timer step {label}: {r[0]} in {dt_prof:.3f}s")
+    return dt, out, fam, dt_prof
 
 
 def result_line(args, value, dt, dtype, config, extra):
@@ -338,12 +342,12 @@ def bench_batch(args, bpf, gen):
     be_max = max(float(bpf.backward_error(A, X, B).max()) for A, X, B in zip(As, Xs, Bs))
     log(f"[bench] batched: {args.steps} steps {dt:.3f}s solved {solved}/{args.steps * nrhs} "
         f"max berr (rechecked) {be_max:.2e} ({args.batch_threads} bins at a time)")
-    # Kernel timers: one more, untimed step with the bins one after another.
+    # Kernel timers: the extra, untimed step runs the bins one after another.
     # Bins solved at the same time interleave their kernels, so an event pair
     # around one launch would also enclose other bins' work.
     bpf.set_option("batch_threads", 1)
     try:
-        _, _, fam, dt_prof = timed(bpf, ctx, step, 1, 0, "batched, bins one after another")
+        _, _, fam, dt_prof = timed(bpf, ctx, step, 0, 0, "batched, bins one after another")
     finally:
         bpf.set_option("batch_threads", args.batch_threads)
     log_families(fam)
