@@ -52,9 +52,8 @@ PEAK_HBM = 8.0e12
 METRIC = "solved RHS/sec at 1e-10 backward error: mixed+batched vs plain FP64, 1 GPU"
 FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "dense_lu"]
 PEAK_FP64_MFMA = 78.6e12
-# family -> (kernels its timers cover, bound, stem of its counter summary
-# profiles/<workload>_<stem>_traffic.md); "mfma" families are priced in FP64
-# flops, "hbm" families in compulsory bytes.
+# family -> (kernels its timers cover, "mfma" for matrix-core kernels else
+# "hbm", stem of its counter summary profiles/<workload>_<stem>_traffic.md).
 KERNELS = {
     "spmm": ("spmm_rm_kernel", "hbm", "spmm_rm_kernel"),
     "trsv": ("tri_level_kernel, tri_tail_kernel", "hbm", "tri_level_kernel"),
@@ -88,11 +87,15 @@ def log_families(fam):
 
 
 def family_roofline(fam, name):
-    """(bound, achieved, peak, unit) of one kernel family: HBM families as
-    algorithmic bytes / time, matrix-core families as FP64 flops / time."""
+    """(bound, achieved, peak, unit) of one kernel family.  A matrix-core
+    family whose arithmetic intensity (algorithmic FP64 flops per algorithmic
+    byte) reaches the ridge point, 78.6 TFLOP/s / 8 TB/s = 9.8 flop/B, is
+    priced as flops / time against the MFMA peak; below it (narrow blocks),
+    and for the other families, as algorithmic bytes / time against HBM."""
     d = fam[name]
     sec = d["ms"] * 1e-3
-    if KERNELS[name][1] == "mfma":
+    ridge = PEAK_FP64_MFMA / PEAK_HBM
+    if KERNELS[name][1] == "mfma" and (d["bytes"] <= 0 or d["flops"] / d["bytes"] >= ridge):
         return "mfma", (d["flops"] / sec / 1e12 if sec > 0 else 0.0), PEAK_FP64_MFMA / 1e12, "TFLOP/s"
     return "hbm", (d["bytes"] / sec / 1e9 if sec > 0 else 0.0), PEAK_HBM / 1e9, "GB/s"
 
