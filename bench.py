@@ -20,12 +20,16 @@ Workloads (--workload):
                 fixed pivots + FP32 triangular solves + up to 2 FP64
                 refinement steps, each residual checked for backward error
                 <= 1e-10 on every RHS (refinement stops once all are).
+  cases         the reference's own benchmark sweep over the case files it
+                ships (tests/golden/mtx): per case ILU(0) + block GMRES with
+                its settings (atol 1e-10, rtol 0, memory 1, itmax 50), then
+                the same without a preconditioner.
 Inputs are resident in HBM before timing.  The timed steps run with the
 in-process kernel timers off; the roofline comes from one more, untimed step
 with them on.  Beside each: the plain FP64 GPU form on the same input, and
 the reference algorithm on CPU arrays (the oracle) on a bounded sample.
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload c3|c2|c4|c5]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload c3|c2|c4|c5|cases]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -535,6 +539,119 @@ def cpu_baseline_refactor(S, vals, rf, p, budget_s=20.0):
     }
 
 
+# ----------------------------------------------------------------- cases
+# The reference's benchmark sweep (benchmarks/benchmarks_amd.jl:24-66) over
+# the case files it ships with a Jacobian (tests/golden/mtx, copies of its
+# benchmarks/dump).
+CASES = ["case9", "case57", "case118", "case300", "case1354pegase", "case_ACTIVSg2000",
+         "case2869pegase", "case9241pegase", "case_ACTIVSg10k"]
+CASE_KW = dict(atol=1e-10, rtol=0.0, memory=1, restart=False, itmax=50)
+
+
+def bench_cases(args, bpf, gen):
+    mtx = os.path.join(ROOT, "tests", "golden", "mtx")
+    ctx = bpf.default_context()
+    cases = []
+    t0 = time.time()
+    for name in CASES:
+        Ah = bpf.mmread(os.path.join(mtx, f"Gx_{name}.mtx.gz"))
+        Bh = bpf.mmread_dense(os.path.join(mtx, f"Gu_{name}.mtx.gz"))
+        A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
+        B = bpf.HipMatrix.from_numpy(Bh)
+        cases.append({"name": name, "Ah": Ah, "Bh": Bh, "A": A, "B": B, "P": bpf.ilu0(A)})
+    log(f"[bench] read {len(cases)} cases from tests/golden/mtx in {time.time() - t0:.1f}s")
+
+    def solve(c, precond):
+        if precond:
+            c["P"].factor_(c["A"])
+            return bpf.block_gmres(c["A"], c["B"], N=c["P"], ldiv=True, **CASE_KW)
+        return bpf.block_gmres(c["A"], c["B"], **CASE_KW)
+
+    def step():
+        # the sweep as the reference runs it: per case the ILU(0)-preconditioned
+        # solve (ILU(0) factorization included), then the unpreconditioned one
+        rows = []
+        for c in cases:
+            r = {}
+            for precond, key in ((True, "ilu0"), (False, "none")):
+                t = time.perf_counter()
+                X, st = solve(c, precond)
+                ctx.synchronize()
+                r[key] = (time.perf_counter() - t, X, st)
+            rows.append(r)
+        ns = sum(c["Bh"].shape[1] for c, r in zip(cases, rows) if r["ilu0"][2].solved)
+        return (f"{ns} RHS solved with ILU(0)", rows)
+
+    dt, out, fam, dt_prof = timed(bpf, ctx, step, args.steps, args.warmup, "reference sweep")
+    rows = out[-1][1]
+    table = []
+    solved_rhs, t_prec = 0, 0.0
+    for c, r in zip(cases, rows):
+        Ah, Bh = c["Ah"], c["Bh"]
+        ent = {"case": c["name"], "n": Ah.shape[0], "nnz": Ah.nnz, "p": Bh.shape[1]}
+        for key in ("ilu0", "none"):
+            sec, X, st = r[key]
+            be = bpf.backward_error(c["A"], X, c["B"])
+            R = Bh - _csr_matvec(Ah, X.to_numpy())
+            ent[key] = {"ms": round(sec * 1e3, 3), "niter": st.niter, "solved": bool(st.solved),
+                        "rnorm": float(np.linalg.norm(R)), "max_berr": float(be.max())}
+        table.append(ent)
+        if ent["ilu0"]["solved"] and ent["ilu0"]["max_berr"] <= BERR:
+            solved_rhs += ent["p"]
+        t_prec += ent["ilu0"]["ms"] * 1e-3
+        log(f"[bench]   {c['name']:18s} n={ent['n']:6d} p={ent['p']:5d}  ILU(0): its {ent['ilu0']['niter']:3d} "
+            f"solved {ent['ilu0']['solved']!s:5s} |R| {ent['ilu0']['rnorm']:.2e} {ent['ilu0']['ms']:9.2f} ms"
+            f"   none: its {ent['none']['niter']:3d} solved {ent['none']['solved']!s:5s} "
+            f"|R| {ent['none']['rnorm']:.2e} {ent['none']['ms']:9.2f} ms")
+    value = solved_rhs / t_prec
+    log_families(fam)
+    roofline = roofline_from(fam, dt_prof, "cases")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_cases(cases)
+        log(f"[bench] cpu baseline: {cpu['value']:.3f} RHS/s ({cpu['sample']})")
+    config = {
+        "workload": ("cases: the reference benchmark sweep (benchmarks/benchmarks_amd.jl) over its "
+                     f"{len(cases)} case files with a Jacobian, A = Gx (CSR FP64), B = dense Gu; per case "
+                     "ILU(0) + block GMRES(atol=1e-10, rtol=0, memory=1, restart=false, itmax=50), then "
+                     "the same without a preconditioner; value = RHS of the cases the ILU(0) run solves "
+                     "(||R||_F <= 1e-10 and per-RHS backward error <= 1e-10) / the ILU(0) runs' time"),
+        "cases": table,
+    }
+    res = result_line(args, value, dt, "f64", config, {
+        "plain_fp64": None, "speedup_vs_plain_fp64": None, "roofline": roofline, "cpu_baseline": cpu})
+    res["data"] = "reference case files (benchmarks/dump)"
+    return res
+
+
+def _csr_matvec(Ah, X):
+    import scipy.sparse as sp
+    return sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape) @ X
+
+
+def cpu_baseline_cases(cases, budget_s=20.0):
+    """The reference algorithm restated on CPU arrays (the oracle's block
+    GMRES + ILU(0), same settings) over the cases with n p <= 6e6 in sweep
+    order, stopping once the budget is spent."""
+    import scipy.sparse as sp
+    o = _oracle()
+    nrhs, t_all, done = 0, 0.0, []
+    for c in cases:
+        Ah, Bh = c["Ah"], c["Bh"]
+        if Ah.shape[0] * Bh.shape[1] > 6e6 or t_all > budget_s:
+            break
+        As = sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape)
+        t = time.perf_counter()
+        F = o.ilu0(Ah.indptr, Ah.indices, Ah.data)
+        _, st = o.block_gmres(As, Bh, N=lambda Z: o.ilu0_apply(Ah.indptr, Ah.indices, F, Z), **CASE_KW)
+        t_all += time.perf_counter() - t
+        nrhs += Bh.shape[1]
+        done.append(c["name"])
+    return {"value": nrhs / t_all, "unit": "RHS/s", "cores": cpu_threads(), "kind": "port",
+            "sample": (f"reference algorithm restated (oracle/oracle.py block_gmres + ILU0, same settings) on "
+                       f"{', '.join(done)} in {t_all:.1f} s, RHS counted whether or not solved")}
+
+
 def copy_probe(nbytes=1 << 30, copies=3):
     """Known-size copies for calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE
     counters: each copy reads and writes nbytes of HBM."""
@@ -568,7 +685,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5", "cases"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block-width", type=int, default=0)
@@ -610,6 +727,8 @@ def main():
         res = bench_batch(args, bpf, gen)
     elif args.workload == "c5":
         res = bench_refactor(args, bpf, gen)
+    elif args.workload == "cases":
+        res = bench_cases(args, bpf, gen)
     else:
         res = bench_single(args, bpf, gen)
     print(json.dumps(res))

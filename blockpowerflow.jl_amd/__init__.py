@@ -34,6 +34,7 @@ __all__ = [
     "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
     "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
     "block_gmres_batched", "householder", "block_hessenberg_lsq", "set_option", "get_option",
+    "HostCSR", "mmread", "mmread_dense", "mmwrite",
 ]
 
 L = _lib.load()
@@ -595,3 +596,4 @@ def block_hessenberg_lsq(H: np.ndarray, G: np.ndarray, ctx: Context | None = Non
 
 from .cusolverrf import CUSOLVERRF  # noqa: E402
 from .batched import BatchedSystems, block_gmres_batched  # noqa: E402
+from .matrixmarket import HostCSR, mmread, mmread_dense, mmwrite  # noqa: E402

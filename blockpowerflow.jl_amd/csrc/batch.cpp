@@ -267,10 +267,25 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
       check_berr();
     } else {
       residual_rm(s, b.A, p, b.B.as<double>(), b.X.as<double>(), b.R.as<double>());
-      for (int j = 0; j < m; ++j) done[j] = done[j] || est_ok[j];
     }
     group_stats(ctx, b, b.R.as<double>(), 1, rn);
     for (auto &v : rn) v = std::sqrt(v);
+    if (!use_berr) {
+      // After a Cholesky breakdown in CholQR2 the residual estimates are not
+      // reliable: the true residual decides (src/block_gmres.jl:349's rule
+      // applied to it), and unsolved systems restart from it.
+      const bool trust = !arn.shifted();
+      for (int j = 0; j < m; ++j) {
+        if (done[j]) continue;
+        if (trust) {
+          done[j] = est_ok[j];
+        } else {
+          double f = 0.0;
+          for (int c = 0; c < p; ++c) f += rn[b.groups[j].slot + c] * rn[b.groups[j].slot + c];
+          done[j] = std::sqrt(f) <= eps[j];
+        }
+      }
+    }
     if (prm.timemax > 0 && now_s() - t0 > prm.timemax) res.overtimed = true;
     if (its == 0) break;
   }

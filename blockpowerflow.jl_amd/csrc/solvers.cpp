@@ -189,7 +189,19 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
     col_norms2(ctx, n, p, blk.R.as<double>(), blk.stat, rn);
     // With a backward-error target the true residual decides; otherwise the
     // reference's rule: the residual estimate ||Lambda||_F <= eps (:349).
-    solved = use_berr ? berr_ok() : est_ok;
+    // A Cholesky breakdown in CholQR2 (a rank-deficient block, as with
+    // linearly dependent right-hand sides; the shifted factor leaves the
+    // basis short of orthonormal) makes the estimate unreliable: then the
+    // true residual decides, and a further cycle restarts from it.
+    if (use_berr) {
+      solved = berr_ok();
+    } else if (arn.shifted()) {
+      double F = 0.0;
+      for (double v : rn) F += v * v;
+      solved = std::sqrt(F) <= eps;
+    } else {
+      solved = est_ok;
+    }
     tired = iter >= itmax;
     if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
     if (its == 0) break;

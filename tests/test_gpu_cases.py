@@ -1,0 +1,95 @@
+"""The reference's benchmark flow (benchmarks/benchmarks_amd.jl:33-66) on its
+own case files: A = mmread(Gx), B = Matrix(mmread(Gu)), P = ilu0(A), then
+block_gmres(A, B; N=P, ldiv=true, atol=1e-10, rtol=0, memory=1,
+restart=false, itmax=50) and the same without a preconditioner.  Checked
+against the CPU restatement of the same algorithm (the oracle) on the same
+inputs, and against a sparse direct solve."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+MTX = os.path.join(ROOT, "tests", "golden", "mtx")
+KW = dict(atol=1e-10, rtol=0.0, memory=1, restart=False, itmax=50)
+
+
+def _case(bpf, name):
+    Ah = bpf.mmread(os.path.join(MTX, f"Gx_{name}.mtx.gz"))
+    Bh = bpf.mmread_dense(os.path.join(MTX, f"Gu_{name}.mtx.gz"))
+    return Ah, Bh
+
+
+@pytest.mark.parametrize("name", ["case9", "case57", "case118"])
+@pytest.mark.parametrize("precond", [True, False])
+def test_benchmark_flow_matches_cpu_restatement(bpf, oracle, name, precond):
+    # p <= 128: one Krylov block of all p columns, as in the reference.  Where
+    # the right-hand-side block has full rank (case9) the GPU and CPU runs
+    # take the same iterations to rounding.  The Gu blocks of case57 and
+    # case118 are rank-deficient (condition numbers ~1e18): Householder QR
+    # completes V_1 with arbitrary orthonormal directions, while CholQR2's
+    # Cholesky breaks down (and is shifted), which leaves the residual
+    # estimate unreliable; the library then judges convergence by the true
+    # residual and restarts from it.  What must hold there: a run reported
+    # solved has a true residual at the tolerance and the direct solution.
+    Ah, Bh = _case(bpf, name)
+    full_rank = np.linalg.matrix_rank(Bh) == Bh.shape[1]
+    A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
+    B = bpf.HipMatrix.from_numpy(Bh)
+    As = sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape)
+    if precond:
+        P = bpf.ilu0(A)
+        X, st = bpf.block_gmres(A, B, N=P, ldiv=True, history=True, **KW)
+        F = oracle.ilu0(Ah.indptr, Ah.indices, Ah.data)
+        Nf = lambda Z: oracle.ilu0_apply(Ah.indptr, Ah.indices, F, Z)
+        Xc, stc = oracle.block_gmres(As, Bh, N=Nf, history=True, **KW)
+    else:
+        X, st = bpf.block_gmres(A, B, history=True, **KW)
+        Xc, stc = oracle.block_gmres(As, Bh, history=True, **KW)
+    r0 = stc.residuals[0]
+    assert st.residuals[0] == pytest.approx(r0, rel=1e-12)
+    Xg = X.to_numpy()
+    rn = np.linalg.norm(Bh - As @ Xg)
+    rnc = np.linalg.norm(Bh - As @ Xc)
+    Xd = oracle.direct_solve(Ah.indptr, Ah.indices, Ah.data, Bh)
+    if full_rank:
+        assert st.solved == stc.solved
+        assert abs(st.niter - stc.niter) <= 1
+        k = min(len(st.residuals), len(stc.residuals), 6)
+        np.testing.assert_allclose(st.residuals[:k], stc.residuals[:k], rtol=1e-6, atol=1e-9 * r0)
+        if not stc.solved:
+            assert rn == pytest.approx(rnc, rel=1e-3)
+    if st.solved:
+        # ||B - A X||_F <= atol, up to the rounding of recomputing it
+        assert rn <= 1e-10 * 10
+        np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
+    else:
+        assert st.niter >= KW["itmax"]
+    if stc.solved:
+        assert rnc <= 1e-10 * 10
+
+
+@pytest.mark.parametrize("name", ["case300", "case1354pegase"])
+def test_benchmark_flow_many_rhs(bpf, oracle, name):
+    # p > 128 (137 and 519 right-hand sides): the library splits the columns
+    # into near-equal Krylov blocks of at most 128.  A run reported solved
+    # has a true residual at the tolerance and matches a direct solve; one
+    # that is not used all its iterations.
+    Ah, Bh = _case(bpf, name)
+    assert Bh.shape[1] > 128
+    A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
+    B = bpf.HipMatrix.from_numpy(Bh)
+    P = bpf.ilu0(A)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, **KW)
+    As = sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape)
+    Xg = X.to_numpy()
+    if st.solved:
+        assert np.linalg.norm(Bh - As @ Xg) <= 1e-10 * 10
+        Xd = oracle.direct_solve(Ah.indptr, Ah.indices, Ah.data, Bh)
+        np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
+    else:
+        assert st.niter >= KW["itmax"]
