@@ -128,6 +128,10 @@ void BlockArnoldi::upload_groups() { upload(d_groups_, h_groups_, st_); }
 BlockArnoldi::~BlockArnoldi() {
   // the second stream, its events and h_rn_ belong to the context
   if (hs_) (void)hipStreamSynchronize(hs_);
+  if (h_fl_) {
+    (void)hipStreamSynchronize(st_);
+    (void)hipHostFree(h_fl_);
+  }
 }
 
 double *BlockArnoldi::col_region(int k) {
@@ -301,6 +305,19 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   vtab_.set(0, V_.block(0), st_);
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
   cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
+  // Snapshots of the CholQR flag word: entry j + 1 after iteration j's
+  // CholQR2, entry 0 after V_0's, so the run reports the breakdowns of the
+  // iterations it keeps only (an overlapped run queues one it drops).
+  if (h_fl_cap_ < maxit + 2) {
+    if (h_fl_) {
+      BPF_CHECK_HIP(hipStreamSynchronize(st_));
+      BPF_CHECK_HIP(hipHostFree(h_fl_));
+      h_fl_ = nullptr;
+    }
+    BPF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_fl_), (size_t)(maxit + 2) * sizeof(int)));
+    h_fl_cap_ = maxit + 2;
+  }
+  BPF_CHECK_HIP(hipMemcpyAsync(h_fl_, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
   // (the context's pinned buffer may have been regrown for another solver
   // since this one was built)
   ctx_->ensure_stream2(std::max(1, nslots_));
@@ -352,6 +369,7 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     const size_t gstr = (size_t)(k + 2) * SS;
     // V_{k+1} C = W; C into block k+1 of each group's column.
     cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr);
+    BPF_CHECK_HIP(hipMemcpyAsync(h_fl_ + k + 1, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
     vtab_.set(k + 1, V_.block(k + 1), st_);
   };
   // QR update of the block Hessenberg matrix for iteration k and the copy
@@ -399,9 +417,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   // the solution assembly below reads the last update's results
   BPF_CHECK_HIP(hipStreamWaitEvent(st_, ev_hess_, 0));
   if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)nslots_ * sizeof(double));
-  int fl = 0;
-  BPF_CHECK_HIP(hipMemcpy(&fl, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost));
-  shifted_ = fl != 0;
+  BPF_CHECK_HIP(hipStreamSynchronize(st_));
+  shifted_ = h_fl_[k_] != 0;   // the flags after the last kept iteration
   form<TV>(X, 1.0);
   if (N_) {
     triop_check(N_->lo, st_);

@@ -127,6 +127,8 @@ class BlockArnoldi {
   int rows_sum_ = 0;           // > 0: groups stacked by rows (sum of heights)
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
+  int *h_fl_ = nullptr;        // pinned snapshots of the CholQR flag word
+  int h_fl_cap_ = 0;
   hipStream_t st_;
   // Second stream (the context's): the Hessenberg update of iteration k runs
   // there while iteration k+1's preconditioner, SpMM and Gram-Schmidt are

@@ -93,3 +93,24 @@ def test_benchmark_flow_many_rhs(bpf, oracle, name):
         np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
     else:
         assert st.niter >= KW["itmax"]
+
+
+def test_breakdown_rule_is_the_same_overlapped_or_not(bpf):
+    # case57's rank-deficient right-hand sides make CholQR2 break down; the
+    # true-residual rule that follows must see the same breakdowns whether
+    # the Hessenberg updates overlap the next iteration (which queues one
+    # iteration it may drop) or not: bitwise the same result.
+    Ah, Bh = _case(bpf, "case57")
+    A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
+    B = bpf.HipMatrix.from_numpy(Bh)
+    out = {}
+    try:
+        for ovl in (0, 1):
+            bpf.set_option("overlap", ovl)
+            P = bpf.ilu0(A)
+            X, st = bpf.block_gmres(A, B, N=P, ldiv=True, **KW)
+            out[ovl] = (X.to_numpy(), st.niter, st.solved)
+    finally:
+        bpf.set_option("overlap", 1)
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
