@@ -69,7 +69,13 @@ struct CholArgs {
   int S;
   int n;               // rows of the block (for the shift)
   const GroupDesc *groups;
-  int *flags;          // bit 0: a shift was needed
+  int *flags;          // [0]: bits kept over a solve, [1]: bits of this factorization;
+                       // bit 0: a shift was needed, bit 1: a column was deflated
+  // 1: a pivot that is not positive (column k lies in the span of the columns
+  // before it to rounding) gives R a zero row k instead of a shift, with 1 on
+  // the diagonal of the factor that Rinv inverts; launch_fill_deflated then
+  // replaces column k of W Rinv.
+  int deflate = 0;
 };
 
 // Hessenberg arena: block column k is a separate region cols[k] holding, per
@@ -129,6 +135,13 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
 template <typename TVV, typename TY>
 void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng);
 void launch_chol(hipStream_t s, CholArgs a, int ng);
+// After a deflating Cholesky (bit 1 of *flags): column k of group g's rows of
+// Q (row-major, ld) becomes pseudo-random values of unit expected norm
+// wherever R[g] (S x S at g*stride) has a zero diagonal entry k.  Returns at
+// once when no column was deflated.
+template <typename TV>
+void launch_fill_deflated(hipStream_t s, TV *Q, int ld, const double *R, int S, size_t stride,
+                          const GroupDesc *groups, int ng, int gmax_rows, const int *flags);
 // C_g = A_g B_g for upper-triangular S x S blocks (A_g, B_g at g*stride,
 // C_g at g*cstride).
 void launch_trmm_upper(hipStream_t s, const double *A, const double *B, double *C, int S,

@@ -485,7 +485,13 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
 // row by forward substitution (one thread per column), the trailing update
 // on matrix cores.  If a pivot is not positive the factorization restarts
 // with a shift sigma = 11 (n w + w (w+1)) u ||G||_F (Fukaya et al. 2020) and
-// flags it.  R^{-1} is formed blockwise: diagonal 16 x 16 inverses, then
+// flags it (bit 0).  With a.deflate, a pivot that is not positive (column k
+// lies in the span of the columns before it to rounding, as with linearly
+// dependent right-hand sides) instead gives R a zero row k (bit 1); the
+// factor that Rinv inverts keeps 1 there, so column k of W Rinv is the
+// rounding-level remainder of w_k, which launch_fill_deflated replaces.  Bits
+// go to flags[0] (kept over a solve) and flags[1] (this factorization).
+// R^{-1} is formed blockwise: diagonal 16 x 16 inverses, then
 // X_IJ = -X_II sum_{I<M<=J} R_IM X_MJ, one block column per wavefront; X is
 // kept transposed in the strict lower triangle with its diagonal in dinv.
 constexpr int kCholLd = 129;
@@ -502,6 +508,7 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   __shared__ double scr[8][16][17];
   __shared__ int s_fail;
   __shared__ double s_norm, s_part[16];
+  __shared__ unsigned char s_dep[128];
   const int g = blockIdx.x;
   if (!a.groups[g].active) return;
   const int w = a.groups[g].w;
@@ -512,6 +519,7 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   double *R = a.R + (size_t)g * a.gstride;
   double *Ri = a.Rinv + (size_t)g * a.gstride;
   if (tid == 0) s_norm = 0.0;
+  if (tid < 128) s_dep[tid] = 0;
   __syncthreads();
   double loc = 0.0;
   for (int e = tid; e < w * w; e += 1024) {
@@ -544,6 +552,19 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
         for (int kk = 0; kk < pw; ++kk) {
           const int k = j0 + kk;
           const double d = G[k * kCholLd + k];
+          if (a.deflate && isfinite(d) && d <= 0.0) {
+            // column k depends on columns < k: zero row k of R, no update
+            if (lane > kk && lane < pw) G[k * kCholLd + j0 + lane] = 0.0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (lane == 0) {
+              G[k * kCholLd + k] = 1.0;
+              s_dep[k] = 1;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            continue;
+          }
           if (!(d > 0.0) || !isfinite(d)) {
             if (lane == 0) s_fail = 1;
             break;
@@ -568,6 +589,10 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
       for (int c = j0 + pw + tid; c < w; c += 1024) {
         for (int kk = 0; kk < pw; ++kk) {
           const int k = j0 + kk;
+          if (s_dep[k]) {
+            G[k * kCholLd + c] = 0.0;
+            continue;
+          }
           double v = G[k * kCholLd + c];
           for (int m = 0; m < kk; ++m) v -= G[(j0 + m) * kCholLd + k] * G[(j0 + m) * kCholLd + c];
           G[k * kCholLd + c] = v / G[k * kCholLd + k];
@@ -598,10 +623,14 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
     }
     __syncthreads();
     if (!s_fail) break;
+    if (tid < 128) s_dep[tid] = 0;
     shift = shift == 0.0 ? 11.0 * ((double)a.n * w + (double)w * (w + 1)) * 1.1102230246251565e-16 * gnorm
                          : shift * 100.0;
     if (shift == 0.0) shift = 1e-300;
-    if (tid == 0) atomicOr(a.flags, 1);
+    if (tid == 0) {
+      atomicOr(a.flags, 1);
+      atomicOr(a.flags + 1, 1);
+    }
   }
   // diagonal 16 x 16 inverses (one thread per column)
   for (int c = tid; c < w; c += 1024) dinv[c] = 1.0 / G[c * kCholLd + c];
@@ -656,13 +685,55 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
     const int r = e / S, c = e % S;
     double rv, iv;
     if (r < w && c < w) {
-      rv = c >= r ? G[r * kCholLd + c] : 0.0;
+      rv = c >= r && !s_dep[r] ? G[r * kCholLd + c] : 0.0;
       iv = c > r ? G[c * kCholLd + r] : (c == r ? dinv[r] : 0.0);
     } else {
       rv = iv = (r == c) ? 1.0 : 0.0;
     }
     R[r * S + c] = rv;
     Ri[r * S + c] = iv;
+  }
+  if (tid == 0) {
+    int dep = 0;
+    for (int k = 0; k < w; ++k) dep |= s_dep[k];
+    if (dep) {
+      atomicOr(a.flags, 2);
+      atomicOr(a.flags + 1, 2);
+    }
+  }
+}
+
+// Pseudo-random fill of deflated columns: splitmix64 of (group, column, row)
+// mapped to a uniform value in [-1, 1), times sqrt(3 / rows) so a column has
+// unit expected norm.  One thread per row of a group.
+__device__ __forceinline__ double fill_value(unsigned long long key) {
+  key += 0x9E3779B97F4A7C15ull;
+  key = (key ^ (key >> 30)) * 0xBF58476D1CE4E5B9ull;
+  key = (key ^ (key >> 27)) * 0x94D049BB133111EBull;
+  key ^= key >> 31;
+  return (double)(key >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kB) void fill_deflated_kernel(TV *Q, int ld, const double *R, int S, size_t stride,
+                                                           const GroupDesc *groups, const int *flags) {
+  if (!(*flags & 2)) return;
+  __shared__ unsigned char dep[128];
+  const int g = blockIdx.y;
+  const GroupDesc gd = groups[g];
+  if (!gd.active) return;
+  const double *Rg = R + (size_t)g * stride;
+  if (threadIdx.x < gd.w) dep[threadIdx.x] = Rg[(size_t)threadIdx.x * S + threadIdx.x] == 0.0;
+  __syncthreads();
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= gd.nr) return;
+  const double scale = sqrt(3.0 / gd.nr);
+  TV *row = Q + (size_t)(gd.r0 + i) * ld + gd.c0;
+  for (int k = 0; k < gd.w; ++k) {
+    if (!dep[k]) continue;
+    const unsigned long long key =
+        ((unsigned long long)g << 48) ^ ((unsigned long long)k << 32) ^ (unsigned long long)i;
+    row[k] = static_cast<TV>(scale * fill_value(key));
   }
 }
 
@@ -1435,6 +1506,19 @@ void launch_chol(hipStream_t s, CholArgs a, int ng) {
   hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a);
   BPF_CHECK_HIP(hipGetLastError());
 }
+
+template <typename TV>
+void launch_fill_deflated(hipStream_t s, TV *Q, int ld, const double *R, int S, size_t stride,
+                          const GroupDesc *groups, int ng, int gmax_rows, const int *flags) {
+  if (ng <= 0 || gmax_rows <= 0) return;
+  dim3 grid((gmax_rows + kB - 1) / kB, ng);
+  hipLaunchKernelGGL(fill_deflated_kernel<TV>, grid, dim3(kB), 0, s, Q, ld, R, S, stride, groups, flags);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+template void launch_fill_deflated<double>(hipStream_t, double *, int, const double *, int, size_t,
+                                           const GroupDesc *, int, int, const int *);
+template void launch_fill_deflated<float>(hipStream_t, float *, int, const double *, int, size_t,
+                                          const GroupDesc *, int, int, const int *);
 
 void launch_trmm_upper(hipStream_t s, const double *A, const double *B, double *C, int S,
                        size_t stride, size_t cstride, const GroupDesc *groups, int ng) {

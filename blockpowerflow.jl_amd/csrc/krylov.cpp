@@ -96,7 +96,7 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   ng_ = (int)h_groups_.size();
   // per-solve work buffers come from the context's pool
   for (DevBuf *b : {&d_groups_, &W_, &P_, &E_, &rnorm_, &Y_, &Zb_, &Rinvb_, &Xtmp_, &Vf_, &Zf_, &tmpcol_,
-                    &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.flags, &cq_.part})
+                    &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.R3, &cq_.R3i, &cq_.flags, &cq_.part})
     b->pool = &ctx->pool;
   V_.pool = Z_.pool = &ctx->pool;
   upload(d_groups_, h_groups_, st_);
@@ -147,41 +147,83 @@ double *BlockArnoldi::col_region(int k) {
 
 void CholQrWork::init(int ng, int S) {
   const size_t ss = (size_t)ng * S * S * sizeof(double);
-  G.ensure(ss); R1.ensure(ss); R1i.ensure(ss); R2.ensure(ss); R2i.ensure(ss);
+  G.ensure(ss); R1.ensure(ss); R1i.ensure(ss); R2.ensure(ss); R2i.ensure(ss); R3.ensure(ss); R3i.ensure(ss);
   flags.ensure(16);
 }
 
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
-                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w) {
+                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
+                    const std::function<void(void *)> &reorth) {
   // Two CholQR passes; C = R2 R1 (upper), Vout = W R1^{-1} R2^{-1}.
   const size_t SS = (size_t)S * S;
-  void *src = W;
-  DevBuf *Rs[2] = {&w.R1, &w.R2};
-  DevBuf *Ris[2] = {&w.R1i, &w.R2i};
-  for (int pass = 0; pass < 2; ++pass) {
+  int *fl = w.flags.as<int>();
+  fill_int(s, fl + 1, 0, 1);
+  DevBuf *Rs[3] = {&w.R1, &w.R2, &w.R3};
+  DevBuf *Ris[3] = {&w.R1i, &w.R2i, &w.R3i};
+  auto pass = [&](int i, void *src, void *dst, bool deflate) {
     ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows, dg, ng, 0, nullptr, rows_sum};
     launch_proj<TV, TV>(s, pa, S, w.G.as<double>(), 0, SS, false, w.part);
-    CholArgs ca{w.G.as<double>(), Rs[pass]->as<double>(), Ris[pass]->as<double>(), SS, S, n_shift, dg,
-                w.flags.as<int>()};
+    CholArgs ca{w.G.as<double>(), Rs[i]->as<double>(), Ris[i]->as<double>(), SS, S, n_shift, dg, fl};
+    ca.deflate = deflate;
     launch_chol(s, ca, ng);
-    void *dst = pass == 1 ? Vout : W;
-    GemmArgs ga{nullptr, src, 0, 1, p, Ris[pass]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, dg};
+    GemmArgs ga{nullptr, src, 0, 1, p, Ris[i]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, dg};
     ga.rows_sum = rows_sum;
     launch_block_gemm<TV, TV>(s, ga, S, ng);
-    src = dst;
+  };
+  // The first pass drops columns whose Cholesky pivot is not positive (they
+  // lie in the span of the columns before them: linearly dependent
+  // right-hand sides, an exhausted Krylov space) and fills them with
+  // pseudo-random directions, as Householder QR completes a rank-deficient
+  // block with arbitrary orthonormal ones; W = Q1 R1 holds with zero rows of
+  // R1 there.
+  pass(0, W, W, true);
+  launch_fill_deflated<TV>(s, static_cast<TV *>(W), p, w.R1.as<double>(), S, SS, dg, ng, gmax_rows, fl + 1);
+  int h = 0;
+  BPF_CHECK_HIP(hipMemcpyAsync(&h, fl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  BPF_CHECK_HIP(hipStreamSynchronize(s));
+  if (!(h & 2)) {
+    pass(1, W, Vout, false);
+    launch_trmm_upper(s, w.R2.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
+    return;
   }
-  launch_trmm_upper(s, w.R2.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
+  // Deflated: the new directions are orthogonal to nothing yet.  The caller
+  // projects the block off the earlier basis (the rest of the block only
+  // moves at rounding level: W was projected before), then two passes make
+  // it orthonormal; C = R3 R2 R1, the product staged in G.
+  if (reorth) reorth(W);
+  pass(1, W, W, false);
+  pass(2, W, Vout, false);
+  launch_trmm_upper(s, w.R3.as<double>(), w.R2.as<double>(), w.G.as<double>(), S, SS, SS, dg, ng);
+  launch_trmm_upper(s, w.G.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
 }
 template void cholqr2_groups<double>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                     void *, double *, size_t, CholQrWork &);
+                                     void *, double *, size_t, CholQrWork &,
+                                     const std::function<void(void *)> &);
 template void cholqr2_groups<float>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                    void *, double *, size_t, CholQrWork &);
+                                    void *, double *, size_t, CholQrWork &,
+                                    const std::function<void(void *)> &);
 
 template <typename TV>
-void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride) {
+void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride, int kprev) {
+  std::function<void(void *)> reorth;
+  if (kprev >= 0) {
+    // one classical Gram-Schmidt pass of the block against V_0..V_kprev
+    reorth = [this, kprev](void *Wp) {
+      const GroupDesc *dg = d_groups_.as<GroupDesc>();
+      const size_t SS = (size_t)S_ * S_, gstr = (size_t)(kprev + 2) * SS;
+      const void *const *vt = vtab_.dev<const void>();
+      tmpcol_.ensure(hcol_elems(kprev, ng_, S_) * sizeof(double));
+      double *t = tmpcol_.as<double>();
+      ProjArgs pa{vt, nullptr, 0, kprev + 1, cfg_.p, Wp, cfg_.p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
+      launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
+      GemmArgs ga{vt, nullptr, 0, kprev + 1, cfg_.p, t, SS, gstr, S_, Wp, cfg_.p, gmax_rows_, -1.0, 1.0, dg};
+      ga.rows_sum = rows_sum_;
+      launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+    };
+  }
   cholqr2_groups<TV>(st_, d_groups_.as<GroupDesc>(), ng_, S_, cfg_.p, gmax_rows_, rows_sum_, cfg_.n, W, Vout,
-                     Cout, cstride, cq_);
+                     Cout, cstride, cq_, reorth);
 }
 
 // Psi_{i,k} = V_i^T W and W -= V_i Psi_{i,k}, i = 0..k, into block column k.
@@ -304,7 +346,7 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   fill_double(st_, E_.as<double>(), 0.0, 2 * (size_t)ng_ * SS);
   vtab_.set(0, V_.block(0), st_);
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
-  cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS);
+  cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS, -1);
   // Snapshots of the CholQR flag word: entry j + 1 after iteration j's
   // CholQR2, entry 0 after V_0's, so the run reports the breakdowns of the
   // iterations it keeps only (an overlapped run queues one it drops).
@@ -368,7 +410,7 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     orthogonalize<TV>(k, col);
     const size_t gstr = (size_t)(k + 2) * SS;
     // V_{k+1} C = W; C into block k+1 of each group's column.
-    cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr);
+    cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr, k);
     BPF_CHECK_HIP(hipMemcpyAsync(h_fl_ + k + 1, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
     vtab_.set(k + 1, V_.block(k + 1), st_);
   };

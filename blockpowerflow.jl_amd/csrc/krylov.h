@@ -47,20 +47,25 @@ struct PtrTable {
   template <typename T> T *const *dev() const { return d.as<T *>(); }
 };
 
-// Buffers of a grouped CholQR2 (per group S x S: Gram matrix, two R factors
-// and their inverses; the shift flag; projection partials).
+// Buffers of a grouped CholQR2 (per group S x S: Gram matrix, up to three R
+// factors and their inverses; the flag words; projection partials).
 struct CholQrWork {
-  DevBuf G, R1, R1i, R2, R2i, flags, part;
+  DevBuf G, R1, R1i, R2, R2i, R3, R3i, flags, part;
   void init(int ng, int S);
 };
 
 // Grouped CholQR2 of W (row-major, ld p): Vout = W R1^{-1} R2^{-1} with
 // C = R2 R1 (upper, per group at Cout + g*cstride).  W is overwritten by the
-// first pass.  n_shift is the row count used in the shift of a failed pivot;
-// w.flags bit 0 is set when a shift was needed (cleared by the caller).
+// first pass.  Columns whose first-pass Cholesky pivot is not positive are
+// deflated (zero rows of R1) and replaced by pseudo-random directions; then
+// `reorth` (if set) projects the block off the earlier basis and two more
+// passes follow, C = R3 R2 R1.  n_shift is the row count used in the shift of
+// a failed pivot; w.flags[0] gathers bit 0 (shift) and bit 1 (deflation)
+// and is cleared by the caller.
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
-                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w);
+                    int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
+                    const std::function<void(void *)> &reorth = {});
 
 // Host-visible description of one Krylov run.
 struct ArnoldiConfig {
@@ -113,7 +118,8 @@ class BlockArnoldi {
   template <typename TV> int run_t(const double *R0, double *X, int maxit, const StopFn &stop,
                                    double *rnorm_out);
   template <typename TV> void form(double *X, double alpha);
-  template <typename TV> void cholqr2(void *W, void *Vout, double *Cout, size_t cstride);
+  // kprev: the last earlier basis block (-1 for V_0)
+  template <typename TV> void cholqr2(void *W, void *Vout, double *Cout, size_t cstride, int kprev);
   template <typename TV> void orthogonalize(int k, double *col);
   double *col_region(int k);
 

@@ -189,10 +189,10 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
     col_norms2(ctx, n, p, blk.R.as<double>(), blk.stat, rn);
     // With a backward-error target the true residual decides; otherwise the
     // reference's rule: the residual estimate ||Lambda||_F <= eps (:349).
-    // A Cholesky breakdown in CholQR2 (a rank-deficient block, as with
-    // linearly dependent right-hand sides; the shifted factor leaves the
-    // basis short of orthonormal) makes the estimate unreliable: then the
-    // true residual decides, and a further cycle restarts from it.
+    // After a breakdown in CholQR2 (a rank-deficient block, as with linearly
+    // dependent right-hand sides: columns completed with new directions, or a
+    // shifted factor) the estimate is not to be trusted: then the true
+    // residual decides, and a further cycle restarts from it.
     if (use_berr) {
       solved = berr_ok();
     } else if (arn.shifted()) {

@@ -223,8 +223,12 @@ int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
  * householder!(Q, R, tau) (src/block_utils.jl:38-53), the orthonormalization
  * of block_gmres (src/block_gmres.jl:276,316).  Computed by CholQR2 (two
  * Cholesky-QR passes, FP64 Gram matrices on matrix cores); R is upper
- * triangular with a positive diagonal (geqrf's R up to row signs).  *shifted
- * (optional) reports that a failed Cholesky pivot was repaired by a shift. */
+ * triangular with a non-negative diagonal (geqrf's R up to row signs).  A
+ * column in the span of the columns before it (a Cholesky pivot that is not
+ * positive) gets a zero row of R and a pseudo-random orthonormal completion
+ * of Q, as Householder QR gives an arbitrary one.  *shifted (optional)
+ * reports that the factorization broke down: a dependent column was
+ * completed, or a failed pivot was repaired by a shift. */
 int bpf_householder(bpf_ctx ctx, int n, int p, double *d_Q, int ldq, double *d_R, int ldr,
                     int *shifted);
 /* Block-Hessenberg least squares of block GMRES, column by column on the

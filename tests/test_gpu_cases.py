@@ -31,11 +31,11 @@ def test_benchmark_flow_matches_cpu_restatement(bpf, oracle, name, precond):
     # the right-hand-side block has full rank (case9) the GPU and CPU runs
     # take the same iterations to rounding.  The Gu blocks of case57 and
     # case118 are rank-deficient (condition numbers ~1e18): Householder QR
-    # completes V_1 with arbitrary orthonormal directions, while CholQR2's
-    # Cholesky breaks down (and is shifted), which leaves the residual
-    # estimate unreliable; the library then judges convergence by the true
-    # residual and restarts from it.  What must hold there: a run reported
-    # solved has a true residual at the tolerance and the direct solution.
+    # completes V_1 with arbitrary orthonormal directions, CholQR2 with
+    # pseudo-random ones projected off the basis, so the iterates differ;
+    # after such a breakdown the library judges convergence by the true
+    # residual.  What must hold there: a run reported solved has a true
+    # residual at the tolerance and the direct solution.
     Ah, Bh = _case(bpf, name)
     full_rank = np.linalg.matrix_rank(Bh) == Bh.shape[1]
     A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
@@ -96,7 +96,7 @@ def test_benchmark_flow_many_rhs(bpf, oracle, name):
 
 
 def test_breakdown_rule_is_the_same_overlapped_or_not(bpf):
-    # case57's rank-deficient right-hand sides make CholQR2 break down; the
+    # case57's rank-deficient right-hand sides make CholQR2 deflate; the
     # true-residual rule that follows must see the same breakdowns whether
     # the Hessenberg updates overlap the next iteration (which queues one
     # iteration it may drop) or not: bitwise the same result.

@@ -36,10 +36,22 @@ def test_householder_ill_conditioned_and_rank_deficient(bpf):
     B = A.copy()
     B[:, 5] = B[:, 3]                                           # exactly rank deficient
     Qd, Rd = bpf.householder(bpf.HipMatrix.from_numpy(B))
-    assert Qd.shifted
+    assert Qd.shifted                                           # reported as a breakdown
     Q, R = Qd.to_numpy(), Rd.to_numpy()
     assert np.all(np.isfinite(Q)) and np.all(np.isfinite(R))
     assert np.linalg.norm(Q @ R - B) <= 1e-6 * np.linalg.norm(B)
+    # exactly dependent columns (pivots at rounding level): Householder QR
+    # completes Q with orthonormal directions and R has zero diagonal entries
+    rng = np.random.default_rng(6)
+    C = rng.standard_normal((n, p))
+    C[:, 7] = 0.0                                               # a zero column
+    C[:, 9] = 2.0 * C[:, 2] - C[:, 4]                           # a combination
+    Qd, Rd = bpf.householder(bpf.HipMatrix.from_numpy(C))
+    Q, R = Qd.to_numpy(), Rd.to_numpy()
+    assert Qd.shifted
+    assert np.linalg.norm(Q @ R - C) <= 1e-12 * np.linalg.norm(C)
+    assert R[7, 7] == 0.0
+    assert np.all(np.isfinite(Q))
 
 
 def _hessenberg(rng, p, K, triangular_sub):
