@@ -116,3 +116,15 @@ def test_write_read_round_trip(tmp_path):
     np.testing.assert_array_equal(A2.indptr, A.indptr)
     np.testing.assert_array_equal(A2.indices, A.indices)
     np.testing.assert_array_equal(A2.data, A.data)
+
+
+def test_empty_and_crlf(tmp_path):
+    f = tmp_path / "empty.mtx"
+    f.write_text("%%MatrixMarket matrix coordinate real general\n3 2 0\n")
+    A = mm.mmread(f)
+    assert A.shape == (3, 2) and A.nnz == 0
+    np.testing.assert_array_equal(A.indptr, [0, 0, 0, 0])
+    np.testing.assert_array_equal(mm.mmread_dense(f), np.zeros((3, 2)))
+    f = tmp_path / "crlf.mtx"
+    f.write_bytes(b"%%MatrixMarket matrix coordinate real general\r\n% c\r\n2 2 2\r\n1 1 3.5\r\n2 2 -1e-3\r\n")
+    np.testing.assert_array_equal(mm.mmread_dense(f), [[3.5, 0], [0, -1e-3]])
