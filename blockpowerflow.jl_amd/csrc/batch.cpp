@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <string>
 #include <thread>
 
 #include "krylov.h"
@@ -431,6 +432,7 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
     } else {
       std::atomic<int> next{0};
       std::vector<std::exception_ptr> err(nthr);
+      std::vector<std::string> msg(nthr);
       std::vector<std::thread> pool;
       for (int t = 0; t < nthr; ++t)
         pool.emplace_back([&, t] {
@@ -439,11 +441,15 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
             for (int i; (i = next.fetch_add(1)) < nbin;) run(i);
           } catch (...) {
             err[t] = std::current_exception();
+            msg[t] = bpf_last_error();   // the message is per thread
           }
         });
       for (auto &th : pool) th.join();
-      for (auto &e : err)
-        if (e) std::rethrow_exception(e);
+      for (int t = 0; t < nthr; ++t)
+        if (err[t]) {
+          set_error("%s", msg[t].c_str());
+          std::rethrow_exception(err[t]);
+        }
     }
     for (const BinResult &r : res) {
       st.niter += r.iters;

@@ -162,3 +162,4 @@ def test_batch_errors(bpf, gen):
         batch.solve([bpf.HipMatrix.from_numpy(np.zeros((S.n, 3)))])
     with pytest.raises(bpf.BpfError):
         batch.solve([bpf.HipMatrix.from_numpy(S.B)], precision="mixed32")
+
