@@ -877,7 +877,11 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
     prof::end(tok, s, prof::FACTOR, bytes, 0.0);
     return;
   }
-  const auto plan = make_plan(lower, 32);
+  // Levels of at most 8 rows go to the single-workgroup tail kernel (one
+  // round of its 16 wavefronts); wider ones get a launch each.  On the
+  // 70,000-bus LU pattern 8 rows beat 32 (the 17-32-row levels took two
+  // rounds there): 206 -> 197 ms of factorization per 32 value sets.
+  const auto plan = make_plan(lower, 8);
   for (const auto &seg : plan) {
     if (seg.wide) {
       const size_t shm = per_wave * wpb;
