@@ -21,7 +21,6 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-import warnings
 
 import numpy as np
 
@@ -320,13 +319,13 @@ class _Preconditioner:
 
 class ILU0(_Preconditioner):
     """ILU(0) (NVIDIA_ILU0 / AMD_ILU0, src/preconditioners_nvidia.jl:37-47,
-    src/preconditioners_amd.jl:110-117)."""
+    src/preconditioners_amd.jl:39-46)."""
     kind = _lib.BPF_PREC_ILU0
 
 
 class IC0(_Preconditioner):
     """IC(0) (NVIDIA_IC0 / AMD_IC0, src/preconditioners_nvidia.jl:1-11,
-    src/preconditioners_amd.jl:72-80)."""
+    src/preconditioners_amd.jl:1-9)."""
     kind = _lib.BPF_PREC_IC0
 
 
@@ -350,16 +349,17 @@ def ldiv_(y, P, x=None):
         return CUSOLVERRF._ldiv(y, P, x)
     if y.dtype != x.dtype:
         raise TypeError("x and y must have the same element type")
-    if np.dtype(x.dtype) != np.float64:
-        raise TypeError("column-major preconditioner apply takes float64 blocks")
-    check(L.bpf_prec_apply(P.handle, _lib.BPF_F64, _lib.BPF_COL_MAJOR, x.ncols, x.ptr, x.nrows,
+    dt = _block_dtype(x)
+    if y.shape != x.shape or x.nrows != P.n:
+        raise ValueError(f"blocks of size {x.shape} / {y.shape} for a preconditioner of size {P.n}")
+    check(L.bpf_prec_apply(P.handle, _dtype_code(dt), _lib.BPF_COL_MAJOR, x.ncols, x.ptr, x.nrows,
                            y.ptr, y.nrows))
     return y
 
 
 # ---------------------------------------------------------------- stats
 class BlockGMRESStats:
-    """BlockGMRESStats (src/block_utils.jl:361-367) plus backward-error fields."""
+    """BlockGMRESStats (src/block_utils.jl:62-68) plus backward-error fields."""
 
     def __init__(self):
         self.niter = 0
@@ -432,7 +432,7 @@ class BlockGMRESSolver:
         n, p = A.n, B.ncols
         self.m, self.n, self.p = n, n, p
         self.memory = max(1, min(n // max(p, 1), memory))
-        self.X = HipMatrix((n, p), np.float64, A.ctx)
+        self.X = HipMatrix((n, p), _block_dtype(B), A.ctx)
         self.stats = BlockGMRESStats()
         self.warm_start = False
         self._X0 = None
@@ -442,18 +442,45 @@ class BlockGMRESSolver:
         field typo Δx/ΔX is not reproduced)."""
         if X0.shape != self.X.shape:
             raise ValueError(f"X0 should have size ({self.n}, {self.p})")
+        if X0.dtype != self.X.dtype:
+            raise TypeError(f"X0 has element type {X0.dtype}, the solver {self.X.dtype}")
         self._X0 = X0
         self.warm_start = True
         return self
 
 
-def _check_system(A, B):
+def _block_dtype(B) -> np.dtype:
+    """Element type of a right-hand-side block: Float64 or Float32 (the
+    reference's solvers are generic over it, src/block_gmres.jl:170-173);
+    anything else raises before any native call."""
+    dt = np.dtype(B.dtype)
+    if dt not in (np.float64, np.float32):
+        raise TypeError(f"element type {dt} is not supported (float64 or float32)")
+    return dt
+
+
+def _check_system(A, B, *others):
+    """Shape and element-type checks of block_gmres! / block_bicgstab!
+    (src/block_gmres.jl:180-191): A square, B with n rows, and every other
+    block (X, X0, C) of B's shape and element type."""
     if A.shape[0] != A.shape[1]:
         raise ValueError("System must be square")
     if A.n != B.nrows:
         raise ValueError("Inconsistent problem size")
-    if B.dtype != np.float64:
-        warnings.warn(f"eltype(B) = {B.dtype} differs from eltype(A) = float64; converting.")
+    dt = _block_dtype(B)
+    for M in others:
+        if M is None:
+            continue
+        if M.shape != B.shape:
+            raise ValueError(f"block of size {M.shape} differs from B's {B.shape}")
+        if np.dtype(M.dtype) != dt:
+            raise TypeError(f"block of element type {M.dtype} differs from B's {dt}")
+    return dt
+
+
+def _default_tol(dt) -> float:
+    """sqrt(eps(T)), the reference's default atol and rtol."""
+    return math.sqrt(float(np.finfo(dt).eps))
 
 
 def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, X0=None, *, M=None,
@@ -462,11 +489,15 @@ def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, 
                  precision="fp64", block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0,
                  bmgs_blocks=1):
     """block_gmres!(solver, A, B[, X0]; kwargs...) -> solver (src/block_gmres.jl:156-411)."""
-    _check_system(A, B)
-    if M is not None:
-        raise NotImplementedError("left preconditioning (M) is not part of the multi-RHS path")
-    if N is not None and not ldiv:
-        raise TypeError("N must be applied with ldiv=true (an ILU0/IC0 object defines ldiv! only)")
+    dt = _check_system(A, B, solver.X, X0, solver._X0 if solver.warm_start else None)
+    tol = _default_tol(dt)
+    atol = tol if atol is None else atol
+    rtol = tol if rtol is None else rtol
+    for name, P in (("M", M), ("N", N)):
+        if P is not None and not isinstance(P, _Preconditioner):
+            raise TypeError(f"{name} must be a preconditioner object of this package")
+        if P is not None and P.n != A.n:
+            raise ValueError(f"{name} has size {P.n}, the system {A.n}")
     if X0 is not None:
         solver.warm_start_(X0)
     prm = _params(solver.memory, restart, reorthogonalization, atol, rtol, itmax, timemax, verbose,
@@ -476,11 +507,15 @@ def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, 
     cap = 1 + (itmax if itmax > 0 else 2 * (A.n // max(1, B.ncols))) + 16
     hist = np.zeros(cap)
     x0 = solver._X0 if solver.warm_start else None
+    # mulorldiv!(y, P, x, ldiv) (src/block_utils.jl:113): ldiv! and mul! of a
+    # preconditioner object both apply its inverse approximation here, so
+    # ldiv only matters to the Julia caller's dispatch.
+    Mh = M.handle if M is not None else None
     Nh = N.handle if N is not None else None
-    check(L.bpf_block_gmres_x0(A.ctx.handle, A.handle, Nh, B.ncols, B.ptr, B.nrows,
-                               x0.ptr if x0 is not None else None, x0.nrows if x0 is not None else 0,
-                               solver.X.ptr, solver.X.nrows, C.byref(prm), C.byref(st),
-                               hist.ctypes.data_as(_lib.dp), cap))
+    check(L.bpf_block_gmres_ex(A.ctx.handle, A.handle, Mh, Nh, _dtype_code(dt), B.ncols, B.ptr, B.nrows,
+                              x0.ptr if x0 is not None else None, x0.nrows if x0 is not None else 0,
+                              solver.X.ptr, solver.X.nrows, C.byref(prm), C.byref(st),
+                              hist.ctypes.data_as(_lib.dp), cap))
     solver.warm_start = False
     solver._X0 = None
     _fill_stats(solver.stats, st, hist, history)
@@ -505,7 +540,7 @@ class BlockBicgstabSolver:
     """Workspace of block_bicgstab! (src/block_bicgstab.jl:25-58)."""
 
     def __init__(self, A, B):
-        self.X = HipMatrix((A.n, B.ncols), np.float64, A.ctx)
+        self.X = HipMatrix((A.n, B.ncols), _block_dtype(B), A.ctx)
         self.stats = SimpleStats()
 
 
@@ -514,17 +549,21 @@ def block_bicgstab_(solver, A, B, *, C=None, N=None, atol=None, rtol=None, itmax
     """block_bicgstab!(solver, A, B; C=B, N, ...) -> (X, stats)
     (src/block_bicgstab.jl:187-299).  N (an ILU0/IC0 object) is applied as
     mul!(Y, N, P), i.e. as its inverse."""
-    _check_system(A, B)
+    dt = _check_system(A, B, solver.X, C)
+    tol = _default_tol(dt)
+    atol = tol if atol is None else atol
+    rtol = tol if rtol is None else rtol
     prm = _params(atol=atol, rtol=rtol, itmax=itmax, timemax=timemax, verbose=verbose, history=history)
     st = _lib.SolverStats()
     cap = 1 + (itmax if itmax > 0 else 2 * A.n) + 16
     cap = min(cap, 1 << 20)
     hist = np.zeros(cap)
     Ch = C.ptr if C is not None else None
-    check(L.bpf_block_bicgstab(A.ctx.handle, A.handle, N.handle if N is not None else None, B.ncols,
-                               B.ptr, B.nrows, Ch, C.nrows if C is not None else 0, solver.X.ptr,
-                               solver.X.nrows, _ctypes_byref(prm), _ctypes_byref(st),
-                               hist.ctypes.data_as(_lib.dp), cap))
+    check(L.bpf_block_bicgstab_t(A.ctx.handle, A.handle, N.handle if N is not None else None,
+                                 _dtype_code(dt), B.ncols, B.ptr, B.nrows, Ch,
+                                 C.nrows if C is not None else 0, solver.X.ptr, solver.X.nrows,
+                                 _ctypes_byref(prm), _ctypes_byref(st), hist.ctypes.data_as(_lib.dp),
+                                 cap))
     _fill_stats(solver.stats, st, hist, history)
     return solver.X, solver.stats
 

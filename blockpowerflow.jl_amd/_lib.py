@@ -91,6 +91,8 @@ SIGNATURES = {
     "bpf_block_gmres": [vp, vp, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_block_gmres_x0": [vp, vp, vp, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_block_bicgstab": [vp, vp, vp, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
+    "bpf_block_gmres_ex": [vp, vp, vp, vp, i, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
+    "bpf_block_bicgstab_t": [vp, vp, vp, i, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_backward_error": [vp, vp, i, vp, i, vp, i, dp],
     "bpf_hessenberg_eigenvalues": [dp, i, dp, dp],
     "bpf_householder": [vp, i, i, vp, i, vp, i, ip],

@@ -95,6 +95,9 @@ def block_gmres_batched(As, Bs, *, precision="mixed", **kw):
     (n_i x p_i).  Keywords as block_gmres (memory, restart, atol, rtol, itmax,
     timemax, berr_tol, precision "fp64" | "mixed", reorthogonalization,
     poly_degree)."""
+    for i, B in enumerate(Bs):
+        if np.dtype(B.dtype) != np.float64:
+            raise TypeError(f"system {i}: the batched solve takes float64 blocks, not {B.dtype}")
     batch = BatchedSystems(As, [B.shape[1] for B in Bs], ctx=Bs[0].ctx)
     try:
         return batch.solve(Bs, precision=precision, **kw)

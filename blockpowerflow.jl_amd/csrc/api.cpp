@@ -6,6 +6,7 @@
 #include <numeric>
 
 #include "bpf_internal.h"
+#include "dense.h"
 
 namespace bpf {
 
@@ -490,30 +491,38 @@ int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p, const void *d_X, in
     if (p <= 0 || P->n == 0) return;
     hipStream_t s = P->ctx->stream;
     const bool f32 = dtype == BPF_F32;
-    const size_t es = f32 ? sizeof(float) : sizeof(double);
     if (layout == BPF_ROW_MAJOR) {
       BPF_REQUIRE(ldx >= p && ldy >= p, "leading dimension < p");
-      if (f32) {
-        trisolve<float>(s, P->lo, p, static_cast<const float *>(d_X), ldx, static_cast<float *>(d_Y), ldy);
-        trisolve<float>(s, P->up, p, static_cast<const float *>(d_Y), ldy, static_cast<float *>(d_Y), ldy);
-      } else {
-        trisolve<double>(s, P->lo, p, static_cast<const double *>(d_X), ldx, static_cast<double *>(d_Y), ldy);
-        trisolve<double>(s, P->up, p, static_cast<const double *>(d_Y), ldy, static_cast<double *>(d_Y), ldy);
-      }
+      if (f32)
+        prec_apply_rm<float>(s, P, p, static_cast<const float *>(d_X), ldx, static_cast<float *>(d_Y), ldy);
+      else
+        prec_apply_rm<double>(s, P, p, static_cast<const double *>(d_X), ldx, static_cast<double *>(d_Y), ldy);
     } else if (layout == BPF_COL_MAJOR) {
-      BPF_REQUIRE(!f32, "column-major apply supports float64 blocks");
       BPF_REQUIRE(ldx >= P->n && ldy >= P->n, "leading dimension < n");
-      P->tmp.ensure((size_t)P->n * p * es);
-      launch_cm_to_rm(s, P->n, p, static_cast<const double *>(d_X), ldx, P->tmp.ptr, p, false);
+      const int n = P->n;
+      P->tmp.ensure((size_t)n * p * sizeof(double));
       double *t = P->tmp.as<double>();
-      trisolve<double>(s, P->lo, p, t, p, t, p);
-      trisolve<double>(s, P->up, p, t, p, t, p);
-      launch_rm_to_cm(s, P->n, p, t, p, false, static_cast<double *>(d_Y), ldy);
+      // Float32 blocks are applied in FP64: widened into a column-major copy,
+      // the result rounded back.
+      DevBuf w(&P->ctx->pool);
+      const double *src = static_cast<const double *>(d_X);
+      if (f32) {
+        w.ensure((size_t)n * p * sizeof(double));
+        copy_cast<float, double>(s, p, n, static_cast<const float *>(d_X), ldx, w.as<double>(), n, 1.0);
+        src = w.as<double>();
+      }
+      launch_cm_to_rm(s, n, p, src, f32 ? n : ldx, t, p, false);
+      prec_apply_rm<double>(s, P, p, t, p, t, p);
+      if (f32) {
+        launch_rm_to_cm(s, n, p, t, p, false, w.as<double>(), n);
+        copy_cast<double, float>(s, p, n, w.as<double>(), n, static_cast<float *>(d_Y), ldy, 1.0);
+      } else {
+        launch_rm_to_cm(s, n, p, t, p, false, static_cast<double *>(d_Y), ldy);
+      }
     } else {
       BPF_THROW(BPF_E_INVALID, "bad layout");
     }
-    triop_check(P->lo, s);
-    triop_check(P->up, s);
+    prec_check(P, s);
   });
 }
 

@@ -69,8 +69,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
       launch_block_gemm<double, double>(s, ga, S, 1);
     };
     auto precond = [&](const double *in, double *out) {
-      trisolve<double>(s, N->lo, p, in, p, out, p);
-      trisolve<double>(s, N->up, p, out, p, out, p);
+      prec_apply_rm<double>(s, N, p, in, p, out, p);
     };
     int nh = 0;
     auto push = [&](double v) {
@@ -126,8 +125,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     launch_rm_to_cm(s, n, p, X.ptr, p, false, d_X, ldx);
     BPF_CHECK_HIP(hipStreamSynchronize(s));
     if (N) {
-      triop_check(N->lo, s);
-      triop_check(N->up, s);
+      prec_check(N, s);
     }
     bpf_solver_stats st{};
     st.niter = iter;
@@ -138,5 +136,40 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     st.timer = now_s() - t0;
     st.max_berr = NAN;
     if (stats) *stats = st;
+  });
+}
+
+extern "C" int bpf_block_bicgstab_t(bpf_ctx ctx, bpf_csr A, bpf_prec N, int dtype, int p, const void *d_B,
+                                    int ldb, const void *d_C, int ldc, void *d_X, int ldx,
+                                    const bpf_solver_params *prm, bpf_solver_stats *stats,
+                                    double *h_hist, int hist_cap) {
+  if (dtype == BPF_F64)
+    return bpf_block_bicgstab(ctx, A, N, p, static_cast<const double *>(d_B), ldb,
+                              static_cast<const double *>(d_C), ldc, static_cast<double *>(d_X), ldx, prm, stats,
+                              h_hist, hist_cap);
+  return guard([&] {
+    BPF_REQUIRE(dtype == BPF_F32, "bad dtype %d", dtype);
+    BPF_REQUIRE(ctx && A && prm, "null argument");
+    BPF_REQUIRE(p > 0 && d_B && d_X, "empty block");
+    BPF_REQUIRE(ldb >= A->n && ldx >= A->n, "leading dimension < n");
+    if (d_C) BPF_REQUIRE(ldc >= A->n, "leading dimension of C < n");
+    // Float32 blocks: widened to FP64 column-major copies, solved in FP64,
+    // X rounded back to Float32.
+    const int n = A->n;
+    hipStream_t s = ctx->stream;
+    DevBuf B(&ctx->pool), Cw(&ctx->pool), X(&ctx->pool);
+    const size_t mb = (size_t)std::max(1, n) * p * sizeof(double);
+    B.ensure(mb);
+    X.ensure(mb);
+    copy_cast<float, double>(s, p, n, static_cast<const float *>(d_B), ldb, B.as<double>(), n, 1.0);
+    if (d_C) {
+      Cw.ensure(mb);
+      copy_cast<float, double>(s, p, n, static_cast<const float *>(d_C), ldc, Cw.as<double>(), n, 1.0);
+    }
+    const int rc = bpf_block_bicgstab(ctx, A, N, p, B.as<double>(), n, d_C ? Cw.as<double>() : nullptr, n,
+                                      X.as<double>(), n, prm, stats, h_hist, hist_cap);
+    if (rc != BPF_OK) throw Error{rc};
+    copy_cast<double, float>(s, p, n, X.as<double>(), n, static_cast<float *>(d_X), ldx, 1.0);
+    BPF_CHECK_HIP(hipStreamSynchronize(s));
   });
 }

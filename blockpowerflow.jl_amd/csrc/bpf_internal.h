@@ -228,6 +228,14 @@ struct bpf_prec_s {
   int max_row_nnz = 0;
 };
 
+namespace bpf {
+// y = P^{-1} x for a row-major n x p block (y may alias x), whatever the
+// kind of the handle; prec_check reports a failed sync-free wait.
+template <typename TV>
+void prec_apply_rm(hipStream_t s, bpf_prec P, int p, const TV *x, int ldx, TV *y, int ldy);
+void prec_check(bpf_prec P, hipStream_t s);
+}  // namespace bpf
+
 struct bpf_rf_s;
 struct bpf_batch_s;
 

@@ -310,8 +310,7 @@ void BlockArnoldi::form(double *X, double alpha) {
     ga.rows_sum = rows_sum_;
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     TV *P = P_.as<TV>();
-    trisolve<TV>(st_, N_->lo, p, P, p, P, p);
-    trisolve<TV>(st_, N_->up, p, P, p, P, p);
+    prec_apply_rm<TV>(st_, N_, p, P, p, P, p);
     Xtmp_.ensure((size_t)n * p * sizeof(double));
     copy_cast<TV, double>(st_, n, p, P, p, Xtmp_.as<double>(), p, 1.0);
     axpby(st_, (size_t)n * p, alpha, Xtmp_.as<double>(), 1.0, X);
@@ -391,20 +390,20 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
         if (flex && poly_ && poly_->degree() > 0) {
           poly_->apply<float>(st_, A_, N_, n, p, vf, zf);
         } else {
-          trisolve<float>(st_, N_->lo, p, vf, p, zf, p);
-          trisolve<float>(st_, N_->up, p, zf, p, zf, p);
+          prec_apply_rm<float>(st_, N_, p, vf, p, zf, p);
         }
         copy_cast<float, TV>(st_, n, p, zf, p, Pk, p, 1.0);
       } else if (flex && poly_ && poly_->degree() > 0) {
         poly_->apply<TV>(st_, A_, N_, n, p, Vk, Pk);
       } else {
-        trisolve<TV>(st_, N_->lo, p, Vk, p, Pk, p);
-        trisolve<TV>(st_, N_->up, p, Pk, p, Pk, p);
+        prec_apply_rm<TV>(st_, N_, p, Vk, p, Pk, p);
       }
     }
     // W = A P
     launch_spmm<TV>(st_, n, A_->nnz, A_->rowptr.as<int>(), A_->colidx.as<int>(),
                     f32 ? A_->val32.ptr : A_->val.ptr, f32, p, Pk, p, W_.as<TV>(), p, 1.0, 0.0);
+    // Q = M W with a left preconditioner (src/block_gmres.jl:300)
+    if (cfg_.M) prec_apply_rm<TV>(st_, cfg_.M, p, W_.as<TV>(), p, W_.as<TV>(), p);
     // Block Gram-Schmidt against V_0..V_k.
     double *col = col_region(k);
     orthogonalize<TV>(k, col);
@@ -462,10 +461,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   BPF_CHECK_HIP(hipStreamSynchronize(st_));
   shifted_ = h_fl_[k_] != 0;   // the flags after the last kept iteration
   form<TV>(X, 1.0);
-  if (N_) {
-    triop_check(N_->lo, st_);
-    triop_check(N_->up, st_);
-  }
+  if (N_) prec_check(N_, st_);
+  if (cfg_.M) prec_check(cfg_.M, st_);
   return it;
 }
 

@@ -76,6 +76,9 @@ struct ArnoldiConfig {
   int orth = 0;                // ORTH_BMGS / ORTH_BCGS / ORTH_BCGS2
   int bmgs_blocks = 1;         // BMGS: basis blocks projected together per step
   bool prec32 = false;         // apply the preconditioner in FP32 (FP64 basis)
+  // Left preconditioner (null = identity): W = M A N^{-1} V_k, and the run
+  // minimises ||M (B - A X)||_F (src/block_gmres.jl:226,300).
+  bpf_prec M = nullptr;
   // Explicit groups (row ranges, widths, slots) for many systems stacked in
   // one block-diagonal operator; empty = column slices of width s.
   std::vector<GroupDesc> groups;

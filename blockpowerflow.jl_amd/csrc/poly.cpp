@@ -178,8 +178,7 @@ void Poly::build(bpf_ctx ctx, bpf_csr A, bpf_prec N, int degree) {
   int steps = d;
   for (int j = 0; j < d; ++j) {
     double *vj = Vp + (size_t)j * n;
-    trisolve<double>(s, N->lo, 1, vj, 1, w, 1);
-    trisolve<double>(s, N->up, 1, w, 1, w, 1);
+    prec_apply_rm<double>(s, N, 1, vj, 1, w, 1);
     double *t = Vp + (size_t)(j + 1) * n;   // A M^{-1} v_j lands in v_{j+1}
     launch_spmm<double>(s, n, A->nnz, A->rowptr.as<int>(), A->colidx.as<int>(), A->val.ptr, false, 1, w,
                         1, t, 1, 1.0, 0.0);
@@ -278,8 +277,7 @@ void Poly::apply(hipStream_t s, bpf_csr A, bpf_prec N, int n, int p, const TV *V
                     alpha, beta);
   };
   auto minv = [&](const TV *x, TV *y) {
-    trisolve<TV>(s, N->lo, p, x, p, y, p);
-    trisolve<TV>(s, N->up, p, y, p, y, p);
+    prec_apply_rm<TV>(s, N, p, x, p, y, p);
   };
   const int d = (int)roots.size();
   bool first = true;

@@ -88,7 +88,10 @@ const char *status_str(int st) {
 // Block GMRES (src/block_gmres.jl:170-411) with an FP64 basis on the device.
 // mixed = false: everything FP64 (the drop-in).  mixed = true: flexible form
 // with the preconditioner (ILU(0) solves, optional GMRES polynomial) in FP32.
-void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, int ldb,
+// M (may be null) is the left preconditioner: the Krylov run sees M (B - A X)
+// and M A N^{-1} (:226,:270,:300), and without a backward-error target it
+// stops on ||M (B - A X)||_F as the reference does.
+void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const double *dB, int ldb,
                  const double *dX0, int ldx0, double *dX, int ldx, const bpf_solver_params &prm,
                  bpf_solver_stats &st, double *h_hist, int hist_cap, bool mixed = false) {
   const double t0 = now_s();
@@ -107,7 +110,19 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   }
   HistoryWriter hist{h_hist, hist_cap};
   std::vector<double> rn, sc, bnorm, xnorm;
-  col_norms2(ctx, n, p, blk.R.as<double>(), blk.stat, rn);
+  // R0 = M (B - A X) with a left preconditioner; blk.R stays the true residual.
+  DevBuf MR(&ctx->pool);
+  double *R0 = blk.R.as<double>();
+  auto left = [&]() {
+    if (!M) return;
+    prec_apply_rm<double>(s, M, p, blk.R.as<double>(), p, R0, p);
+  };
+  if (M) {
+    MR.ensure((size_t)n * p * sizeof(double));
+    R0 = MR.as<double>();
+  }
+  left();
+  col_norms2(ctx, n, p, R0, blk.stat, rn);
   double RNorm = 0.0;
   for (double v : rn) RNorm += v * v;
   RNorm = std::sqrt(RNorm);
@@ -133,6 +148,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   ArnoldiConfig cfg;
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = mixed;
   cfg.prec32 = mixed;
+  cfg.M = M;
   // Block MGS as in the reference (src/block_gmres.jl:301-304); with
   // reorthogonalization=true two classical passes.  (A single classical pass
   // loses orthogonality on these Jacobians and stalls the solve.)
@@ -148,7 +164,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   int cycles = 0;
   while (!(solved || tired || overtimed)) {
     ++cycles;
-    scale_rhs(ctx, n, p, blk.R.as<double>(), blk, rn, sc);
+    scale_rhs(ctx, n, p, R0, blk, rn, sc);
     const int maxit = prm.restart ? std::min(mem, itmax - iter) : itmax - iter;
     // Per-column targets: peek at the solution once the residual has dropped,
     // to turn the backward-error tolerance into a residual target.
@@ -183,10 +199,11 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
       if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
       return ok || overtimed;
     };
-    const int its = arn.run(blk.R.as<double>(), blk.X.as<double>(), maxit, stop, nullptr);
+    const int its = arn.run(R0, blk.X.as<double>(), maxit, stop, nullptr);
     iter += its;
     residual_rm(s, A, p, blk.B.as<double>(), blk.X.as<double>(), blk.R.as<double>());
-    col_norms2(ctx, n, p, blk.R.as<double>(), blk.stat, rn);
+    left();
+    col_norms2(ctx, n, p, R0, blk.stat, rn);
     // With a backward-error target the true residual decides; otherwise the
     // reference's rule: the residual estimate ||Lambda||_F <= eps (:349).
     // After a breakdown in CholQR2 (a rank-deficient block, as with linearly
@@ -341,23 +358,8 @@ int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *
                        const double *d_X0, int ldx0, double *d_X, int ldx,
                        const bpf_solver_params *prm, bpf_solver_stats *stats, double *h_history,
                        int hist_cap) {
-  return guard([&] {
-    check_common(ctx, A, N, p, d_B, ldb, d_X, ldx, prm);
-    if (d_X0) BPF_REQUIRE(ldx0 >= A->n, "leading dimension of X0 < n");
-    bpf_solver_stats st{};
-    if (A->n == 0) {
-      st.solved = 1;
-      st.status = BPF_STATUS_SOLVED;
-    } else if (prm->precision == BPF_PRECISION_MIXED) {
-      gmres_plain(ctx, A, N, p, d_B, ldb, d_X0, ldx0, d_X, ldx, *prm, st, h_history, hist_cap, true);
-    } else if (prm->precision == BPF_PRECISION_MIXED32) {
-      BPF_REQUIRE(N == nullptr || N->kind == BPF_PREC_ILU0 || N->kind == BPF_PREC_IC0, "bad preconditioner");
-      gmres_mixed(ctx, A, N, p, d_B, ldb, d_X0, ldx0, d_X, ldx, *prm, st, h_history, hist_cap);
-    } else {
-      gmres_plain(ctx, A, N, p, d_B, ldb, d_X0, ldx0, d_X, ldx, *prm, st, h_history, hist_cap);
-    }
-    if (stats) *stats = st;
-  });
+  return bpf_block_gmres_ex(ctx, A, nullptr, N, BPF_F64, p, d_B, ldb, d_X0, ldx0, d_X, ldx, prm, stats,
+                            h_history, hist_cap);
 }
 
 int bpf_block_gmres(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *d_B, int ldb,
@@ -365,6 +367,59 @@ int bpf_block_gmres(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *d_B
                     double *h_history, int hist_cap) {
   return bpf_block_gmres_x0(ctx, A, N, p, d_B, ldb, nullptr, 0, d_X, ldx, prm, stats, h_history,
                             hist_cap);
+}
+
+// Left and right preconditioning and the element type of the blocks (the
+// reference's solvers are generic over it, src/block_gmres.jl:170-173):
+// Float32 B and X0 are widened to FP64 column-major copies, the solve runs
+// in FP64 and X is rounded back to Float32.
+int bpf_block_gmres_ex(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int dtype, int p, const void *d_B,
+                       int ldb, const void *d_X0, int ldx0, void *d_X, int ldx, const bpf_solver_params *prm,
+                       bpf_solver_stats *stats, double *h_history, int hist_cap) {
+  return guard([&] {
+    BPF_REQUIRE(dtype == BPF_F64 || dtype == BPF_F32, "bad dtype %d", dtype);
+    check_common(ctx, A, N, p, d_B, ldb, d_X, ldx, prm);
+    if (M) BPF_REQUIRE(M->n == A->n, "left preconditioner size %d differs from n = %d", M->n, A->n);
+    if (d_X0) BPF_REQUIRE(ldx0 >= A->n, "leading dimension of X0 < n");
+    const int n = A->n;
+    hipStream_t s = ctx->stream;
+    const double *B = static_cast<const double *>(d_B), *X0 = static_cast<const double *>(d_X0);
+    double *X = static_cast<double *>(d_X);
+    int lb = ldb, lx0 = ldx0, lx = ldx;
+    DevBuf Bw(&ctx->pool), X0w(&ctx->pool), Xw(&ctx->pool);
+    if (dtype == BPF_F32) {
+      const size_t mb = (size_t)std::max(1, n) * p * sizeof(double);
+      Bw.ensure(mb);
+      Xw.ensure(mb);
+      copy_cast<float, double>(s, p, n, static_cast<const float *>(d_B), ldb, Bw.as<double>(), n, 1.0);
+      B = Bw.as<double>();
+      X = Xw.as<double>();
+      lb = lx = n;
+      if (d_X0) {
+        X0w.ensure(mb);
+        copy_cast<float, double>(s, p, n, static_cast<const float *>(d_X0), ldx0, X0w.as<double>(), n, 1.0);
+        X0 = X0w.as<double>();
+        lx0 = n;
+      }
+    }
+    bpf_solver_stats st{};
+    if (n == 0) {
+      st.solved = 1;
+      st.status = BPF_STATUS_SOLVED;
+    } else if (prm->precision == BPF_PRECISION_MIXED) {
+      gmres_plain(ctx, A, M, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap, true);
+    } else if (prm->precision == BPF_PRECISION_MIXED32) {
+      if (M) BPF_THROW(BPF_E_UNSUPPORTED, "left preconditioning is not available with precision MIXED32");
+      gmres_mixed(ctx, A, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap);
+    } else {
+      gmres_plain(ctx, A, M, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap);
+    }
+    if (dtype == BPF_F32 && n > 0) {
+      copy_cast<double, float>(s, p, n, X, n, static_cast<float *>(d_X), ldx, 1.0);
+      BPF_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    if (stats) *stats = st;
+  });
 }
 
 int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx, const double *d_B,

@@ -94,7 +94,7 @@ int bpf_timer_start(bpf_ctx ctx);
 int bpf_timer_stop(bpf_ctx ctx, double *ms);
 
 /* In-process kernel timing (no reference counterpart; the reference only
- * reports wall time, src/block_utils.jl:354-356).  When enabled, instrumented
+ * reports wall time, src/block_gmres.jl:408, src/block_utils.jl:57).  When enabled, instrumented
  * launches on any context are bracketed by HIP events on their stream and
  * summed per family with their algorithmic bytes and flops:
  *   0 SpMM, 1 triangular-solve level launch, 2 block projection (V^T W),
@@ -139,8 +139,8 @@ int bpf_prec_create(bpf_ctx ctx, bpf_csr A, int kind, bpf_prec *P);
 /* Numeric refactorization with A's current values (same pattern). */
 int bpf_prec_factor(bpf_prec P, bpf_csr A);
 /* y = P^{-1} x : ldiv!(y, ilu, x) (src/preconditioners_nvidia.jl:49-59,
- * src/preconditioners_amd.jl:119-131); ldiv!(y, ic, x) (nvidia :13-23,
- * amd :82-94).  Also what mul!(Y, N, P) means for block_bicgstab's N
+ * src/preconditioners_amd.jl:55-60); ldiv!(y, ic, x) (nvidia :13-23,
+ * amd :18-23).  Also what mul!(Y, N, P) means for block_bicgstab's N
  * (src/block_bicgstab.jl:251,263).  d_Y may alias d_X. */
 int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p,
                    const void *d_X, int ldx, void *d_Y, int ldy);
@@ -219,6 +219,24 @@ int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
                        const double *d_B, int ldb, const double *d_C, int ldc,
                        double *d_X, int ldx, const bpf_solver_params *prm,
                        bpf_solver_stats *stats, double *h_history, int hist_cap);
+/* block_gmres(A, B[, X0]; M, N, ldiv, ...) with both preconditioners
+ * (src/block_gmres.jl:127-154,194-203): M (may be NULL) is the left
+ * preconditioner, N (may be NULL) the right one; a preconditioner handle is
+ * applied as its inverse approximation, which is what mulorldiv! does with
+ * either value of ldiv (src/block_utils.jl:113).  With M the run stops on
+ * ||M (B - A X)||_F <= atol + rtol ||M B||_F (:226-230,:341-349).  dtype
+ * (BPF_F64 or BPF_F32) is the element type of B, X0 and X: the reference's
+ * solvers are generic over it (:170-173).  Float32 blocks are widened to
+ * FP64 on the device, the solve runs in FP64 and X is rounded back to
+ * Float32.  bpf_block_gmres_x0 is this call with M = NULL, dtype BPF_F64.
+ * bpf_block_bicgstab_t is block_bicgstab with the element type given the
+ * same way (src/block_bicgstab.jl:182-190). */
+int bpf_block_gmres_ex(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int dtype, int p, const void *d_B,
+                       int ldb, const void *d_X0, int ldx0, void *d_X, int ldx, const bpf_solver_params *prm,
+                       bpf_solver_stats *stats, double *h_history, int hist_cap);
+int bpf_block_bicgstab_t(bpf_ctx ctx, bpf_csr A, bpf_prec N, int dtype, int p, const void *d_B, int ldb,
+                         const void *d_C, int ldc, void *d_X, int ldx, const bpf_solver_params *prm,
+                         bpf_solver_stats *stats, double *h_history, int hist_cap);
 /* Thin QR of a device n x p block (n >= p, p <= 128), in place:
  * householder!(Q, R, tau) (src/block_utils.jl:38-53), the orthonormalization
  * of block_gmres (src/block_gmres.jl:276,316).  Computed by CholQR2 (two

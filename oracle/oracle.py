@@ -178,7 +178,7 @@ class Stats:
 
 
 def _householder(Q, compact=False):
-    """Q, R, tau = householder!(Q, R, tau; compact) -- src/block_utils.jl:345-352
+    """Q, R, tau = householder!(Q, R, tau; compact) -- src/block_utils.jl:46-53
     (LAPACK geqrf, copy of the upper triangle, orgqr unless compact)."""
     qr, tau, work, info = lapack.dgeqrf(np.asfortranarray(Q))
     k = Q.shape[1]
@@ -195,14 +195,16 @@ def _ormqr_t(H, tau, D):
     return out
 
 
-def block_gmres(A, B, *, memory=20, N=None, restart=False, atol=None, rtol=None,
+def block_gmres(A, B, *, memory=20, M=None, N=None, restart=False, atol=None, rtol=None,
                 itmax=0, history=False):
-    """Restatement of block_gmres!(solver, A, B; N, ldiv, restart, ...)
-    (src/block_gmres.jl:170-411) without left preconditioning.
+    """Restatement of block_gmres!(solver, A, B; M, N, ldiv, restart, ...)
+    (src/block_gmres.jl:170-411).
 
-    ``A`` is anything with ``A @ X``; ``N`` (right preconditioner) is ``None``
-    (identity) or a callable applying N^{-1} (the ``ldiv=true`` form,
-    src/block_utils.jl:412).  Returns ``(X, stats)``.
+    ``A`` is anything with ``A @ X``; ``M`` (left preconditioner) and ``N``
+    (right preconditioner) are ``None`` (identity) or callables applying the
+    preconditioner (what mulorldiv! does with the operator, src/block_utils.jl:113).
+    The residual it estimates and stops on is ||M (B - A X)||_F (:226-227,:341).
+    Returns ``(X, stats)``.
     """
     t0 = time.perf_counter()
     n, p = B.shape
@@ -213,6 +215,8 @@ def block_gmres(A, B, *, memory=20, N=None, restart=False, atol=None, rtol=None,
     stats = Stats()
     X = np.zeros((n, p), order="F")                              # :216
     W = np.array(B, dtype=np.float64, order="F")                 # :224
+    if M is not None:
+        W = np.asfortranarray(M(W))                              # :226
     RNorm = np.linalg.norm(W)                                    # :227
     if history:
         stats.residuals.append(RNorm)
@@ -230,6 +234,8 @@ def block_gmres(A, B, *, memory=20, N=None, restart=False, atol=None, rtol=None,
             Xr[:] = 0.0
             if npass >= 1:
                 W = B - A @ X                                     # :268-269
+                if M is not None:
+                    W = M(W)                                      # :270
         Vk, Z1, _ = _householder(W)                               # :275-276
         V.append(Vk)
         Z.append(Z1)
@@ -241,6 +247,8 @@ def block_gmres(A, B, *, memory=20, N=None, restart=False, atol=None, rtol=None,
             inner += 1
             P = V[inner - 1] if N is None else N(V[inner - 1])   # :297-298
             Q = np.asfortranarray(A @ P)                         # :299
+            if M is not None:
+                Q = np.asfortranarray(M(Q))                      # :300
             for i in range(inner):                               # :301-304
                 Psi = V[i].T @ Q
                 Q = Q - V[i] @ Psi
@@ -300,10 +308,13 @@ def block_gmres(A, B, *, memory=20, N=None, restart=False, atol=None, rtol=None,
     return X, stats
 
 
-def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, history=False):
+def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, history=False,
+                   callback=None):
     """Restatement of the CPU block_bicgstab! (src/block_bicgstab.jl:70-178).
     ``N`` is None or a callable applying N^{-1} (the reference calls
-    mul!(Y, N, P) with an operator that applies the inverse, :134)."""
+    mul!(Y, N, P) with an operator that applies the inverse, :134).
+    ``callback(iter, X, R)``, if given, sees the iterate and the recursively
+    updated residual after every iteration."""
     t0 = time.perf_counter()
     n, p = B.shape
     eps = np.finfo(np.float64).eps
@@ -342,6 +353,8 @@ def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, histo
         U = P - omega * V                                        # :151
         P = R - U @ beta                                         # :152-153
         it += 1
+        if callback is not None:
+            callback(it, X, R)
         rNorm = np.linalg.norm(R)                                # :156
         if history:
             stats.residuals.append(rNorm)

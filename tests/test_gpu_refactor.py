@@ -91,7 +91,7 @@ def test_rf_refined_newton_sequence(bpf, gen, oracle):
 
 
 def test_rf_batch(bpf, gen, oracle):
-    # test/cusolverRF.jl:108-143: batch of 32, RHS i*b -> i*(J\b)
+    # test/cusolverRF.jl:34-68: batch of 32, RHS i*b -> i*(J\b)
     _, S = _sys(gen, 300, 1, 46)
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
     nbatch = 32

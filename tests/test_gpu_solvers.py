@@ -186,3 +186,78 @@ def test_block_bicgstab_matches_cpu_restatement(bpf, gen, oracle, p):
     Xc, stc = oracle.block_bicgstab(S.scipy(), S.B, N=Nf, atol=0.0, rtol=1e-8, itmax=200, history=True)
     k = min(len(st.residuals), len(stc.residuals), 8)
     np.testing.assert_allclose(st.residuals[:k], stc.residuals[:k], rtol=1e-6)
+
+
+# ------------------------------------------------ left preconditioning (M)
+@pytest.mark.parametrize("side", ["left", "both"])
+def test_block_gmres_left_preconditioner_matches_oracle(bpf, gen, oracle, side):
+    # block_gmres(A, B; M, N) (src/block_gmres.jl:194-203,226,300): the run
+    # estimates ||M (B - A X)||_F; the oracle restates the same recurrence, so
+    # the residual histories agree to rounding over the first iterations and
+    # both reach the direct solution.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 8, 41)
+    P = bpf.ilu0(A, 8)
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    apply = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    kw = dict(M=P) if side == "left" else dict(M=P, N=P)
+    okw = dict(M=apply) if side == "left" else dict(M=apply, N=apply)
+    X, st = bpf.block_gmres(A, B, memory=60, atol=0.0, rtol=1e-11, history=True, ldiv=True, **kw)
+    Xo, sto = oracle.block_gmres(S.scipy(), S.B, memory=60, atol=0.0, rtol=1e-11, history=True, **okw)
+    assert st.solved and sto.solved
+    k = min(6, len(sto.residuals))
+    np.testing.assert_allclose(st.residuals[:k], sto.residuals[:k], rtol=1e-7)
+    assert abs(st.niter - sto.niter) <= 1
+    # stopping rule on the M-weighted residual, as the reference
+    MR = apply(S.B - S.scipy() @ X.to_numpy())
+    assert np.linalg.norm(MR) <= 1e-11 * np.linalg.norm(apply(S.B)) * 10
+    assert np.all(_rel(X.to_numpy(), Xd) < 1e-6)
+
+
+def test_block_gmres_left_preconditioner_backward_error(bpf, gen, oracle):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 2000, 32, 42)
+    P = bpf.ilu0(A, 32)
+    X, st = bpf.block_gmres(A, B, M=P, memory=40, berr_tol=1e-10, itmax=500)
+    be = oracle.backward_error(S.indptr, S.indices, S.data, X.to_numpy(), S.B)
+    assert st.solved and be.max() <= 1e-10
+    assert np.all(_rel(X.to_numpy(), Xd) < 1e-5)
+
+
+# ------------------------------------------------------ Float32 blocks
+def test_block_gmres_float32_blocks(bpf, gen, oracle):
+    # The reference is generic over the element type of B (default tolerances
+    # sqrt(eps(Float32))); X comes back as Float32.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 16, 43)
+    B32 = bpf.HipMatrix.from_numpy(S.B.astype(np.float32))
+    P = bpf.ilu0(A, 16)
+    X, st = bpf.block_gmres(A, B32, N=P, ldiv=True, memory=30)
+    assert X.dtype == np.float32 and st.solved
+    Xh = X.to_numpy().astype(np.float64)
+    r = np.linalg.norm(S.B.astype(np.float32).astype(np.float64) - S.scipy() @ Xh)
+    eps32 = np.sqrt(np.finfo(np.float32).eps)
+    assert r <= 2 * (eps32 + eps32 * np.linalg.norm(S.B)) + 1e-6 * np.linalg.norm(S.B)
+    # warm start with a Float32 guess, and an explicit tolerance
+    X2, st2 = bpf.block_gmres(A, B32, X, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-6)
+    assert X2.dtype == np.float32 and st2.niter <= st.niter
+    with pytest.raises(TypeError):
+        bpf.block_gmres(A, B32, bpf.HipMatrix.from_numpy(np.zeros(S.B.shape)), N=P, ldiv=True)
+
+
+def test_block_bicgstab_float32_blocks(bpf, gen, oracle):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 118, 4, 44)
+    P = bpf.ilu0(A, 4)
+    X, st = bpf.block_bicgstab(A, bpf.HipMatrix.from_numpy(S.B.astype(np.float32)), N=P,
+                               atol=0.0, rtol=1e-6, itmax=200)
+    assert X.dtype == np.float32 and st.solved
+    assert np.all(_rel(X.to_numpy().astype(np.float64), Xd) < 1e-3)
+
+
+def test_ldiv_float32_block(bpf, gen, oracle):
+    S, A, B, Xd = _setup(bpf, gen, oracle, 300, 5, 45)
+    P = bpf.ilu0(A, 5)
+    x32 = bpf.HipMatrix.from_numpy(S.B.astype(np.float32))
+    y32 = bpf.HipMatrix((S.n, 5), np.float32)
+    bpf.ldiv_(y32, P, x32)
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    ref = oracle.ilu0_apply(S.indptr, S.indices, F, S.B.astype(np.float32).astype(np.float64))
+    np.testing.assert_allclose(y32.to_numpy(), ref.astype(np.float32), rtol=1e-5,
+                               atol=1e-5 * np.abs(ref).max())

@@ -400,6 +400,175 @@ def test_block_bicgstab_uses_abs_omega(oracle):
     np.testing.assert_allclose(X[:, 0], x, rtol=1e-12)
 
 
+def _textbook_bicgstab(A, b, N, iters, shadow=None):
+    """Right-preconditioned BiCGSTAB in van der Vorst's rho-ratio form
+    (van der Vorst 1992; Saad, Iterative Methods, Alg. 7.7): beta from
+    rho_{i+1}/rho_i * alpha/omega, omega = (t, s)/(t, t) without absolute
+    values.  Returns x, the residual norms and the omegas."""
+    rhat = b.copy() if shadow is None else shadow
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    rho = rhat @ r
+    res, omegas = [np.linalg.norm(r)], []
+    for _ in range(iters):
+        y = N(p)
+        v = A @ y
+        alpha = rho / (rhat @ v)
+        s = r - alpha * v
+        z = N(s)
+        t = A @ z
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * y + omega * z
+        r = s - omega * t
+        rho_new = rhat @ r
+        beta = (rho_new / rho) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        rho = rho_new
+        res.append(np.linalg.norm(r))
+        omegas.append(omega)
+    return x, np.array(res), np.array(omegas)
+
+
+def _pd_symmetric_part(n, seed):
+    # A + A^T positive definite: (A s, s) > 0 for every s, so omega > 0 and
+    # the reference's |omega| (src/block_bicgstab.jl:146) equals omega.
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, n, density=0.05, random_state=seed, format="csr")
+    M = M - M.T + sp.diags(4.0 + rng.random(n))
+    return sp.csr_matrix(M)
+
+
+def test_block_bicgstab_p1_is_textbook_bicgstab(oracle):
+    A = _pd_symmetric_part(120, 21)
+    b = np.random.default_rng(22).standard_normal((120, 1))
+    X, st = oracle.block_bicgstab(A, b, atol=0.0, rtol=1e-30, itmax=10, history=True)
+    x, res, om = _textbook_bicgstab(A, b[:, 0], lambda v: v, 10)
+    assert np.all(om > 0)
+    np.testing.assert_allclose(st.residuals, res, rtol=1e-8)
+    np.testing.assert_allclose(X[:, 0], x, rtol=1e-8, atol=1e-12 * np.abs(x).max())
+
+
+def test_block_bicgstab_p1_preconditioned_is_textbook(oracle, gen):
+    _, S = gen.make_system(118, 1, seed=23)
+    A = S.scipy()
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    N = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    X, st = oracle.block_bicgstab(A, S.B, N=N, atol=0.0, rtol=1e-30, itmax=6, history=True)
+    x, res, om = _textbook_bicgstab(A, S.B[:, 0], lambda v: N(v[:, None])[:, 0], 6)
+    assert np.all(om > 0)      # on these 6 steps |omega| = omega
+    np.testing.assert_allclose(st.residuals, res, rtol=1e-7)
+    np.testing.assert_allclose(X[:, 0], x, rtol=1e-7, atol=1e-10 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("p", [1, 8])
+def test_block_bicgstab_converges_to_direct_solution(oracle, p):
+    M = _rand_csr(300, 0.02, 24, diag=8.0)
+    B = np.random.default_rng(25).standard_normal((300, p))
+    X, st = oracle.block_bicgstab(M, B, atol=0.0, rtol=1e-12, itmax=300)
+    assert st.solved and st.status == "solution good enough given atol and rtol"
+    Xd = spla.splu(sp.csc_matrix(M)).solve(B)
+    np.testing.assert_allclose(X, Xd, rtol=1e-9, atol=1e-10 * np.abs(Xd).max())
+
+
+def test_block_bicgstab_recursive_residual_is_true_residual(oracle):
+    M = _rand_csr(200, 0.03, 26, diag=6.0)
+    B = np.random.default_rng(27).standard_normal((200, 4))
+    F = oracle.ilu0(M.indptr, M.indices, M.data)
+    N = lambda Z: oracle.ilu0_apply(M.indptr, M.indices, F, Z)
+    gaps = []
+
+    def cb(it, X, R):
+        gaps.append(np.linalg.norm(R - (B - M @ X)) / np.linalg.norm(B))
+
+    X, st = oracle.block_bicgstab(M, B, N=N, atol=0.0, rtol=1e-12, itmax=60, callback=cb)
+    assert st.solved and len(gaps) == st.niter
+    assert max(gaps) < 1e-12
+
+
+def test_block_bicgstab_shadow_block_C(oracle):
+    # C = B is the default shadow; another C gives another (valid) iteration
+    M = _rand_csr(150, 0.03, 28, diag=6.0)
+    B = np.random.default_rng(29).standard_normal((150, 3))
+    C = np.random.default_rng(30).standard_normal((150, 3))
+    X1, s1 = oracle.block_bicgstab(M, B, atol=0.0, rtol=1e-12, itmax=200)
+    X2, s2 = oracle.block_bicgstab(M, B, C=C, atol=0.0, rtol=1e-12, itmax=200, history=True)
+    X3, s3 = oracle.block_bicgstab(M, B, C=B, atol=0.0, rtol=1e-12, itmax=200)
+    assert s1.solved and s2.solved
+    np.testing.assert_array_equal(X1, X3)
+    np.testing.assert_allclose(X1, X2, rtol=1e-8, atol=1e-10 * np.abs(X1).max())
+    x, res, _ = _textbook_bicgstab(M, B[:, 0], lambda v: v, 3, shadow=C[:, 0])
+    Xs, ss = oracle.block_bicgstab(M, B[:, :1], C=C[:, :1], atol=0.0, rtol=1e-30, itmax=3, history=True)
+    np.testing.assert_allclose(ss.residuals, res, rtol=1e-8)
+
+
+# --------------------------------------------------- left preconditioning
+def _textbook_left_gmres(A, b, Minv, m):
+    """Left-preconditioned GMRES without restart (Saad, Alg. 9.4): Arnoldi on
+    M^{-1} A from r0 = M^{-1} b; returns ||M^{-1}(b - A x_k)|| per step."""
+    r0 = Minv(b)
+    beta = np.linalg.norm(r0)
+    n = len(b)
+    V = np.zeros((n, m + 1))
+    H = np.zeros((m + 1, m))
+    V[:, 0] = r0 / beta
+    res = [beta]
+    for j in range(m):
+        w = Minv(A @ V[:, j])
+        for i in range(j + 1):
+            H[i, j] = V[:, i] @ w
+            w = w - H[i, j] * V[:, i]
+        H[j + 1, j] = np.linalg.norm(w)
+        V[:, j + 1] = w / H[j + 1, j]
+        e = np.zeros(j + 2)
+        e[0] = beta
+        y = np.linalg.lstsq(H[:j + 2, :j + 1], e, rcond=None)[0]
+        res.append(np.linalg.norm(e - H[:j + 2, :j + 1] @ y))
+    return np.array(res)
+
+
+def test_block_gmres_p1_left_is_textbook(oracle, gen):
+    _, S = gen.make_system(118, 1, seed=31)
+    A = S.scipy()
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    Mv = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    X, st = oracle.block_gmres(A, S.B, memory=30, M=Mv, atol=0.0, rtol=1e-12, itmax=10, history=True)
+    ref = _textbook_left_gmres(A, S.B[:, 0], lambda v: Mv(v[:, None])[:, 0], len(st.residuals) - 1)
+    np.testing.assert_allclose(st.residuals, ref, rtol=1e-8)
+    # the estimate is the M-weighted true residual (not ||b - A x||)
+    true_m = np.linalg.norm(Mv(S.B - A @ X))
+    assert abs(st.residuals[-1] - true_m) <= 1e-9 * np.linalg.norm(Mv(S.B))
+
+
+def test_block_gmres_left_and_right_converge(oracle, gen):
+    _, S = gen.make_system(300, 6, seed=32)
+    A = S.scipy()
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    P = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    for kw in (dict(M=P), dict(M=P, N=P)):
+        X, st = oracle.block_gmres(A, S.B, memory=60, atol=0.0, rtol=1e-12, **kw)
+        assert st.solved
+        np.testing.assert_allclose(X, Xd, rtol=1e-7, atol=1e-8 * np.abs(Xd).max())
+
+
+def test_ic0_apply_order(oracle):
+    # y = L^{-T} (L^{-1} x): L first, then its transpose
+    # (src/preconditioners_nvidia.jl:19-22)
+    n = 60
+    W = sp.random(n, n, density=0.08, random_state=33)
+    W = W + W.T
+    A = sp.csr_matrix(sp.diags(np.asarray(abs(W).sum(1)).ravel() + 1.0) - W)
+    A.sort_indices()
+    F = oracle.ic0(A.indptr, A.indices, A.data)
+    L = np.tril(sp.csr_matrix((F, A.indices, A.indptr), shape=A.shape).toarray())
+    x = np.random.default_rng(34).standard_normal((n, 3))
+    ref = np.linalg.solve(L.T, np.linalg.solve(L, x))
+    np.testing.assert_allclose(oracle.ic0_apply(A.indptr, A.indices, F, x), ref, rtol=1e-10, atol=1e-12)
+    wrong = np.linalg.solve(L, np.linalg.solve(L.T, x))
+    assert np.abs(wrong - ref).max() > 1e-6
+
+
 # ---------------------------------------------------- direct solve, berr
 def test_direct_solve_and_backward_error(oracle):
     M = _rand_csr(40, 0.1, 14)
