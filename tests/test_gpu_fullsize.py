@@ -1,0 +1,159 @@
+"""The benchmark configurations at their full sizes against the FP64 oracle.
+
+c3: 70,000 buses (n = 125,998), 128 RHS (8 N(0,1) + 120 unit injections),
+    mixed precision flexible block GMRES as the benchmark runs it, and the
+    plain FP64 drop-in on the same input;
+c2: 9,000 buses, 64 N(0,1) RHS, block BiCGSTAB and block GMRES(30) with ILU(0);
+c4: the 256-system batch (n log-uniform in [500, 20000], p in {8,16,32,64});
+c5: the 70,000-bus pattern, 3 Newton value sets, 256 RHS, fixed-pattern
+    refactorization + FP32 solves + FP64 refinement.
+Tolerances: per-RHS backward error <= 1e-10 and per-column relative error
+<= 1e-5 against the sparse direct solve (the Jacobians' condition numbers
+are ~1e4-1e5), factor values to 1e-9 of the oracle's refactorization."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BERR = 1e-10
+
+
+def _rel(X, Xd):
+    return np.abs(X - Xd).max(axis=0) / np.abs(Xd).max(axis=0)
+
+
+@pytest.fixture(scope="module")
+def c3(gen, oracle):
+    grid, S = gen.make_system(70000, 128, seed=1000, rhs_seed=2000, rhs="injection", nunit=120)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    return grid, S, Xd
+
+
+@pytest.mark.parametrize("form", ["mixed", "fp64"])
+def test_c3_full_size(bpf, oracle, c3, form):
+    grid, S, Xd = c3
+    print(f"c3 {form}: n={S.n} nnz={S.nnz}", flush=True)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    P = bpf.ilu0(A, 128)
+    kw = dict(precision="mixed", poly_degree=4, bmgs_blocks=8, block_width=128) if form == "mixed" else {}
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, itmax=2000, timemax=120.0, berr_tol=BERR,
+                            max_refine=8, **kw)
+    Xh = X.to_numpy()
+    be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+    assert st.solved, st
+    assert be.max() <= BERR, be.max()
+    assert np.all(_rel(Xh, Xd) <= 1e-5), _rel(Xh, Xd).max()
+    # the library's per-RHS backward error agrees with the oracle's
+    np.testing.assert_allclose(bpf.backward_error(A, X, B), be, rtol=1e-6, atol=1e-15)
+
+
+@pytest.fixture(scope="module")
+def c2(gen, oracle):
+    grid, S = gen.make_system(9000, 64, seed=1000, rhs_seed=2000)
+    F = oracle.ilu0(S.indptr, S.indices, S.data)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    return S, F, Xd
+
+
+def test_c2_bicgstab_follows_reference_recurrence(bpf, oracle, c2):
+    # Block BiCGSTAB with ILU(0) (src/block_bicgstab.jl:182-299) on the c2
+    # system: the GPU iterates follow the CPU restatement over the first 8
+    # iterations and both runs end with the same status after 40.
+    S, F, Xd = c2
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    P = bpf.ilu0(A, 64)
+    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-8, itmax=40, history=True)
+    Nf = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    Xc, stc = oracle.block_bicgstab(S.scipy(), S.B, N=Nf, atol=0.0, rtol=1e-8, itmax=40, history=True)
+    print(f"c2 bicgstab: gpu {st.status} {st.niter} its, final {st.residuals[-1]:.3e}; "
+          f"cpu {stc.status} {stc.niter} its, final {stc.residuals[-1]:.3e}", flush=True)
+    np.testing.assert_allclose(st.residuals[:8], stc.residuals[:8], rtol=1e-6)
+    assert st.status == stc.status and st.solved == stc.solved
+    if st.solved:
+        assert np.all(_rel(X.to_numpy(), Xd) <= 1e-5)
+
+
+def test_c2_bicgstab_single_rhs_converges(bpf, oracle, c2):
+    # p = 1 on the same system converges to the direct solution
+    S, F, Xd = c2
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B[:, :1])
+    P = bpf.ilu0(A)
+    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-12, itmax=3000)
+    assert st.solved, st
+    assert _rel(X.to_numpy(), Xd[:, :1]).max() <= 1e-5
+
+
+def test_c2_gmres30(bpf, oracle, c2):
+    # Block GMRES(30) with ILU(0), FP64: the restarted run follows the CPU
+    # restatement over its first cycle; unrestarted (memory 30 as a hint) it
+    # reaches the backward-error target and the direct solution.
+    S, F, Xd = c2
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    P = bpf.ilu0(A, 64)
+    Xr, str_ = bpf.block_gmres(A, B, N=P, ldiv=True, memory=30, restart=True, atol=0.0, rtol=1e-10,
+                               itmax=60, history=True)
+    Nf = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    Xc, stc = oracle.block_gmres(S.scipy(), S.B, N=Nf, memory=30, restart=True, atol=0.0, rtol=1e-10,
+                                 itmax=60, history=True)
+    k = min(31, len(stc.residuals), len(str_.residuals))
+    np.testing.assert_allclose(str_.residuals[:k], stc.residuals[:k], rtol=1e-6)
+    X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=30, berr_tol=BERR, itmax=1000)
+    Xh = X.to_numpy()
+    be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+    assert st.solved and be.max() <= BERR
+    assert np.all(_rel(Xh, Xd) <= 1e-5)
+
+
+def test_c4_full_batch(bpf, gen, oracle):
+    nsys = 256
+    nbus, ps = gen.batch_sizes(nsys, 4000)
+    systems = [gen.make_system(int(nbus[i]), int(ps[i]), seed=5000 + i, rhs_seed=6000 + i)[1]
+               for i in range(nsys)]
+    print(f"c4: {nsys} systems, sum n {sum(S.n for S in systems)}, sum p {int(sum(ps))}", flush=True)
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    batch = bpf.BatchedSystems(As, [int(x) for x in ps])
+    try:
+        Xs, st, solved = batch.solve(Bs, precision="mixed", berr_tol=BERR, memory=40, itmax=1000,
+                                     timemax=120.0, bmgs_blocks=16)
+    finally:
+        batch.destroy()
+    assert solved.all(), np.flatnonzero(~solved)
+    worst_be, worst_rel = 0.0, 0.0
+    for S, X in zip(systems, Xs):
+        Xh = X.to_numpy()
+        be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+        Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+        worst_be = max(worst_be, be.max())
+        worst_rel = max(worst_rel, _rel(Xh, Xd).max())
+    print(f"c4: worst backward error {worst_be:.2e}, worst relative error {worst_rel:.2e}", flush=True)
+    assert worst_be <= BERR and worst_rel <= 1e-5
+
+
+def test_c5_refactor_full_size(bpf, gen, oracle):
+    grid, S = gen.make_system(70000, 256, seed=1000, rhs_seed=2000, rhs="injection", nunit=240)
+    vals = gen.newton_value_sets(grid, 3, seed=7000)
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    X = bpf.HipMatrix(B.shape)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, ordering=":AMD")
+    P, Q, rp, ci, _ = rf.factors()
+    print(f"c5: n={S.n} factor nnz {len(ci)}", flush=True)
+    for k, v in enumerate(vals):
+        A.set_values(v)
+        bpf.CUSOLVERRF.update_(rf, A)
+        _, _, _, _, fac = rf.factors()
+        ref = oracle.lu_refactor(S.indptr, S.indices, v, P, Q, rp, ci)
+        np.testing.assert_allclose(fac, ref, rtol=1e-9, atol=1e-11 * np.abs(ref).max())
+        _, berr = rf.solve_refined(A, B, refine=2, X=X, berr_tol=BERR)
+        Xh = X.to_numpy()
+        be = oracle.backward_error(S.indptr, S.indices, v, Xh, S.B)
+        Xd = oracle.direct_solve(S.indptr, S.indices, v, S.B)
+        print(f"c5 set {k}: refinement steps {rf.refine_steps}, berr {be.max():.2e}", flush=True)
+        assert be.max() <= BERR
+        np.testing.assert_allclose(berr, be, rtol=1e-6, atol=1e-15)
+        assert np.all(_rel(Xh, Xd) <= 1e-5)

@@ -236,8 +236,10 @@ def test_block_gmres_float32_blocks(bpf, gen, oracle):
     eps32 = np.sqrt(np.finfo(np.float32).eps)
     assert r <= 2 * (eps32 + eps32 * np.linalg.norm(S.B)) + 1e-6 * np.linalg.norm(S.B)
     # warm start with a Float32 guess, and an explicit tolerance
-    X2, st2 = bpf.block_gmres(A, B32, X, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-6)
-    assert X2.dtype == np.float32 and st2.niter <= st.niter
+    X2, st2 = bpf.block_gmres(A, B32, X, N=P, ldiv=True, memory=30, atol=0.0, rtol=1e-3)
+    assert X2.dtype == np.float32 and st2.solved
+    r2 = np.linalg.norm(S.B.astype(np.float32).astype(np.float64) - S.scipy() @ X2.to_numpy().astype(np.float64))
+    assert r2 <= 1.01 * r + 1e-6 * np.linalg.norm(S.B)
     with pytest.raises(TypeError):
         bpf.block_gmres(A, B32, bpf.HipMatrix.from_numpy(np.zeros(S.B.shape)), N=P, ldiv=True)
 
