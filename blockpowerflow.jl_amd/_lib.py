@@ -94,6 +94,7 @@ SIGNATURES = {
     "bpf_block_gmres_ex": [vp, vp, vp, vp, i, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_block_bicgstab_t": [vp, vp, vp, i, i, vp, i, vp, i, vp, i, C.POINTER(SolverParams), C.POINTER(SolverStats), dp, i],
     "bpf_backward_error": [vp, vp, i, vp, i, vp, i, dp],
+    "bpf_backward_error_noalloc": [vp, vp, i, vp, i, vp, i, vp],
     "bpf_hessenberg_eigenvalues": [dp, i, dp, dp],
     "bpf_householder": [vp, i, i, vp, i, vp, i, ip],
     "bpf_block_hessenberg_lsq": [vp, i, i, dp, i, dp, i, dp, i, dp],

@@ -180,6 +180,7 @@ int bpf_ctx_create(int device, bpf_ctx *out) {
     BPF_CHECK_HIP(hipGetDeviceCount(&c));
     BPF_REQUIRE(device >= 0 && device < c, "device %d not present (%d devices)", device, c);
     BPF_CHECK_HIP(hipSetDevice(device));
+    noalloc_setup(device);   // per-device address of the no-allocation scratch
     auto *ctx = new bpf_ctx_s();
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

@@ -234,6 +234,9 @@ namespace bpf {
 template <typename TV>
 void prec_apply_rm(hipStream_t s, bpf_prec P, int p, const TV *x, int ldx, TV *y, int ldy);
 void prec_check(bpf_prec P, hipStream_t s);
+// Device address of the no-allocation entry point's scratch array on
+// `device` (the current device), resolved once per device.
+void noalloc_setup(int device);
 }  // namespace bpf
 
 struct bpf_rf_s;

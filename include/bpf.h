@@ -268,6 +268,16 @@ int bpf_hessenberg_eigenvalues(const double *h_H, int d, double *h_re, double *h
 int bpf_backward_error(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx,
                        const double *d_B, int ldb, double *h_berr);
 
+/* bpf_backward_error without any device allocation (no reference
+ * counterpart: it is for callers that cannot allocate, such as code under
+ * HIP graph capture or a tight serving loop).  Writes the p per-RHS backward
+ * errors to device memory d_berr, asynchronously on the context's stream.
+ * Its scratch is a fixed device array of 3 x 4096 doubles per GPU, so p is
+ * at most 4096 (BPF_E_UNSUPPORTED beyond).  Calls that overlap on one GPU
+ * (several contexts or host threads at once) share that array and race. */
+int bpf_backward_error_noalloc(bpf_ctx ctx, bpf_csr A, int p, const double *d_X, int ldx,
+                               const double *d_B, int ldb, double *d_berr);
+
 /* ---- batched systems ------------------------------------------------------- */
 /* Many independent systems of uneven size solved together (no reference
  * counterpart; the reference solves one system per call).  Host CSR arrays
