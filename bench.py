@@ -558,22 +558,29 @@ def bench_cases(args, bpf, gen):
         Bh = bpf.mmread_dense(os.path.join(mtx, f"Gu_{name}.mtx.gz"))
         A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
         B = bpf.HipMatrix.from_numpy(Bh)
-        cases.append({"name": name, "Ah": Ah, "Bh": Bh, "A": A, "B": B, "P": bpf.ilu0(A)})
+        # the reference's block-Jacobi leg: size_block = 32, nblocks = div(n, 32)
+        # (benchmarks/benchmarks_nvidia.jl:46-51)
+        BJ = bpf.BlockJacobiPreconditioner(A, max(1, Ah.shape[0] // 32))
+        cases.append({"name": name, "Ah": Ah, "Bh": Bh, "A": A, "B": B, "P": bpf.ilu0(A), "BJ": BJ})
     log(f"[bench] read {len(cases)} cases from tests/golden/mtx in {time.time() - t0:.1f}s")
 
     def solve(c, precond):
-        if precond:
+        if precond == "ilu0":
             c["P"].factor_(c["A"])
             return bpf.block_gmres(c["A"], c["B"], N=c["P"], ldiv=True, **CASE_KW)
+        if precond == "bj":
+            bpf.update(c["BJ"], c["A"])
+            return bpf.block_gmres(c["A"], c["B"], N=c["BJ"], **CASE_KW)
         return bpf.block_gmres(c["A"], c["B"], **CASE_KW)
 
     def step():
         # the sweep as the reference runs it: per case the ILU(0)-preconditioned
-        # solve (ILU(0) factorization included), then the unpreconditioned one
+        # solve (ILU(0) factorization included), the block-Jacobi one (block
+        # inverses included), then the unpreconditioned one
         rows = []
         for c in cases:
             r = {}
-            for precond, key in ((True, "ilu0"), (False, "none")):
+            for precond, key in (("ilu0", "ilu0"), ("bj", "bj"), (None, "none")):
                 t = time.perf_counter()
                 X, st = solve(c, precond)
                 ctx.synchronize()
@@ -589,7 +596,7 @@ def bench_cases(args, bpf, gen):
     for c, r in zip(cases, rows):
         Ah, Bh = c["Ah"], c["Bh"]
         ent = {"case": c["name"], "n": Ah.shape[0], "nnz": Ah.nnz, "p": Bh.shape[1]}
-        for key in ("ilu0", "none"):
+        for key in ("ilu0", "bj", "none"):
             sec, X, st = r[key]
             be = bpf.backward_error(c["A"], X, c["B"])
             R = Bh - _csr_matvec(Ah, X.to_numpy())
@@ -601,6 +608,8 @@ def bench_cases(args, bpf, gen):
         t_prec += ent["ilu0"]["ms"] * 1e-3
         log(f"[bench]   {c['name']:18s} n={ent['n']:6d} p={ent['p']:5d}  ILU(0): its {ent['ilu0']['niter']:3d} "
             f"solved {ent['ilu0']['solved']!s:5s} |R| {ent['ilu0']['rnorm']:.2e} {ent['ilu0']['ms']:9.2f} ms"
+            f"   block Jacobi: its {ent['bj']['niter']:3d} solved {ent['bj']['solved']!s:5s} "
+            f"|R| {ent['bj']['rnorm']:.2e} {ent['bj']['ms']:9.2f} ms"
             f"   none: its {ent['none']['niter']:3d} solved {ent['none']['solved']!s:5s} "
             f"|R| {ent['none']['rnorm']:.2e} {ent['none']['ms']:9.2f} ms")
     value = solved_rhs / t_prec
@@ -614,7 +623,8 @@ def bench_cases(args, bpf, gen):
         "workload": ("cases: the reference benchmark sweep (benchmarks/benchmarks_amd.jl) over its "
                      f"{len(cases)} case files with a Jacobian, A = Gx (CSR FP64), B = dense Gu; per case "
                      "ILU(0) + block GMRES(atol=1e-10, rtol=0, memory=1, restart=false, itmax=50), then "
-                     "the same without a preconditioner; value = RHS of the cases the ILU(0) run solves "
+                     "the same with the block-Jacobi preconditioner (blocks of 32) and without a "
+                     "preconditioner; value = RHS of the cases the ILU(0) run solves "
                      "(||R||_F <= 1e-10 and per-RHS backward error <= 1e-10) / the ILU(0) runs' time"),
         "cases": table,
     }

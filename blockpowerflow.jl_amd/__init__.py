@@ -33,7 +33,7 @@ __all__ = [
     "block_bicgstab", "block_bicgstab_", "BlockBicgstabSolver", "BlockGMRESStats", "SimpleStats",
     "backward_error", "spmm", "CUSOLVERRF", "BpfError", "SingularException", "BatchedSystems",
     "block_gmres_batched", "householder", "block_hessenberg_lsq", "set_option", "get_option",
-    "HostCSR", "mmread", "mmread_dense", "mmwrite",
+    "HostCSR", "mmread", "mmread_dense", "mmwrite", "BlockJacobiPreconditioner", "update",
 ]
 
 L = _lib.load()
@@ -327,6 +327,40 @@ class IC0(_Preconditioner):
     """IC(0) (NVIDIA_IC0 / AMD_IC0, src/preconditioners_nvidia.jl:1-11,
     src/preconditioners_amd.jl:1-9)."""
     kind = _lib.BPF_PREC_IC0
+
+
+class BlockJacobiPreconditioner(_Preconditioner):
+    """LinearSolvers.BlockJacobiPreconditioner(J, nblocks, device) of ExaPF, the
+    right preconditioner the reference's benchmark and tests hand to
+    block_gmres / block_bicgstab (benchmarks/benchmarks_nvidia.jl:46-51,71;
+    test/block_gmres.jl:5,18-25): nblocks parts of the symmetrized graph of
+    A (at most 128 unknowns each), dense inverses of the diagonal blocks on
+    the device.  ``mul_(y, P, x)`` applies them; ``update(P, A)`` recomputes
+    them from A's current values."""
+    kind = _lib.BPF_PREC_BLOCK_JACOBI
+
+    def __init__(self, A: HipSparseMatrixCSR, nblocks: int, device=None):
+        self.ctx = A.ctx
+        self.n = A.n
+        self.nrhs = None
+        self.nblocks = int(nblocks)
+        h = C.c_void_p()
+        check(L.bpf_block_jacobi_create(self.ctx.handle, A.handle, self.nblocks, C.byref(h)))
+        self.handle = h
+        self.pattern = (A.indptr, A.indices)
+
+    @property
+    def partition(self) -> np.ndarray:
+        """Part (0..nblocks-1) of every unknown."""
+        out = np.empty(self.n, dtype=np.int32)
+        check(L.bpf_block_jacobi_partition(self.handle, out.ctypes.data_as(_lib.ip)))
+        return out
+
+
+def update(P, A: HipSparseMatrixCSR, device=None):
+    """LinearSolvers.update(P, A, device): refactor a preconditioner with A's
+    current values (same pattern)."""
+    return P.factor_(A)
 
 
 def ilu0(A: HipSparseMatrixCSR, nrhs: int | None = None) -> ILU0:

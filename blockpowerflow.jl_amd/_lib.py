@@ -15,7 +15,7 @@ BPF_OK = 0
 BPF_E_INVALID, BPF_E_HIP, BPF_E_SINGULAR, BPF_E_NOMEM, BPF_E_UNSUPPORTED, BPF_E_BREAKDOWN = 1, 2, 3, 4, 5, 6
 BPF_F64, BPF_F32 = 0, 1
 BPF_COL_MAJOR, BPF_ROW_MAJOR = 0, 1
-BPF_PREC_ILU0, BPF_PREC_IC0 = 0, 1
+BPF_PREC_ILU0, BPF_PREC_IC0, BPF_PREC_BLOCK_JACOBI = 0, 1, 2
 BPF_STATUS_UNKNOWN, BPF_STATUS_SOLVED, BPF_STATUS_TIRED, BPF_STATUS_OVERTIMED = 0, 1, 2, 3
 BPF_PRECISION_FP64, BPF_PRECISION_MIXED, BPF_PRECISION_MIXED32 = 0, 1, 2
 ORDERINGS = {"AMD": 0, "MDQ": 1, "METIS": 2, "RCM": 3, "NATURAL": 4}
@@ -85,6 +85,8 @@ SIGNATURES = {
     "bpf_prec_factor": [vp, vp],
     "bpf_prec_apply": [vp, i, i, i, vp, i, vp, i],
     "bpf_prec_get_factor": [vp, dp],
+    "bpf_block_jacobi_create": [vp, vp, i, C.POINTER(vp)],
+    "bpf_block_jacobi_partition": [vp, ip],
     "bpf_prec_levels": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bpf_prec_destroy": [vp],
     "bpf_solver_params_default": [C.POINTER(SolverParams)],

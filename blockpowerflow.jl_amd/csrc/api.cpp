@@ -34,7 +34,10 @@ void *BufPool::take(size_t nbytes, size_t *got) {
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, nbytes);
   if (e != hipSuccess) {
-    // out of memory with blocks cached: release them and retry once
+    // out of memory with blocks cached: release them and retry once (the
+    // failed call's error is cleared, so that a later hipGetLastError()
+    // after a kernel launch does not report it)
+    (void)hipGetLastError();
     trim();
     e = hipMalloc(&p, nbytes);
   }
@@ -414,6 +417,10 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A) {
     BPF_REQUIRE(A->n == P->n && A->nnz == P->nnz && A->h_colidx == P->h_colidx &&
                     A->h_rowptr == P->h_rowptr,
                 "matrix pattern differs from the analysed pattern");
+    if (P->kind == BPF_PREC_BLOCK_JACOBI) {
+      bj_factor(P, A);   // LS.update(BJ, A, device): new block inverses
+      return;
+    }
     hipStream_t s = P->ctx->stream;
     BPF_CHECK_HIP(hipMemcpyAsync(P->fac.ptr, A->val.ptr, (size_t)P->nnz * sizeof(double),
                                  hipMemcpyDeviceToDevice, s));
@@ -530,6 +537,7 @@ int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p, const void *d_X, in
 int bpf_prec_get_factor(bpf_prec P, double *h_val) {
   return guard([&] {
     BPF_REQUIRE(P && h_val, "null argument");
+    BPF_REQUIRE(P->kind != BPF_PREC_BLOCK_JACOBI, "a block-Jacobi preconditioner has no combined factor");
     BPF_CHECK_HIP(hipMemcpyAsync(h_val, P->fac.ptr, (size_t)P->nnz * sizeof(double),
                                  hipMemcpyDeviceToHost, P->ctx->stream));
     BPF_CHECK_HIP(hipStreamSynchronize(P->ctx->stream));

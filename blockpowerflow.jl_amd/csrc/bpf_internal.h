@@ -226,6 +226,14 @@ struct bpf_prec_s {
   bpf::TriOp lo, up;                          // solve operators
   bpf::DevBuf tmp;                            // n x p scratch for apply
   int max_row_nnz = 0;
+  // Block Jacobi (kind BPF_PREC_BLOCK_JACOBI): rows grouped into nblocks
+  // parts; part b holds rows bj_rows[bj_ptr[b] .. bj_ptr[b+1]) and its dense
+  // inverse (row-major, leading dimension bj_mmax) at b * bj_mmax^2 of
+  // bj_inv (FP64) and bj_inv32 (FP32 copy).  bj_part[i] = part of row i,
+  // bj_loc[i] = position of row i within its part.
+  int bj_nblocks = 0, bj_mmax = 0;
+  std::vector<int> h_bj_part;
+  bpf::DevBuf bj_rows, bj_ptr, bj_part, bj_loc, bj_inv, bj_inv32;
 };
 
 namespace bpf {
@@ -237,6 +245,11 @@ void prec_check(bpf_prec P, hipStream_t s);
 // Device address of the no-allocation entry point's scratch array on
 // `device` (the current device), resolved once per device.
 void noalloc_setup(int device);
+// Block Jacobi: dense inverses of the diagonal blocks from A's current
+// values (throws BPF_E_SINGULAR on a singular block), and their application.
+void bj_factor(bpf_prec P, bpf_csr A);
+template <typename TV>
+void bj_apply(hipStream_t s, bpf_prec P, int p, const TV *x, int ldx, TV *y, int ldy);
 }  // namespace bpf
 
 struct bpf_rf_s;

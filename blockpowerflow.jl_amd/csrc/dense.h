@@ -58,6 +58,9 @@ struct GemmArgs {
   // 0: inactive groups are skipped.  1 (solution assembly): a group takes
   // part iff K > 0, over its first min(nb, K) blocks.
   int use_k = 0;
+  // 1: groups that do not take part leave Y untouched even when beta == 0
+  // (passes over a subset of the groups); 0: their fresh Y is zeroed.
+  int keep_skipped = 0;
   int rows_sum = 0;    // stacked groups: sum of heights (traffic accounting)
   double cells = -1.0; // rows x width of the groups that take part (accounting; negative = all)
 };
@@ -70,12 +73,16 @@ struct CholArgs {
   int n;               // rows of the block (for the shift)
   const GroupDesc *groups;
   int *flags;          // [0]: bits kept over a solve, [1]: bits of this factorization;
-                       // bit 0: a shift was needed, bit 1: a column was deflated
+                       // bit 0: a shift was needed, bit 1: a column was deflated,
+                       // bit 2: a pivot at rounding level (kept, re-projected)
   // 1: a pivot that is not positive (column k lies in the span of the columns
   // before it to rounding) gives R a zero row k instead of a shift, with 1 on
   // the diagonal of the factor that Rinv inverts; launch_fill_deflated then
   // replaces column k of W Rinv.
   int deflate = 0;
+  // Per-group bits of this factorization (deflation 2, rounding-level pivot
+  // 4) at gflags[g], written for active groups when deflate is set (may be null).
+  int *gflags = nullptr;
 };
 
 // Hessenberg arena: block column k is a separate region cols[k] holding, per
@@ -135,6 +142,10 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
 template <typename TVV, typename TY>
 void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng);
 void launch_chol(hipStream_t s, CholArgs a, int ng);
+// Splits the groups by their per-group CholQR bits: out[g] (breakdown, bits
+// 2 or 4) and out[ng + g] (none) copy groups[g], each active only where
+// groups[g] is active and the bits say so.
+void launch_split_groups(hipStream_t s, const GroupDesc *groups, const int *gflags, int ng, GroupDesc *out);
 // After a deflating Cholesky (bit 1 of *flags): column k of group g's rows of
 // Q (row-major, ld) becomes pseudo-random values of unit expected norm
 // wherever R[g] (S x S at g*stride) has a zero diagonal entry k.  Returns at

@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
   const bool takes_part = a.use_k ? gd.K > 0 : gd.active != 0;
   if (!takes_part) {
     // Skipped group: a fresh block (beta == 0) is zeroed, anything else kept.
-    if (a.beta == 0.0) {
+    if (a.beta == 0.0 && !a.keep_skipped) {
       TY *Y = static_cast<TY *>(a.Y) + gd.c0;
       for (int e = tid; e < RT * gd.w; e += blockDim.x) {
         const int r = r0 + e / gd.w, c = e % gd.w;
@@ -491,6 +491,12 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
 // factor that Rinv inverts keeps 1 there, so column k of W Rinv is the
 // rounding-level remainder of w_k, which launch_fill_deflated replaces.  Bits
 // go to flags[0] (kept over a solve) and flags[1] (this factorization).
+// A positive pivot d_k <= 1e-10 G_kk (w_k's component off the columns
+// before it below 1e-5 of its norm: rounding noise of a dependent column, or
+// a severely ill-conditioned block) is kept but flagged (bit 2): 1/sqrt(d_k)
+// amplifies the noise, whose components along the earlier basis would
+// survive the second pass, so the caller re-projects the block and adds a
+// third pass as after a deflation.
 // R^{-1} is formed blockwise: diagonal 16 x 16 inverses, then
 // X_IJ = -X_II sum_{I<M<=J} R_IM X_MJ, one block column per wavefront; X is
 // kept transposed in the strict lower triangle with its diagonal in dinv.
@@ -509,6 +515,7 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   __shared__ int s_fail;
   __shared__ double s_norm, s_part[16];
   __shared__ unsigned char s_dep[128];
+  __shared__ int s_small;
   const int g = blockIdx.x;
   if (!a.groups[g].active) return;
   const int w = a.groups[g].w;
@@ -518,7 +525,10 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
   const double *Gin = a.G + (size_t)g * a.gstride;
   double *R = a.R + (size_t)g * a.gstride;
   double *Ri = a.Rinv + (size_t)g * a.gstride;
-  if (tid == 0) s_norm = 0.0;
+  if (tid == 0) {
+    s_norm = 0.0;
+    s_small = 0;
+  }
   if (tid < 128) s_dep[tid] = 0;
   __syncthreads();
   double loc = 0.0;
@@ -569,6 +579,7 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
             if (lane == 0) s_fail = 1;
             break;
           }
+          if (a.deflate && lane == 0 && d <= 1e-10 * Gin[k * S + k]) s_small = 1;
           const double rkk = sqrt(d);
           if (lane > kk && lane < pw) G[k * kCholLd + j0 + lane] /= rkk;
           __builtin_amdgcn_wave_barrier();
@@ -700,6 +711,11 @@ __global__ __launch_bounds__(1024) void chol_kernel(CholArgs a) {
       atomicOr(a.flags, 2);
       atomicOr(a.flags + 1, 2);
     }
+    if (s_small) {
+      atomicOr(a.flags, 4);
+      atomicOr(a.flags + 1, 4);
+    }
+    if (a.deflate && a.gflags) a.gflags[g] = (dep ? 2 : 0) | (s_small ? 4 : 0);
   }
 }
 
@@ -737,11 +753,24 @@ __global__ __launch_bounds__(kB) void fill_deflated_kernel(TV *Q, int ld, const 
   }
 }
 
+__global__ __launch_bounds__(kB) void split_groups_kernel(const GroupDesc *groups, const int *gflags, int ng,
+                                                          GroupDesc *out) {
+  const int g = blockIdx.x * kB + threadIdx.x;
+  if (g >= ng) return;
+  GroupDesc a = groups[g], b = groups[g];
+  const bool brk = (gflags[g] & 6) != 0;
+  a.active = groups[g].active && brk;
+  b.active = groups[g].active && !brk;
+  out[g] = a;
+  out[ng + g] = b;
+}
+
 // C = A * B for upper-triangular S x S blocks per group (C may not alias).
 __global__ __launch_bounds__(kB) void trmm_upper_kernel(const double *A, const double *B, double *C,
                                                         int S, size_t stride, size_t cstride,
                                                         const GroupDesc *groups) {
   const int g = blockIdx.y;
+  if (!groups[g].active) return;
   const int w = groups[g].w;
   const int e = blockIdx.x * kB + threadIdx.x;
   if (e >= S * S) return;
@@ -1504,6 +1533,12 @@ template void launch_block_gemm<float, double>(hipStream_t, GemmArgs, int, int);
 void launch_chol(hipStream_t s, CholArgs a, int ng) {
   if (a.S > 128) BPF_THROW(BPF_E_UNSUPPORTED, "block width %d too large for Cholesky", a.S);
   hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a);
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void launch_split_groups(hipStream_t s, const GroupDesc *groups, const int *gflags, int ng, GroupDesc *out) {
+  if (ng <= 0) return;
+  hipLaunchKernelGGL(split_groups_kernel, dim3((ng + kB - 1) / kB), dim3(kB), 0, s, groups, gflags, ng, out);
   BPF_CHECK_HIP(hipGetLastError());
 }
 

@@ -96,7 +96,8 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
   ng_ = (int)h_groups_.size();
   // per-solve work buffers come from the context's pool
   for (DevBuf *b : {&d_groups_, &W_, &P_, &E_, &rnorm_, &Y_, &Zb_, &Rinvb_, &Xtmp_, &Vf_, &Zf_, &tmpcol_,
-                    &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.R3, &cq_.R3i, &cq_.flags, &cq_.part})
+                    &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.R3, &cq_.R3i, &cq_.flags, &cq_.part,
+                    &cq_.gflags, &cq_.gsets})
     b->pool = &ctx->pool;
   V_.pool = Z_.pool = &ctx->pool;
   upload(d_groups_, h_groups_, st_);
@@ -149,26 +150,36 @@ void CholQrWork::init(int ng, int S) {
   const size_t ss = (size_t)ng * S * S * sizeof(double);
   G.ensure(ss); R1.ensure(ss); R1i.ensure(ss); R2.ensure(ss); R2i.ensure(ss); R3.ensure(ss); R3i.ensure(ss);
   flags.ensure(16);
+  gflags.ensure(std::max(1, ng) * sizeof(int));
+  gsets.ensure(2 * (size_t)std::max(1, ng) * sizeof(GroupDesc));
 }
 
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
                     int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
-                    const std::function<void(void *)> &reorth) {
+                    const ReorthFn &reorth) {
   // Two CholQR passes; C = R2 R1 (upper), Vout = W R1^{-1} R2^{-1}.
   const size_t SS = (size_t)S * S;
   int *fl = w.flags.as<int>();
   fill_int(s, fl + 1, 0, 1);
+  w.gflags.ensure(std::max(1, ng) * sizeof(int));
+  w.gsets.ensure(2 * (size_t)std::max(1, ng) * sizeof(GroupDesc));
   DevBuf *Rs[3] = {&w.R1, &w.R2, &w.R3};
   DevBuf *Ris[3] = {&w.R1i, &w.R2i, &w.R3i};
-  auto pass = [&](int i, void *src, void *dst, bool deflate) {
-    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows, dg, ng, 0, nullptr, rows_sum};
+  // cells < 0: every group takes part (kernel-timer accounting); the passes
+  // over a subset of groups chosen on the device are counted as no traffic
+  auto pass = [&](int i, void *src, void *dst, bool deflate, const GroupDesc *grp, double cells) {
+    ProjArgs pa{nullptr, src, 0, 1, p, src, p, gmax_rows, grp, ng, 0, nullptr, rows_sum};
+    pa.cells = cells;
     launch_proj<TV, TV>(s, pa, S, w.G.as<double>(), 0, SS, false, w.part);
-    CholArgs ca{w.G.as<double>(), Rs[i]->as<double>(), Ris[i]->as<double>(), SS, S, n_shift, dg, fl};
+    CholArgs ca{w.G.as<double>(), Rs[i]->as<double>(), Ris[i]->as<double>(), SS, S, n_shift, grp, fl};
     ca.deflate = deflate;
+    ca.gflags = w.gflags.as<int>();
     launch_chol(s, ca, ng);
-    GemmArgs ga{nullptr, src, 0, 1, p, Ris[i]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, dg};
+    GemmArgs ga{nullptr, src, 0, 1, p, Ris[i]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, grp};
     ga.rows_sum = rows_sum;
+    ga.cells = cells;
+    ga.keep_skipped = grp != dg;   // a subset pass leaves the other groups alone
     launch_block_gemm<TV, TV>(s, ga, S, ng);
   };
   // The first pass drops columns whose Cholesky pivot is not positive (they
@@ -177,49 +188,65 @@ void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, in
   // pseudo-random directions, as Householder QR completes a rank-deficient
   // block with arbitrary orthonormal ones; W = Q1 R1 holds with zero rows of
   // R1 there.
-  pass(0, W, W, true);
+  pass(0, W, W, true, dg, -1.0);
   launch_fill_deflated<TV>(s, static_cast<TV *>(W), p, w.R1.as<double>(), S, SS, dg, ng, gmax_rows, fl + 1);
+  // The host learns whether any group broke down (one 4-byte read); the
+  // common case, none, is the plain second pass over all groups.
   int h = 0;
   BPF_CHECK_HIP(hipMemcpyAsync(&h, fl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   BPF_CHECK_HIP(hipStreamSynchronize(s));
-  if (!(h & 2)) {
-    pass(1, W, Vout, false);
+  if (!(h & 6)) {
+    pass(1, W, Vout, false, dg, -1.0);
     launch_trmm_upper(s, w.R2.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
     return;
   }
-  // Deflated: the new directions are orthogonal to nothing yet.  The caller
-  // projects the block off the earlier basis (the rest of the block only
-  // moves at rounding level: W was projected before), then two passes make
-  // it orthonormal; C = R3 R2 R1, the product staged in G.
-  if (reorth) reorth(W);
-  pass(1, W, W, false);
-  pass(2, W, Vout, false);
-  launch_trmm_upper(s, w.R3.as<double>(), w.R2.as<double>(), w.G.as<double>(), S, SS, SS, dg, ng);
-  launch_trmm_upper(s, w.G.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, dg, ng);
+  // Otherwise the groups are split on the device by their own bits, so no
+  // group's arithmetic depends on another's.
+  GroupDesc *brk = w.gsets.as<GroupDesc>(), *clean = brk + ng;
+  launch_split_groups(s, dg, w.gflags.as<int>(), ng, brk);
+  // Groups without a breakdown: the second pass.
+  pass(1, W, Vout, false, clean, 0.0);
+  launch_trmm_upper(s, w.R2.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, clean, ng);
+  // Groups with one: the new directions are orthogonal to nothing yet, and
+  // amplified noise may lean on the earlier basis.  The block is projected
+  // off the earlier basis (the other columns only move at rounding level, W
+  // having been projected already), then two passes make it orthonormal;
+  // C = R3 R2 R1, the product staged in G.
+  if (reorth) reorth(W, brk);
+  pass(1, W, W, false, brk, 0.0);
+  pass(2, W, Vout, false, brk, 0.0);
+  launch_trmm_upper(s, w.R3.as<double>(), w.R2.as<double>(), w.G.as<double>(), S, SS, SS, brk, ng);
+  launch_trmm_upper(s, w.G.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, brk, ng);
 }
 template void cholqr2_groups<double>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                     void *, double *, size_t, CholQrWork &,
-                                     const std::function<void(void *)> &);
+                                     void *, double *, size_t, CholQrWork &, const ReorthFn &);
 template void cholqr2_groups<float>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                    void *, double *, size_t, CholQrWork &,
-                                    const std::function<void(void *)> &);
+                                    void *, double *, size_t, CholQrWork &, const ReorthFn &);
 
 template <typename TV>
 void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride, int kprev) {
-  std::function<void(void *)> reorth;
+  ReorthFn reorth;
   if (kprev >= 0) {
-    // one classical Gram-Schmidt pass of the block against V_0..V_kprev
-    reorth = [this, kprev](void *Wp) {
-      const GroupDesc *dg = d_groups_.as<GroupDesc>();
-      const size_t SS = (size_t)S_ * S_, gstr = (size_t)(kprev + 2) * SS;
+    // one Gram-Schmidt pass of the block against V_0..V_kprev for the groups
+    // given, in steps of at most 8 basis blocks (bounded partial sums);
+    // traffic not counted (the subset is chosen on the device)
+    reorth = [this, kprev](void *Wp, const GroupDesc *dg) {
+      constexpr int kStep = 8;
+      const size_t SS = (size_t)S_ * S_, gstr = (size_t)kStep * SS;
       const void *const *vt = vtab_.dev<const void>();
-      tmpcol_.ensure(hcol_elems(kprev, ng_, S_) * sizeof(double));
+      const size_t need = hcol_elems(kStep - 2, ng_, S_) * sizeof(double);
+      tmpcol_.ensure(need);
       double *t = tmpcol_.as<double>();
-      ProjArgs pa{vt, nullptr, 0, kprev + 1, cfg_.p, Wp, cfg_.p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
-      launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
-      GemmArgs ga{vt, nullptr, 0, kprev + 1, cfg_.p, t, SS, gstr, S_, Wp, cfg_.p, gmax_rows_, -1.0, 1.0, dg};
-      ga.rows_sum = rows_sum_;
-      launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+      for (int i = 0; i <= kprev; i += kStep) {
+        const int nb = std::min(kStep, kprev + 1 - i);
+        ProjArgs pa{vt, nullptr, i, nb, cfg_.p, Wp, cfg_.p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
+        pa.cells = 0.0;
+        launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
+        GemmArgs ga{vt, nullptr, i, nb, cfg_.p, t, SS, gstr, S_, Wp, cfg_.p, gmax_rows_, -1.0, 1.0, dg};
+        ga.rows_sum = rows_sum_;
+        ga.cells = 0.0;
+        launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
+      }
     };
   }
   cholqr2_groups<TV>(st_, d_groups_.as<GroupDesc>(), ng_, S_, cfg_.p, gmax_rows_, rows_sum_, cfg_.n, W, Vout,

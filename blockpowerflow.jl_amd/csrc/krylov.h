@@ -51,21 +51,28 @@ struct PtrTable {
 // factors and their inverses; the flag words; projection partials).
 struct CholQrWork {
   DevBuf G, R1, R1i, R2, R2i, R3, R3i, flags, part;
+  DevBuf gflags, gsets;        // per-group bits; the groups split by them (2 ng)
   void init(int ng, int S);
 };
 
 // Grouped CholQR2 of W (row-major, ld p): Vout = W R1^{-1} R2^{-1} with
 // C = R2 R1 (upper, per group at Cout + g*cstride).  W is overwritten by the
 // first pass.  Columns whose first-pass Cholesky pivot is not positive are
-// deflated (zero rows of R1) and replaced by pseudo-random directions; then
-// `reorth` (if set) projects the block off the earlier basis and two more
-// passes follow, C = R3 R2 R1.  n_shift is the row count used in the shift of
-// a failed pivot; w.flags[0] gathers bit 0 (shift) and bit 1 (deflation)
-// and is cleared by the caller.
+// deflated (zero rows of R1) and replaced by pseudo-random directions, and a
+// positive pivot at rounding level is flagged; in the groups where either
+// happened `reorth` (if set) projects the block off the earlier basis and two
+// more passes follow, C = R3 R2 R1.  The host reads one flag word to see
+// whether any group broke down; which groups take the extra passes is
+// decided on the device, group by group, so no group's arithmetic depends on
+// another's.
+// n_shift is the row count used in the shift of a failed pivot; w.flags[0]
+// gathers bit 0 (shift), bit 1 (deflation) and bit 2 (rounding-level pivot)
+// and is cleared by the caller.  reorth(W, groups) acts on the groups given.
+using ReorthFn = std::function<void(void *, const GroupDesc *)>;
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
                     int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
-                    const std::function<void(void *)> &reorth = {});
+                    const ReorthFn &reorth = {});
 
 // Host-visible description of one Krylov run.
 struct ArnoldiConfig {

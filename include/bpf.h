@@ -127,7 +127,7 @@ int bpf_spmm(bpf_ctx ctx, bpf_csr A, int dtype, int layout, int p,
              const void *d_X, int ldx, void *d_Y, int ldy, double alpha, double beta);
 
 /* ---- ILU(0) / IC(0) preconditioners -------------------------------------- */
-enum { BPF_PREC_ILU0 = 0, BPF_PREC_IC0 = 1 };
+enum { BPF_PREC_ILU0 = 0, BPF_PREC_IC0 = 1, BPF_PREC_BLOCK_JACOBI = 2 };
 /* ilu0(A, nrhs) / ilu0(A) (src/preconditioners_nvidia.jl:42-47,
  * src/preconditioners_amd.jl:43-46) and ic0(A, nrhs) / ic0(A)
  * (src/preconditioners_nvidia.jl:6-11, src/preconditioners_amd.jl:5-9):
@@ -144,6 +144,17 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A);
  * (src/block_bicgstab.jl:251,263).  d_Y may alias d_X. */
 int bpf_prec_apply(bpf_prec P, int dtype, int layout, int p,
                    const void *d_X, int ldx, void *d_Y, int ldy);
+/* LS.BlockJacobiPreconditioner(J, nblocks, device) + LS.update (ExaPF, as
+ * passed to block_gmres / block_bicgstab by benchmarks/benchmarks_nvidia.jl:46-51,71
+ * and test/block_gmres.jl:5,18-25): the unknowns are split into nblocks parts of
+ * the symmetrized graph of A (ceil(n / nblocks) <= 128 unknowns each,
+ * BPF_E_UNSUPPORTED beyond), the dense diagonal blocks A[part, part] are
+ * inverted on the device (BPF_E_SINGULAR for a singular block).
+ * bpf_prec_apply (mul!(y, P, x)) computes y[part] = inv(A[part, part]) x[part];
+ * bpf_prec_factor recomputes the inverses from A's current values
+ * (LS.update).  bpf_block_jacobi_partition writes the part of each row. */
+int bpf_block_jacobi_create(bpf_ctx ctx, bpf_csr A, int nblocks, bpf_prec *P);
+int bpf_block_jacobi_partition(bpf_prec P, int32_t *h_part);
 /* Copy the combined factor values (nnz float64, A's pattern) to the host. */
 int bpf_prec_get_factor(bpf_prec P, double *h_val);
 /* Number of level sets of the lower / upper solve schedules. */

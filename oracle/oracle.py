@@ -16,13 +16,18 @@ Contents
 * ``direct_solve``: FP64 sparse direct solve (SuperLU via scipy), the known
   answer the reference's own test uses (UMFPACK ``J \\ b``,
   test/cusolverRF.jl:12);
-* ``backward_error``: per-RHS ||A x - b||_inf / (||A||_inf ||x||_inf + ||b||_inf).
+* ``backward_error``: per-RHS ||A x - b||_inf / (||A||_inf ||x||_inf + ||b||_inf);
+* ``block_jacobi``: dense per-block inverses for a given partition (ExaPF's
+  BlockJacobiPreconditioner, benchmarks/benchmarks_nvidia.jl:46-51).
 
-Pinning (tests/test_oracle.py): block_gmres reduces to textbook GMRES for p=1
-(numpy restatement of Saad Alg. 6.9 with Givens), terminates exactly in
-ceil(n/p) iterations on tiny systems, and its residual history equals the true
-residual norm; block_bicgstab with p=1 matches the scalar BiCGSTAB recurrence
-with |omega|; both converge to the direct solution.
+Pinning (tests/test_oracle.py): block_gmres reduces to textbook GMRES for p=1,
+right-preconditioned (Saad Alg. 6.9 / 9.5 with Givens) and left-preconditioned
+(Alg. 9.4), terminates exactly in ceil(n/p) iterations on tiny systems, and its
+residual estimate equals the true (M-weighted) residual norm; block_bicgstab
+with p=1 matches van der Vorst's rho-ratio BiCGSTAB where omega > 0, its
+recursive residual equals B - A X at every step, and both solvers converge to
+the direct solution; block_jacobi equals the exact inverse for one part and
+on block-diagonal matrices.
 """
 from __future__ import annotations
 
@@ -129,6 +134,28 @@ def ilu0_apply(indptr, indices, fval, X):
 def ic0_apply(indptr, indices, fval, X):
     """y = L^{-T} L^{-1} x (src/preconditioners_nvidia.jl:19-22)."""
     return trsm(TRSM_LOWER_T, indptr, indices, fval, trsm(TRSM_LOWER, indptr, indices, fval, X))
+
+
+def block_jacobi(indptr, indices, data, part):
+    """Block-Jacobi preconditioner for a given partition (ExaPF's
+    BlockJacobiPreconditioner as passed by benchmarks/benchmarks_nvidia.jl:46-51):
+    returns a callable Y = apply(X) with Y[idx] = inv(A[idx, idx]) X[idx] for
+    the rows idx of each part (dense inverses by LAPACK)."""
+    ip, ix, dv = _csr(indptr, indices, data)
+    A = sp.csr_matrix((dv, ix, ip), shape=(len(ip) - 1, len(ip) - 1))
+    part = np.asarray(part)
+    blocks = []
+    for b in range(int(part.max()) + 1):
+        idx = np.flatnonzero(part == b)
+        blocks.append((idx, np.linalg.inv(A[idx][:, idx].toarray())))
+
+    def apply(X):
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.empty_like(X)
+        for idx, Binv in blocks:
+            Y[idx] = Binv @ X[idx]
+        return Y
+    return apply
 
 
 def lu_refactor(indptr, indices, data, P, Q, frow, fcol):
@@ -339,8 +366,10 @@ def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, histo
         V = A @ Y                                                # :135
         K = Ct @ V                                               # :136
         D = Ct @ R                                               # :137
-        Kf = scipy.linalg.lu_factor(K)                           # :138
-        alpha = scipy.linalg.lu_solve(Kf, D)                     # :139
+        # (no finiteness check: a breakdown propagates as NaN into R, as
+        # LAPACK getrf does in the reference, and ends the run below)
+        Kf = scipy.linalg.lu_factor(K, check_finite=False)       # :138
+        alpha = scipy.linalg.lu_solve(Kf, D, check_finite=False)  # :139
         U = R - V @ alpha                                        # :140-141
         X = X + Y @ alpha                                        # :142-143
         Y = U if N is None else N(U)                             # :144
@@ -349,7 +378,7 @@ def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, histo
         X = X + omega * Y                                        # :147
         R = U - omega * Q                                        # :148
         D = Ct @ Q                                               # :149
-        beta = scipy.linalg.lu_solve(Kf, D)                      # :150
+        beta = scipy.linalg.lu_solve(Kf, D, check_finite=False)  # :150
         U = P - omega * V                                        # :151
         P = R - U @ beta                                         # :152-153
         it += 1
@@ -360,6 +389,12 @@ def block_bicgstab(A, B, *, C=None, N=None, atol=None, rtol=None, itmax=0, histo
             stats.residuals.append(rNorm)
         solved = rNorm <= eps_stop
         tired = it >= itmax
+        if not np.isfinite(rNorm):
+            # Breakdown (a singular C^T V): the reference would iterate on
+            # NaN until itmax; the restatement stops here with status
+            # "unknown", as the device solver does.
+            tired = False
+            break
     if tired:
         stats.status = "maximum number of iterations exceeded"
     if solved:

@@ -63,22 +63,30 @@ def test_benchmark_flow_matches_cpu_restatement(bpf, oracle, name, precond):
         np.testing.assert_allclose(st.residuals[:k], stc.residuals[:k], rtol=1e-6, atol=1e-9 * r0)
         if not stc.solved:
             assert rn == pytest.approx(rnc, rel=1e-3)
-    if st.solved:
-        # ||B - A X||_F <= atol, up to the rounding of recomputing it
-        assert rn <= 1e-10 * 10
-        np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
-    else:
-        assert st.niter >= KW["itmax"]
-    if stc.solved:
-        assert rnc <= 1e-10 * 10
+    # every one of these runs converges, on the GPU as on the CPU, within
+    # the iterations measured on one MI355X (+2)
+    assert st.solved and stc.solved, (st, stc)
+    assert st.niter <= ITS[(name, precond)] + 2, st.niter
+    # ||B - A X||_F <= atol, up to the rounding of recomputing it
+    assert rn <= 1e-10 * 10
+    np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
+    assert rnc <= 1e-10 * 10
 
 
-@pytest.mark.parametrize("name", ["case300", "case1354pegase"])
+# block iterations to ||B - A X||_F <= 1e-10 on one MI355X (ILU(0) / none);
+# the bound allows two more (rank-deficient blocks are completed with
+# pseudo-random directions)
+ITS = {("case9", True): 3, ("case9", False): 3, ("case57", True): 8, ("case57", False): 9,
+       ("case118", True): 8, ("case118", False): 6, ("case300", True): 22, ("case1354pegase", True): 27,
+       ("case_ACTIVSg2000", True): 36, ("case2869pegase", True): 46}
+
+
+@pytest.mark.parametrize("name", ["case300", "case1354pegase", "case_ACTIVSg2000", "case2869pegase"])
 def test_benchmark_flow_many_rhs(bpf, oracle, name):
-    # p > 128 (137 and 519 right-hand sides): the library splits the columns
-    # into near-equal Krylov blocks of at most 128.  A run reported solved
-    # has a true residual at the tolerance and matches a direct solve; one
-    # that is not used all its iterations.
+    # p > 128 (137 to 1,019 right-hand sides): the library splits the columns
+    # into near-equal Krylov blocks of at most 128.  Each of these cases is
+    # solved within itmax = 50, with a true residual at the tolerance and the
+    # direct solution.
     Ah, Bh = _case(bpf, name)
     assert Bh.shape[1] > 128
     A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
@@ -87,12 +95,11 @@ def test_benchmark_flow_many_rhs(bpf, oracle, name):
     X, st = bpf.block_gmres(A, B, N=P, ldiv=True, **KW)
     As = sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape)
     Xg = X.to_numpy()
-    if st.solved:
-        assert np.linalg.norm(Bh - As @ Xg) <= 1e-10 * 10
-        Xd = oracle.direct_solve(Ah.indptr, Ah.indices, Ah.data, Bh)
-        np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
-    else:
-        assert st.niter >= KW["itmax"]
+    assert st.solved, st
+    assert st.niter <= ITS[(name, True)] + 2, st.niter
+    assert np.linalg.norm(Bh - As @ Xg) <= 1e-10 * 10
+    Xd = oracle.direct_solve(Ah.indptr, Ah.indices, Ah.data, Bh)
+    np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
 
 
 def test_breakdown_rule_is_the_same_overlapped_or_not(bpf):

@@ -39,7 +39,11 @@ def test_householder_ill_conditioned_and_rank_deficient(bpf):
     assert Qd.shifted                                           # reported as a breakdown
     Q, R = Qd.to_numpy(), Rd.to_numpy()
     assert np.all(np.isfinite(Q)) and np.all(np.isfinite(R))
-    assert np.linalg.norm(Q @ R - B) <= 1e-6 * np.linalg.norm(B)
+    # the dependent column is deflated or re-orthogonalized: Q stays
+    # orthonormal and Q R reproduces B to rounding
+    assert np.abs(Q.T @ Q - np.eye(p)).max() <= 1e-12
+    assert np.linalg.norm(Q @ R - B) <= 1e-12 * np.linalg.norm(B)
+    assert abs(R[5, 5]) <= 1e-8 * np.linalg.norm(B[:, 5])
     # exactly dependent columns (pivots at rounding level): Householder QR
     # completes Q with orthonormal directions and R has zero diagonal entries
     rng = np.random.default_rng(6)
@@ -52,6 +56,11 @@ def test_householder_ill_conditioned_and_rank_deficient(bpf):
     assert np.linalg.norm(Q @ R - C) <= 1e-12 * np.linalg.norm(C)
     assert R[7, 7] == 0.0
     assert np.all(np.isfinite(Q))
+    # the combination column: a pivot at rounding level of either sign is
+    # deflated (R[9, 9] = 0) or kept and re-projected; either way Q is
+    # orthonormal and R[9, 9] is at rounding level
+    assert np.abs(Q.T @ Q - np.eye(p)).max() <= 1e-12
+    assert abs(R[9, 9]) <= 1e-8 * np.linalg.norm(C[:, 9])
 
 
 def _hessenberg(rng, p, K, triangular_sub):

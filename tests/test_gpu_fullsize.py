@@ -58,8 +58,11 @@ def c2(gen, oracle):
 
 def test_c2_bicgstab_follows_reference_recurrence(bpf, oracle, c2):
     # Block BiCGSTAB with ILU(0) (src/block_bicgstab.jl:182-299) on the c2
-    # system: the GPU iterates follow the CPU restatement over the first 8
-    # iterations and both runs end with the same status after 40.
+    # system does not converge, on the CPU restatement as on the GPU: the
+    # residual swings between ~0.5x and ~5x its initial value, so rounding
+    # differences grow by orders of magnitude per step.  The GPU iterates
+    # follow the restatement over the first 5 iterations, and both runs are
+    # "tired" after 40.
     S, F, Xd = c2
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
     B = bpf.HipMatrix.from_numpy(S.B)
@@ -69,21 +72,31 @@ def test_c2_bicgstab_follows_reference_recurrence(bpf, oracle, c2):
     Xc, stc = oracle.block_bicgstab(S.scipy(), S.B, N=Nf, atol=0.0, rtol=1e-8, itmax=40, history=True)
     print(f"c2 bicgstab: gpu {st.status} {st.niter} its, final {st.residuals[-1]:.3e}; "
           f"cpu {stc.status} {stc.niter} its, final {stc.residuals[-1]:.3e}", flush=True)
-    np.testing.assert_allclose(st.residuals[:8], stc.residuals[:8], rtol=1e-6)
-    assert st.status == stc.status and st.solved == stc.solved
-    if st.solved:
-        assert np.all(_rel(X.to_numpy(), Xd) <= 1e-5)
+    np.testing.assert_allclose(st.residuals[:5], stc.residuals[:5], rtol=1e-6)
+    assert st.status == stc.status == "maximum number of iterations exceeded"
+    assert not st.solved and not stc.solved
 
 
-def test_c2_bicgstab_single_rhs_converges(bpf, oracle, c2):
-    # p = 1 on the same system converges to the direct solution
+def test_c2_bicgstab_single_rhs(bpf, oracle, c2):
+    # p = 1 on the same system: with the reference's |omega| the residual
+    # never drops below its initial value and C^T V eventually becomes
+    # singular (a non-finite residual ends the run with status "unknown", on
+    # the CPU restatement and on the GPU alike).
     S, F, Xd = c2
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
     B = bpf.HipMatrix.from_numpy(S.B[:, :1])
     P = bpf.ilu0(A)
-    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-12, itmax=3000)
-    assert st.solved, st
-    assert _rel(X.to_numpy(), Xd[:, :1]).max() <= 1e-5
+    X, st = bpf.block_bicgstab(A, B, N=P, atol=0.0, rtol=1e-12, itmax=3000, history=True)
+    Nf = lambda Z: oracle.ilu0_apply(S.indptr, S.indices, F, Z)
+    Xc, stc = oracle.block_bicgstab(S.scipy(), S.B[:, :1], N=Nf, atol=0.0, rtol=1e-12, itmax=3000,
+                                    history=True)
+    print(f"c2 bicgstab p=1: gpu {st.status} {st.niter} its; cpu {stc.status} {stc.niter} its", flush=True)
+    np.testing.assert_allclose(st.residuals[:10], stc.residuals[:10], rtol=1e-6)
+    assert not st.solved and not stc.solved
+    assert st.status in ("unknown", "maximum number of iterations exceeded")
+    assert st.status == stc.status
+    r = np.asarray(st.residuals)
+    assert np.nanmin(r[np.isfinite(r)]) >= 0.99 * r[0]
 
 
 def test_c2_gmres30(bpf, oracle, c2):

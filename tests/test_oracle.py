@@ -569,6 +569,35 @@ def test_ic0_apply_order(oracle):
     assert np.abs(wrong - ref).max() > 1e-6
 
 
+# ------------------------------------------------------------ block Jacobi
+def test_block_jacobi_one_part_is_exact_inverse(oracle):
+    M = _rand_csr(40, 0.1, 35)
+    X = np.random.default_rng(36).standard_normal((40, 3))
+    apply = oracle.block_jacobi(M.indptr, M.indices, M.data, np.zeros(40, int))
+    np.testing.assert_allclose(apply(X), np.linalg.solve(M.toarray(), X), rtol=1e-11, atol=1e-12)
+
+
+def test_block_jacobi_exact_on_block_diagonal(oracle):
+    rng = np.random.default_rng(37)
+    sizes = [5, 3, 7, 1]
+    D = sp.block_diag([sp.csr_matrix(rng.standard_normal((k, k)) + 4 * np.eye(k)) for k in sizes]).tocsr()
+    perm = rng.permutation(D.shape[0])        # parts need not be contiguous
+    Pm = sp.eye(D.shape[0]).tocsr()[perm]
+    A = (Pm @ D @ Pm.T).tocsr()
+    A.sort_indices()
+    part_d = np.repeat(np.arange(len(sizes)), sizes)
+    part = np.empty_like(part_d)
+    part[np.argsort(perm)] = part_d
+    X = rng.standard_normal((A.shape[0], 2))
+    apply = oracle.block_jacobi(A.indptr, A.indices, A.data, part)
+    np.testing.assert_allclose(apply(X), np.linalg.solve(A.toarray(), X), rtol=1e-11, atol=1e-12)
+    # dropping the coupling: on a matrix with off-block entries only the
+    # diagonal blocks count
+    E = A + sp.csr_matrix((np.ones(1), ([0], [(int(np.flatnonzero(part != part[0])[0]))])), shape=A.shape)
+    apply2 = oracle.block_jacobi(E.indptr, E.indices, E.data, part)
+    np.testing.assert_allclose(apply2(X), apply(X), rtol=1e-13)
+
+
 # ---------------------------------------------------- direct solve, berr
 def test_direct_solve_and_backward_error(oracle):
     M = _rand_csr(40, 0.1, 14)
