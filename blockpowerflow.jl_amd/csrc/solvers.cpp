@@ -265,9 +265,11 @@ void gmres_mixed(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *dB, in
   col_inf(ctx, n, p, blk.B.as<double>(), blk.stat, bnorm);
   ArnoldiConfig cfg;
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.flexible = true;
-  // FP32 basis: two classical passes keep it orthonormal to working precision.
+  // FP32 basis: two classical passes keep it orthonormal to working
+  // precision (bmgs_blocks > 1: block MGS over steps of that many blocks).
   cfg.f32 = true;
-  cfg.orth = ORTH_BCGS2;
+  cfg.orth = prm.bmgs_blocks > 1 ? ORTH_BMGS : ORTH_BCGS2;
+  cfg.bmgs_blocks = std::max(1, prm.bmgs_blocks);
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
   Poly poly;

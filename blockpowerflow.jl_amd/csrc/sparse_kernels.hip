@@ -40,6 +40,12 @@ inline void lane_shape(int p, int &glog, int &cpl) {
   }
 }
 
+// Rows of a row-major block start on cpl-element boundaries (vector loads).
+template <typename TV>
+inline bool vec_ok(const void *ptr, int ld, int cpl) {
+  return ld % cpl == 0 && reinterpret_cast<uintptr_t>(ptr) % (sizeof(TV) * cpl) == 0;
+}
+
 template <typename T> __device__ __forceinline__ T ld_val(const void *v, int k);
 template <> __device__ __forceinline__ double ld_val<double>(const void *v, int k) {
   return static_cast<const double *>(v)[k];
@@ -49,7 +55,27 @@ template <> __device__ __forceinline__ float ld_val<float>(const void *v, int k)
 }
 
 // ------------------------------------------------------------------ SpMM
-template <typename TA, typename TV, int CPL>
+// CPL elements of a row-major block, CPL consecutive columns of one row,
+// loaded or stored as one vector access.
+template <typename TV, int CPL>
+struct alignas(sizeof(TV) * CPL) VecT {
+  TV v[CPL];
+};
+
+// Column of element m of a lane: with VEC the lane owns CPL consecutive
+// columns (one vector load per gathered row); otherwise columns lane + m*G.
+template <int CPL, bool VEC>
+__device__ __forceinline__ int lane_col(int cbase, int lane, int m, int G) {
+  return VEC ? cbase + lane * CPL + m : cbase + lane + m * G;
+}
+
+// Y = alpha A X + beta Y.  Each row's non-zeros are taken eight at a time,
+// the last group padded (its column indices and values clamped to the row's
+// last entry, the padded products skipped): one round trip for the indices
+// and one for all gathers per eight non-zeros, the products added in k
+// order, so the result does not depend on the unrolling, the launch shape or
+// VEC.
+template <typename TA, typename TV, int CPL, bool VEC>
 __global__ __launch_bounds__(kBlock) void spmm_rm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TA *__restrict__ val, int p, const TV *__restrict__ X, int ldx,
@@ -64,48 +90,51 @@ __global__ __launch_bounds__(kBlock) void spmm_rm_kernel(
     TV acc[CPL];
 #pragma unroll
     for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
-    int k = k0;
-    // Eight non-zeros per step: their column indices and values first, then
-    // all 8 * CPL gathers in flight; products are added in k order, so the
-    // result does not depend on the unrolling or the launch shape.
-    for (; k + 7 < k1; k += 8) {
+    for (int k = k0; k < k1; k += 8) {
+      const int last = k1 - 1 - k;   // >= 0
       int j[8];
       TV a[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        j[u] = colidx[k + u];
-        a[u] = static_cast<TV>(val[k + u]);
+        const int kk = k + min(u, last);
+        j[u] = colidx[kk];
+        a[u] = static_cast<TV>(val[kk]);
       }
       TV xv[8][CPL];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const TV *xr = X + (size_t)j[u] * ldx;
+        if (VEC) {
+          const VecT<TV, CPL> w = *reinterpret_cast<const VecT<TV, CPL> *>(xr + cbase + lane * CPL);
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) {
-          const int c = cbase + lane + m * G;
-          xv[u][m] = c < p ? xr[c] : TV(0);
+          for (int m = 0; m < CPL; ++m) xv[u][m] = w.v[m];
+        } else {
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const int c = lane_col<CPL, VEC>(cbase, lane, m, G);
+            xv[u][m] = c < p ? xr[c] : TV(0);
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
+        if (u <= last)
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) acc[m] = fma(a[u], xv[u][m], acc[m]);
-    }
-    for (; k < k1; ++k) {
-      const int j = colidx[k];
-      const TV a = static_cast<TV>(val[k]);
-      const TV *xr = X + (size_t)j * ldx;
-#pragma unroll
-      for (int m = 0; m < CPL; ++m) {
-        const int c = cbase + lane + m * G;
-        if (c < p) acc[m] = fma(a, xr[c], acc[m]);
-      }
+          for (int m = 0; m < CPL; ++m) acc[m] = fma(a[u], xv[u][m], acc[m]);
     }
     TV *yr = Y + (size_t)row * ldy;
+    if (VEC) {
+      VecT<TV, CPL> w;
+      if (beta != TV(0)) w = *reinterpret_cast<const VecT<TV, CPL> *>(yr + cbase + lane * CPL);
 #pragma unroll
-    for (int m = 0; m < CPL; ++m) {
-      const int c = cbase + lane + m * G;
-      if (c < p) yr[c] = beta == TV(0) ? alpha * acc[m] : fma(beta, yr[c], alpha * acc[m]);
+      for (int m = 0; m < CPL; ++m) w.v[m] = beta == TV(0) ? alpha * acc[m] : fma(beta, w.v[m], alpha * acc[m]);
+      *reinterpret_cast<VecT<TV, CPL> *>(yr + cbase + lane * CPL) = w;
+    } else {
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = lane_col<CPL, VEC>(cbase, lane, m, G);
+        if (c < p) yr[c] = beta == TV(0) ? alpha * acc[m] : fma(beta, yr[c], alpha * acc[m]);
+      }
     }
   }
 }
@@ -134,8 +163,10 @@ struct TriDev {
 };
 
 // Solves one stored row t of a level-scheduled triangular operator for all
-// p columns owned by this lane group.
-template <typename TF, typename TV, bool UNIT, int CPL>
+// p columns owned by this lane group.  Non-zeros are taken eight at a time
+// as in the SpMM (the last group padded, padded products skipped), so the
+// products are added in k order whatever the shape.
+template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __device__ __forceinline__ void tri_row(const TriDev &a, int t, int lane, int G,
                                         const TV *x, int ldx, TV *y, int ldy, int p) {
   const int i = a.order[t];
@@ -145,93 +176,66 @@ __device__ __forceinline__ void tri_row(const TriDev &a, int t, int lane, int G,
   if (!UNIT) dinv = TV(1) / static_cast<TV>(static_cast<const TF *>(a.diag)[t]);
   for (int cbase = 0; cbase < p; cbase += G * CPL) {
     TV acc[CPL];
+    if (VEC) {
+      const VecT<TV, CPL> w = *reinterpret_cast<const VecT<TV, CPL> *>(x + (size_t)i * ldx + cbase + lane * CPL);
 #pragma unroll
-    for (int m = 0; m < CPL; ++m) {
-      const int c = cbase + lane + m * G;
-      acc[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
+      for (int m = 0; m < CPL; ++m) acc[m] = w.v[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = lane_col<CPL, VEC>(cbase, lane, m, G);
+        acc[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
+      }
     }
-    int k = k0;
-    for (; k + 7 < k1; k += 8) {        // eight non-zeros per step: 8 * CPL loads in flight
+    for (int k = k0; k < k1; k += 8) {
+      const int last = k1 - 1 - k;
       int j[8];
       TV v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        j[u] = a.col[k + u];
-        v[u] = static_cast<TV>(val[k + u]);
+        const int kk = k + min(u, last);
+        j[u] = a.col[kk];
+        v[u] = static_cast<TV>(val[kk]);
       }
       TV av[8][CPL];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const TV *yu = y + (size_t)j[u] * ldy;
+        if (VEC) {
+          const VecT<TV, CPL> w = *reinterpret_cast<const VecT<TV, CPL> *>(yu + cbase + lane * CPL);
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) {
-          const int c = cbase + lane + m * G;
-          av[u][m] = c < p ? yu[c] : TV(0);
+          for (int m = 0; m < CPL; ++m) av[u][m] = w.v[m];
+        } else {
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const int c = lane_col<CPL, VEC>(cbase, lane, m, G);
+            av[u][m] = c < p ? yu[c] : TV(0);
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
+        if (u <= last)
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) acc[m] = fma(-v[u], av[u][m], acc[m]);
+          for (int m = 0; m < CPL; ++m) acc[m] = fma(-v[u], av[u][m], acc[m]);
     }
-    for (; k + 3 < k1; k += 4) {        // four non-zeros per step: 4 * CPL loads in flight
-      const int j0 = a.col[k], j1 = a.col[k + 1], j2 = a.col[k + 2], j3 = a.col[k + 3];
-      const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + 1]);
-      const TV v2 = static_cast<TV>(val[k + 2]), v3 = static_cast<TV>(val[k + 3]);
-      const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
-      const TV *y2 = y + (size_t)j2 * ldy, *y3 = y + (size_t)j3 * ldy;
-      TV a0[CPL], a1[CPL], a2[CPL], a3[CPL];
+    if (VEC) {
+      VecT<TV, CPL> w;
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) w.v[m] = UNIT ? acc[m] : acc[m] * dinv;
+      *reinterpret_cast<VecT<TV, CPL> *>(y + (size_t)i * ldy + cbase + lane * CPL) = w;
+    } else {
 #pragma unroll
       for (int m = 0; m < CPL; ++m) {
-        const int c = cbase + lane + m * G;
-        const bool ok = c < p;
-        a0[m] = ok ? y0[c] : TV(0);
-        a1[m] = ok ? y1[c] : TV(0);
-        a2[m] = ok ? y2[c] : TV(0);
-        a3[m] = ok ? y3[c] : TV(0);
+        const int c = lane_col<CPL, VEC>(cbase, lane, m, G);
+        if (c < p) y[(size_t)i * ldy + c] = UNIT ? acc[m] : acc[m] * dinv;
       }
-#pragma unroll
-      for (int m = 0; m < CPL; ++m) {
-        acc[m] = fma(-v0, a0[m], acc[m]);
-        acc[m] = fma(-v1, a1[m], acc[m]);
-        acc[m] = fma(-v2, a2[m], acc[m]);
-        acc[m] = fma(-v3, a3[m], acc[m]);
-      }
-    }
-    for (; k + 1 < k1; k += 2) {        // two non-zeros per step (loads overlap)
-      const int j0 = a.col[k], j1 = a.col[k + 1];
-      const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + 1]);
-      const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
-#pragma unroll
-      for (int m = 0; m < CPL; ++m) {
-        const int c = cbase + lane + m * G;
-        if (c < p) {
-          const TV a0 = y0[c], a1 = y1[c];
-          acc[m] = fma(-v0, a0, acc[m]);
-          acc[m] = fma(-v1, a1, acc[m]);
-        }
-      }
-    }
-    if (k < k1) {
-      const int j = a.col[k];
-      const TV v = static_cast<TV>(val[k]);
-      const TV *yr = y + (size_t)j * ldy;
-#pragma unroll
-      for (int m = 0; m < CPL; ++m) {
-        const int c = cbase + lane + m * G;
-        if (c < p) acc[m] = fma(-v, yr[c], acc[m]);
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < CPL; ++m) {
-      const int c = cbase + lane + m * G;
-      if (c < p) y[(size_t)i * ldy + c] = UNIT ? acc[m] : acc[m] * dinv;
     }
   }
 }
 
 // One launch per wide level: rows [t0, t1) of the level-ordered list.
-template <typename TF, typename TV, bool UNIT, int CPL>
+template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __global__ __launch_bounds__(kBlock) void tri_level_kernel(TriDev a, int t0, int t1,
                                                            const TV *x, int ldx, TV *y,
                                                            int ldy, int p, int glog) {
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void tri_level_kernel(TriDev a, int t0, int
   const int rpb = kBlock >> glog;
   const int t = t0 + blockIdx.x * rpb + (threadIdx.x >> glog);
   if (t >= t1) return;
-  tri_row<TF, TV, UNIT, CPL>(a, t, threadIdx.x & (G - 1), G, x, ldx, y, ldy, p);
+  tri_row<TF, TV, UNIT, CPL, VEC>(a, t, threadIdx.x & (G - 1), G, x, ldx, y, ldy, p);
 }
 
 // ------------------------------------------------------------- sync-free
@@ -281,7 +285,7 @@ __device__ __forceinline__ void sf_publish(int *flags, int i, int epoch, int lan
   if (lane == 0) __hip_atomic_store(flags + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename TF, typename TV, bool UNIT, int CPL>
+template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __global__ __launch_bounds__(kBlock) void tri_syncfree_kernel(TriDev a, int n, const TV *x, int ldx, TV *y,
                                                               int ldy, int p, int *flags, int epoch,
                                                               int *status) {
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void tri_syncfree_kernel(TriDev a, int n, c
   if (t >= n) return;
   const int lane = threadIdx.x & 63;
   if (sf_wait(flags, a.col, a.ptr[t], a.ptr[t + 1], epoch, lane, status))
-    tri_row<TF, TV, UNIT, CPL>(a, t, lane, 64, x, ldx, y, ldy, p);
+    tri_row<TF, TV, UNIT, CPL, VEC>(a, t, lane, 64, x, ldx, y, ldy, p);
   sf_publish(flags, a.order[t], epoch, lane);
 }
 
@@ -301,7 +305,7 @@ constexpr int kTailBlock = 1024;
 // over several wavefronts; the partial sums meet in LDS.  Deep elimination
 // tails (one long row per level, as in fill-reducing LU orderings) then run
 // at the speed of the whole workgroup instead of one wavefront.
-template <typename TF, typename TV, bool UNIT, int CPL>
+template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, int l1,
                                                               const TV *x, int ldx, TV *y,
                                                               int ldy, int p, int glog) {
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
     const int rows = t1 - t0;
     if (!can_split || rows >= NW) {
       for (int t = t0 + gid; t < t1; t += ngroups)
-        tri_row<TF, TV, UNIT, CPL>(a, t, lane, G, x, ldx, y, ldy, p);
+        tri_row<TF, TV, UNIT, CPL, VEC>(a, t, lane, G, x, ldx, y, ldy, p);
     } else {
       const int wpr = NW / rows;          // wavefronts per row
       const int r = gid / wpr, sub = gid % wpr;
@@ -632,9 +636,18 @@ void spmm_shape(hipStream_t s, int n, const int *rowptr, const int *colidx, cons
                 int p, const TV *X, int ldx, TV *Y, int ldy, TV a, TV b, int glog, int cpl) {
   const int rpb = kBlock >> glog;
   dim3 grid((n + rpb - 1) / rpb);
-#define BPF_SPMM(TA, C)                                                                 \
-  hipLaunchKernelGGL((spmm_rm_kernel<TA, TV, C>), grid, dim3(kBlock), 0, s, n, rowptr,  \
-                     colidx, static_cast<const TA *>(val), p, X, ldx, Y, ldy, a, b, glog)
+  // cpl > 100: vector loads of cpl - 100 consecutive columns per lane
+  const bool vec = cpl > 100;
+  if (vec) cpl -= 100;
+#define BPF_SPMM(TA, C)                                                                                \
+  do {                                                                                                  \
+    if (vec)                                                                                            \
+      hipLaunchKernelGGL((spmm_rm_kernel<TA, TV, C, true>), grid, dim3(kBlock), 0, s, n, rowptr, colidx,  \
+                         static_cast<const TA *>(val), p, X, ldx, Y, ldy, a, b, glog);                  \
+    else                                                                                                \
+      hipLaunchKernelGGL((spmm_rm_kernel<TA, TV, C, false>), grid, dim3(kBlock), 0, s, n, rowptr, colidx, \
+                         static_cast<const TA *>(val), p, X, ldx, Y, ldy, a, b, glog);                  \
+  } while (0)
   if (val32) {
     if (cpl == 1) BPF_SPMM(float, 1); else if (cpl == 2) BPF_SPMM(float, 2); else BPF_SPMM(float, 4);
   } else {
@@ -669,6 +682,10 @@ std::pair<int, int> spmm_autotune(hipStream_t s, int n, const int *rowptr, const
     int glog = 0;
     while ((1 << glog) < g) ++glog;
     cand.push_back({glog, cpl});
+    // the same shape with vector loads when its lanes cover the columns
+    // exactly and the rows are aligned (the result is the same bit for bit)
+    if (cpl > 1 && g * cpl == p && vec_ok<TV>(X, ldx, cpl) && vec_ok<TV>(X, p, cpl))
+      cand.push_back({glog, cpl + 100});
   }
   if (cand.size() == 1) return cand[0];
   DevBuf scratch;
@@ -726,7 +743,11 @@ void launch_spmm(hipStream_t s, int n, int nnz, const int *rowptr, const int *co
     }
     glog = shape.first;
     cpl = shape.second;
+  } else if (cpl > 1 && (1 << glog) * cpl == p) {
+    cpl += 100;   // vector loads (checked against the operands below)
   }
+  // a vector shape needs rows on cpl-element boundaries in X and Y
+  if (cpl > 100 && !(vec_ok<TV>(X, ldx, cpl - 100) && vec_ok<TV>(Y, ldy, cpl - 100))) cpl -= 100;
   const int tok = prof::begin(s);
   spmm_shape<TV>(s, n, rowptr, colidx, val, val32, p, X, ldx, Y, ldy, static_cast<TV>(alpha), static_cast<TV>(beta),
                  glog, cpl);
@@ -791,8 +812,8 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
     const int epoch = ++op.epoch;
     dim3 grid((op.n + kBlock / 64 - 1) / (kBlock / 64));
     const int tok = prof::begin(s);
-#define BPF_SF(U, C)                                                                               \
-  hipLaunchKernelGGL((tri_syncfree_kernel<TV, TV, U, C>), grid, dim3(kBlock), 0, s, a, op.n, x, ldx, y, \
+#define BPF_SF(U, C)                                                                                      \
+  hipLaunchKernelGGL((tri_syncfree_kernel<TV, TV, U, C, false>), grid, dim3(kBlock), 0, s, a, op.n, x, ldx, y, \
                      ldy, p, op.flags.as<int>(), epoch, op.status.as<int>())
     if (op.unit) {
       if (cpl == 1) BPF_SF(true, 1); else if (cpl == 2) BPF_SF(true, 2); else BPF_SF(true, 4);
@@ -810,15 +831,24 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
   lane_shape(p, glog, cpl);
   const int rpb = kBlock >> glog;
   const auto plan = make_plan(op, 2 * (kTailBlock >> glog));
+  // Vector loads when the lanes of a row cover its columns exactly and the
+  // rows are aligned (each lane then owns cpl consecutive columns).
+  const bool vec = cpl > 1 && ((1 << glog) * cpl == p) && vec_ok<TV>(x, ldx, cpl) && vec_ok<TV>(y, ldy, cpl);
   for (const auto &seg : plan) {
     if (seg.wide) {
       for (int l = seg.l0; l < seg.l1; ++l) {
         const int t0 = op.h_lvl[l], t1 = op.h_lvl[l + 1];
         dim3 grid((t1 - t0 + rpb - 1) / rpb);
         const int tok = prof::begin(s);
-#define BPF_TRI(U, C)                                                                     \
-  hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C>), grid, dim3(kBlock), 0, s, a, t0, t1, \
-                     x, ldx, y, ldy, p, glog)
+#define BPF_TRI(U, C)                                                                                   \
+  do {                                                                                                   \
+    if (vec)                                                                                             \
+      hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, true>), grid, dim3(kBlock), 0, s, a, t0, t1, x,  \
+                         ldx, y, ldy, p, glog);                                                          \
+    else                                                                                                 \
+      hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, false>), grid, dim3(kBlock), 0, s, a, t0, t1, x, \
+                         ldx, y, ldy, p, glog);                                                          \
+  } while (0)
         if (op.unit) {
           if (cpl == 1) BPF_TRI(true, 1); else if (cpl == 2) BPF_TRI(true, 2); else BPF_TRI(true, 4);
         } else {
@@ -831,9 +861,15 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
       }
     } else {
       const int tok = prof::begin(s);
-#define BPF_TAIL(U, C)                                                                       \
-  hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0, \
-                     seg.l1, x, ldx, y, ldy, p, glog)
+#define BPF_TAIL(U, C)                                                                                      \
+  do {                                                                                                       \
+    if (vec)                                                                                                 \
+      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, true>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,   \
+                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
+    else                                                                                                     \
+      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, false>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,  \
+                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
+  } while (0)
       if (op.unit) {
         if (cpl == 1) BPF_TAIL(true, 1); else if (cpl == 2) BPF_TAIL(true, 2); else BPF_TAIL(true, 4);
       } else {
