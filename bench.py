@@ -266,8 +266,12 @@ def bench_single(args, bpf, gen):
         P.factor_(A)
         X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=args.memory, restart=args.restart, itmax=2000,
                                 timemax=120.0, berr_tol=BERR, precision=precision, block_width=bw,
-                                max_refine=8, poly_degree=args.poly_degree if precision != "fp64" else 0,
+                                max_refine=8,
+                                poly_degree=(args.poly_degree if precision == "mixed" else
+                                             (args.poly_degree or 4) if precision == "mixed32" else 0),
                                 bmgs_blocks=args.bmgs_blocks if precision == "mixed" else 1,
+                                inner_its=args.inner_its if precision == "mixed" else 0,
+                                inner_width=args.inner_width,
                                 reorthogonalization=args.orth == "bcgs2")
         be = bpf.backward_error(A, X, B)
         return (f"its {st.niter} max berr {be.max():.2e}", st, be)
@@ -302,7 +306,7 @@ def bench_single(args, bpf, gen):
         _, stm, bem = outm[-1]
         mixed32 = {"value": int(np.sum(bem <= BERR)) / dtm, "ms_per_step": dtm * 1e3, "niter": stm.niter,
                    "refinement_steps": stm.refine_steps, "max_berr": float(bem.max()),
-                   "form": "FP32 basis (two classical Gram-Schmidt passes), FP32 ILU0 + polynomial, "
+                   "form": "FP32 basis (two classical Gram-Schmidt passes), FP32 ILU0 + degree-4 polynomial, "
                            "FP64 Gram/Cholesky/Hessenberg, FP64 iterative refinement with the true residual"}
         log(f"[bench] mixed32: {dtm:.3f}s its {stm.niter} refinements {stm.refine_steps} berr {bem.max():.2e}")
     config = {
@@ -313,12 +317,19 @@ def bench_single(args, bpf, gen):
                         "does not converge on this system)")
                      + f", block width {s_mixed}, {args.orth}"
                      + (f" over {args.bmgs_blocks}-block steps" if args.orth == "bmgs" else "")
-                     + f"; precision split: FP64 basis, Gram-Schmidt, CholQR2, Hessenberg least squares, "
-                     f"outer SpMM and true residual, FP32 ILU0 triangular solves and the FP32 SpMMs of a "
-                     f"degree-{args.poly_degree} GMRES polynomial around them (the polynomial is an addition "
-                     f"that cuts the iterations; plain_fp64 is the reference's form), to per-RHS backward "
-                     f"error 1e-10"),
+                     + "; precision split: FP64 basis, Gram-Schmidt, CholQR2, Hessenberg least squares, "
+                     "outer SpMM and true residual; FP32 in the preconditioner step: "
+                     + (f"{args.inner_its} iterations of an inner FP32 block GMRES (groups of "
+                        f"{args.inner_width} columns, one classical Gram-Schmidt and one CholQR pass) with "
+                        "FP32 SpMM and FP32 ILU0 triangular solves (the inner solve is an addition that "
+                        "cuts the outer iterations; plain_fp64 is the reference's form)"
+                        if args.inner_its > 0 else
+                        f"FP32 ILU0 triangular solves and the FP32 SpMMs of a degree-{args.poly_degree} "
+                        "GMRES polynomial around them (an addition that cuts the iterations; plain_fp64 is "
+                        "the reference's form)")
+                     + ", to per-RHS backward error 1e-10"),
         "n": S.n, "nnz": S.nnz, "nrhs": p, "block_width": s_mixed, "poly_degree": args.poly_degree,
+        "inner_its": args.inner_its, "inner_width": args.inner_width,
         "orth": args.orth, "bmgs_blocks": args.bmgs_blocks, "restart": bool(args.restart),
         "iterations": st_last.niter, "refinement_steps": st_last.refine_steps,
         "max_backward_error": float(be_last.max()),
@@ -355,7 +366,8 @@ def bench_batch(args, bpf, gen):
     def step():
         _, st, solved = batch.solve(Bs, Xs, precision="mixed", berr_tol=BERR, memory=40, itmax=1000,
                                     timemax=120.0, poly_degree=args.poly_degree,
-                                    reorthogonalization=args.orth == "bcgs2", bmgs_blocks=args.bmgs_blocks)
+                                    reorthogonalization=args.orth == "bcgs2", bmgs_blocks=args.bmgs_blocks,
+                                    inner_its=args.inner_its, inner_width=args.inner_width)
         nsolved = int(sum(int(ps[i]) for i in range(nsys) if solved[i]))
         return (f"block its {st.niter} solved RHS {nsolved}/{nrhs} max berr {st.max_berr:.2e}", st, nsolved)
 
@@ -726,7 +738,12 @@ def main():
                          "(default 8 for c2/c3, 16 for c4; the plain FP64 form keeps the reference's "
                          "one-block loop)")
     ap.add_argument("--poly-degree", type=int, default=None,
-                    help="GMRES-polynomial degree of the mixed form (default 4 for c2/c3, 0 for c4)")
+                    help="GMRES-polynomial degree of the mixed form (default 0; used when --inner-its is 0)")
+    ap.add_argument("--inner-its", type=int, default=None,
+                    help="mixed form: iterations of the inner FP32 block GMRES that is the preconditioner "
+                         "step (default 8 for c2/c3, 6 for c4)")
+    ap.add_argument("--inner-width", type=int, default=16,
+                    help="columns per independent Krylov block of the inner solve (default 16)")
     ap.add_argument("--batch-threads", type=int, default=4,
                     help="c4: RHS-count bins solved at the same time, each from its own host thread")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
@@ -740,8 +757,10 @@ def main():
         copy_probe()
         return
     batched = args.workload == "c4"
+    if args.inner_its is None:
+        args.inner_its = 6 if batched else 8
     if args.poly_degree is None:
-        args.poly_degree = 0 if batched else 4
+        args.poly_degree = 0
     if args.bmgs_blocks is None:
         args.bmgs_blocks = 16 if batched else 8
     if args.gpus != 1:

@@ -433,7 +433,8 @@ def _fill_stats(stats, st: _lib.SolverStats, hist, history):
 
 def _params(memory=20, restart=False, reorthogonalization=False, atol=None, rtol=None, itmax=0,
             timemax=math.inf, verbose=0, history=False, berr_tol=0.0, precision="fp64",
-            block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0, bmgs_blocks=1):
+            block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0, bmgs_blocks=1, inner_its=0,
+            inner_width=0):
     prm = _lib.SolverParams()
     L.bpf_solver_params_default(C.byref(prm))
     eps = math.sqrt(np.finfo(np.float64).eps)
@@ -454,6 +455,8 @@ def _params(memory=20, restart=False, reorthogonalization=False, atol=None, rtol
     prm.inner_rtol = float(inner_rtol)
     prm.poly_degree = int(poly_degree)
     prm.bmgs_blocks = int(bmgs_blocks)
+    prm.inner_its = int(inner_its)
+    prm.inner_width = int(inner_width)
     return prm
 
 
@@ -521,7 +524,7 @@ def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, 
                  N=None, ldiv=False, restart=False, reorthogonalization=False, atol=None, rtol=None,
                  itmax=0, timemax=math.inf, verbose=0, history=False, berr_tol=0.0,
                  precision="fp64", block_width=0, max_refine=10, inner_rtol=1e-6, poly_degree=0,
-                 bmgs_blocks=1):
+                 bmgs_blocks=1, inner_its=0, inner_width=0):
     """block_gmres!(solver, A, B[, X0]; kwargs...) -> solver (src/block_gmres.jl:156-411)."""
     dt = _check_system(A, B, solver.X, X0, solver._X0 if solver.warm_start else None)
     tol = _default_tol(dt)
@@ -536,7 +539,7 @@ def block_gmres_(solver: BlockGMRESSolver, A: HipSparseMatrixCSR, B: HipMatrix, 
         solver.warm_start_(X0)
     prm = _params(solver.memory, restart, reorthogonalization, atol, rtol, itmax, timemax, verbose,
                   history, berr_tol, precision, block_width, max_refine, inner_rtol, poly_degree,
-                  bmgs_blocks)
+                  bmgs_blocks, inner_its, inner_width)
     st = _lib.SolverStats()
     cap = 1 + (itmax if itmax > 0 else 2 * (A.n // max(1, B.ncols))) + 16
     hist = np.zeros(cap)
@@ -561,9 +564,11 @@ def block_gmres(A: HipSparseMatrixCSR, B: HipMatrix, X0: HipMatrix | None = None
     """(X, stats) = block_gmres(A, B[, X0]; memory, M, N, ldiv, restart,
     reorthogonalization, atol, rtol, itmax, timemax, verbose, history)
     (src/block_gmres.jl:127-154).  Extra keywords: berr_tol, precision
-    ("fp64" | "mixed"), block_width, max_refine, inner_rtol, poly_degree,
-    bmgs_blocks (basis blocks projected together per block-MGS step; 1 = the
-    reference's one-block loop)."""
+    ("fp64" | "mixed" | "mixed32"), block_width, max_refine, inner_rtol,
+    poly_degree, bmgs_blocks (basis blocks projected together per block-MGS
+    step; 1 = the reference's one-block loop), inner_its / inner_width (mixed:
+    the preconditioner step is inner_its iterations of an FP32 block GMRES on
+    groups of inner_width columns)."""
     solver = BlockGMRESSolver(A, B, memory=memory)
     block_gmres_(solver, A, B, X0, **kw)
     return solver.X, solver.stats

@@ -30,6 +30,8 @@ class SolverParams(C.Structure):
         ("max_refine", C.c_int), ("inner_rtol", C.c_double), ("reorthogonalization", C.c_int),
         ("poly_degree", C.c_int),
         ("bmgs_blocks", C.c_int),
+        ("inner_its", C.c_int),
+        ("inner_width", C.c_int),
     ]
 
 

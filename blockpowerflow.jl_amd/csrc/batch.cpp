@@ -118,7 +118,8 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
   BPF_CHECK_HIP(hipMemcpyAsync(b.R.ptr, b.B.ptr, blk, hipMemcpyDeviceToDevice, s));
   const bool mixed = prm.precision == BPF_PRECISION_MIXED;
   const bool use_berr = prm.berr_tol > 0.0;
-  const int mode = mixed ? 1 : 0;
+  const int inner_its = mixed ? std::max(0, prm.inner_its) : 0;
+  const int mode = mixed ? 1 + 2 * inner_its : 0;
   if (!b.arn || b.arn_mode != mode) {
     ArnoldiConfig cfg;
     cfg.n = rows;
@@ -130,6 +131,7 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
     cfg.orth = prm.reorthogonalization ? ORTH_BCGS2 : ORTH_BMGS;
     cfg.bmgs_blocks = std::max(1, prm.bmgs_blocks);
     cfg.groups = b.groups;
+    cfg.inner_its = inner_its;
     b.arn.reset();
     b.arn.reset(new BlockArnoldi(ctx, b.A, b.N, cfg));
     b.arn_mode = mode;
@@ -137,7 +139,7 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
   BlockArnoldi &arn = *b.arn;
   arn.set_scale(b.scale.as<double>());
   Poly poly;
-  if (mixed && prm.poly_degree > 0) {
+  if (mixed && prm.poly_degree > 0 && inner_its == 0) {
     poly.build(ctx, b.A, b.N, prm.poly_degree);
     arn.set_poly(&poly);
   } else {

@@ -174,6 +174,7 @@ struct TriOp {
   // word set when a bounded dependency wait ran out.
   DevBuf flags, status;
   int epoch = 0;
+  int checked = 0;             // epoch of the last status check
 };
 
 // Process-wide switch between the sync-free single-launch triangular

@@ -753,6 +753,119 @@ __global__ __launch_bounds__(kB) void fill_deflated_kernel(TV *Q, int ld, const 
   }
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// Cholesky of a block of width w <= 16 (S = 16): one wavefront per group,
+// the same unblocked steps as chol_kernel's 16-column diagonal panel (so the
+// same results), the inverse by the same per-column recurrence; shifts,
+// deflation and flags as there.
+__global__ __launch_bounds__(64) void chol16_kernel(CholArgs a) {
+  constexpr int S = 16, LD = 17;
+  __shared__ double G[S * LD];
+  __shared__ double dinv[S];
+  __shared__ unsigned char dep[S];
+  const int g = blockIdx.x;
+  if (!a.groups[g].active) return;
+  const int w = a.groups[g].w;
+  const int lane = threadIdx.x;
+  const double *Gin = a.G + (size_t)g * a.gstride;
+  double *R = a.R + (size_t)g * a.gstride;
+  double *Ri = a.Rinv + (size_t)g * a.gstride;
+  double loc = 0.0;
+  for (int e = lane; e < w * w; e += 64) {
+    const double v = Gin[(e / w) * a.S + e % w];
+    loc += v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) loc += __shfl_xor(loc, off, 64);
+  const double gnorm = sqrt(loc);
+  double shift = 0.0;
+  bool small = false;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    for (int e = lane; e < S * LD; e += 64) {
+      const int r = e / LD, c = e % LD;
+      G[e] = (r < w && c < w) ? Gin[r * a.S + c] + (r == c ? shift : 0.0) : 0.0;
+    }
+    if (lane < S) dep[lane] = 0;
+    small = false;
+    bool fail = false;
+    wave_sync();
+    for (int k = 0; k < w; ++k) {
+      const double d = G[k * LD + k];
+      if (a.deflate && isfinite(d) && d <= 0.0) {
+        if (lane > k && lane < w) G[k * LD + lane] = 0.0;
+        wave_sync();
+        if (lane == 0) {
+          G[k * LD + k] = 1.0;
+          dep[k] = 1;
+        }
+        wave_sync();
+        continue;
+      }
+      if (!(d > 0.0) || !isfinite(d)) {
+        fail = true;
+        break;
+      }
+      if (a.deflate && d <= 1e-10 * Gin[k * a.S + k]) small = true;
+      const double rkk = sqrt(d);
+      if (lane > k && lane < w) G[k * LD + lane] /= rkk;
+      wave_sync();
+      if (lane == 0) G[k * LD + k] = rkk;
+      for (int e = lane; e < 256; e += 64) {
+        const int i = e >> 4, j = e & 15;
+        if (i > k && j >= i && j < w) G[i * LD + j] -= G[k * LD + i] * G[k * LD + j];
+      }
+      wave_sync();
+    }
+    if (!fail) break;
+    shift = shift == 0.0 ? 11.0 * ((double)a.n * w + (double)w * (w + 1)) * 1.1102230246251565e-16 * gnorm
+                         : shift * 100.0;
+    if (shift == 0.0) shift = 1e-300;
+    if (lane == 0) {
+      atomicOr(a.flags, 1);
+      atomicOr(a.flags + 1, 1);
+    }
+  }
+  if (lane < w) dinv[lane] = 1.0 / G[lane * LD + lane];
+  wave_sync();
+  if (lane < w) {
+    const int c = lane;
+    for (int r = c - 1; r >= 0; --r) {
+      double v = G[r * LD + c] * dinv[c];
+      for (int m = r + 1; m < c; ++m) v += G[r * LD + m] * G[c * LD + m];
+      G[c * LD + r] = -v * dinv[r];
+    }
+  }
+  wave_sync();
+  for (int e = lane; e < S * S; e += 64) {
+    const int r = e / S, c = e % S;
+    double rv, iv;
+    if (r < w && c < w) {
+      rv = c >= r && !dep[r] ? G[r * LD + c] : 0.0;
+      iv = c > r ? G[c * LD + r] : (c == r ? dinv[r] : 0.0);
+    } else {
+      rv = iv = (r == c) ? 1.0 : 0.0;
+    }
+    R[r * a.S + c] = rv;
+    Ri[r * a.S + c] = iv;
+  }
+  if (lane == 0) {
+    int dp = 0;
+    for (int k = 0; k < w; ++k) dp |= dep[k];
+    if (dp) {
+      atomicOr(a.flags, 2);
+      atomicOr(a.flags + 1, 2);
+    }
+    if (small) {
+      atomicOr(a.flags, 4);
+      atomicOr(a.flags + 1, 4);
+    }
+    if (a.deflate && a.gflags) a.gflags[g] = (dp ? 2 : 0) | (small ? 4 : 0);
+  }
+}
+
 __global__ __launch_bounds__(kB) void split_groups_kernel(const GroupDesc *groups, const int *gflags, int ng,
                                                           GroupDesc *out) {
   const int g = blockIdx.x * kB + threadIdx.x;
@@ -1532,6 +1645,11 @@ template void launch_block_gemm<float, double>(hipStream_t, GemmArgs, int, int);
 
 void launch_chol(hipStream_t s, CholArgs a, int ng) {
   if (a.S > 128) BPF_THROW(BPF_E_UNSUPPORTED, "block width %d too large for Cholesky", a.S);
+  if (a.S == 16) {
+    hipLaunchKernelGGL(chol16_kernel, dim3(ng), dim3(64), 0, s, a);
+    BPF_CHECK_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(chol_kernel, dim3(ng), dim3(1024), 0, s, a);
   BPF_CHECK_HIP(hipGetLastError());
 }

@@ -68,11 +68,14 @@ struct CholQrWork {
 // n_shift is the row count used in the shift of a failed pivot; w.flags[0]
 // gathers bit 0 (shift), bit 1 (deflation) and bit 2 (rounding-level pivot)
 // and is cleared by the caller.  reorth(W, groups) acts on the groups given.
+// With host_check false the host does not wait for the flag word: every
+// group takes the plain second pass (deflated columns keep their
+// pseudo-random completion without re-projection).
 using ReorthFn = std::function<void(void *, const GroupDesc *)>;
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
                     int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
-                    const ReorthFn &reorth = {});
+                    const ReorthFn &reorth = {}, bool host_check = true, int passes = 2);
 
 // Host-visible description of one Krylov run.
 struct ArnoldiConfig {
@@ -89,6 +92,17 @@ struct ArnoldiConfig {
   // Explicit groups (row ranges, widths, slots) for many systems stacked in
   // one block-diagonal operator; empty = column slices of width s.
   std::vector<GroupDesc> groups;
+  // Flexible runs with an FP64 basis: the preconditioner step is inner_its
+  // iterations of an FP32 block GMRES (ILU(0)-right-preconditioned, groups
+  // of inner_s columns, 0 = p) on A z = V_k, from z = 0 (0 = off).
+  int inner_its = 0;
+  int inner_s = 0;
+  // A run nested inside another one's preconditioner step: one stream, its
+  // own event and pinned buffer, no second-stream overlap.
+  bool nested = false;
+  // CholQR passes per orthonormalization: 2 (CholQR2) or 1 (nested runs,
+  // where the block only has to be near-orthonormal).
+  int cholqr_passes = 2;
 };
 enum { ORTH_BMGS = 0, ORTH_BCGS = 1, ORTH_BCGS2 = 2 };
 
@@ -135,6 +149,9 @@ class BlockArnoldi {
 
   const double *scale_ = nullptr;
   Poly *poly_ = nullptr;
+  std::unique_ptr<BlockArnoldi> inner_;   // nested FP32 block GMRES (inner_its > 0)
+  hipEvent_t own_ev_ = nullptr;           // nested runs: their own event ...
+  double *own_rn_ = nullptr;              // ... and pinned residual buffer
   bpf_ctx ctx_;
   bpf_csr A_;
   bpf_prec N_;

@@ -149,6 +149,10 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const do
   cfg.n = n; cfg.p = p; cfg.s = prm.block_width; cfg.f32 = false; cfg.flexible = mixed;
   cfg.prec32 = mixed;
   cfg.M = M;
+  if (mixed && N) {
+    cfg.inner_its = std::max(0, prm.inner_its);
+    cfg.inner_s = prm.inner_width;
+  }
   // Block MGS as in the reference (src/block_gmres.jl:301-304); with
   // reorthogonalization=true two classical passes.  (A single classical pass
   // loses orthogonality on these Jacobians and stalls the solve.)
@@ -157,7 +161,7 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const do
   BlockArnoldi arn(ctx, A, N, cfg);
   arn.set_scale(blk.scale.as<double>());
   Poly poly;
-  if (mixed && N && prm.poly_degree > 0) {
+  if (mixed && N && prm.poly_degree > 0 && cfg.inner_its == 0) {
     poly.build(ctx, A, N, prm.poly_degree);
     arn.set_poly(&poly);
   }
@@ -354,6 +358,8 @@ void bpf_solver_params_default(bpf_solver_params *prm) {
   prm->reorthogonalization = 0;
   prm->poly_degree = 0;
   prm->bmgs_blocks = 1;
+  prm->inner_its = 0;
+  prm->inner_width = 0;
 }
 
 int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, const double *d_B, int ldb,

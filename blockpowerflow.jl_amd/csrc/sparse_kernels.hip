@@ -786,7 +786,9 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
 std::atomic<bool> g_syncfree{false};
 
 void triop_check(TriOp &op, hipStream_t s) {
-  if (op.n <= 0) return;
+  // only a sync-free launch since the last check can have set the word
+  if (op.n <= 0 || op.epoch == op.checked) return;
+  op.checked = op.epoch;
   int st = 0;
   BPF_CHECK_HIP(hipMemcpyAsync(&st, op.status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   BPF_CHECK_HIP(hipStreamSynchronize(s));

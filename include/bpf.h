@@ -196,6 +196,11 @@ typedef struct {
                           this many consecutive basis blocks per step (one
                           projection and one update launch for them; <= 1 =
                           one block per step, the reference's loop)             */
+  int inner_its;       /* mixed: > 0 = the flexible preconditioner step is this
+                          many iterations of an FP32 ILU(0)-preconditioned block
+                          GMRES on A z = V_k (in place of the polynomial)       */
+  int inner_width;     /* mixed, inner_its > 0: columns per independent Krylov
+                          block of the inner solve (0 = all p)                  */
 } bpf_solver_params;
 
 typedef struct {

@@ -1,8 +1,9 @@
 """The benchmark configurations at their full sizes against the FP64 oracle.
 
 c3: 70,000 buses (n = 125,998), 128 RHS (8 N(0,1) + 120 unit injections),
-    mixed precision flexible block GMRES as the benchmark runs it, and the
-    plain FP64 drop-in on the same input;
+    mixed precision flexible block GMRES as the benchmark runs it (inner FP32
+    block GMRES preconditioner step), and the plain FP64 drop-in on the same
+    input;
 c2: 9,000 buses, 64 N(0,1) RHS, block BiCGSTAB and block GMRES(30) with ILU(0);
 c4: the 256-system batch (n log-uniform in [500, 20000], p in {8,16,32,64});
 c5: the 70,000-bus pattern, 3 Newton value sets, 256 RHS, fixed-pattern
@@ -36,7 +37,8 @@ def test_c3_full_size(bpf, oracle, c3, form):
     A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
     B = bpf.HipMatrix.from_numpy(S.B)
     P = bpf.ilu0(A, 128)
-    kw = dict(precision="mixed", poly_degree=4, bmgs_blocks=8, block_width=128) if form == "mixed" else {}
+    kw = (dict(precision="mixed", inner_its=8, inner_width=16, bmgs_blocks=8, block_width=128)
+          if form == "mixed" else {})
     X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, itmax=2000, timemax=120.0, berr_tol=BERR,
                             max_refine=8, **kw)
     Xh = X.to_numpy()

@@ -263,3 +263,42 @@ def test_ldiv_float32_block(bpf, gen, oracle):
     ref = oracle.ilu0_apply(S.indptr, S.indices, F, S.B.astype(np.float32).astype(np.float64))
     np.testing.assert_allclose(y32.to_numpy(), ref.astype(np.float32), rtol=1e-5,
                                atol=1e-5 * np.abs(ref).max())
+
+
+# ------------------------------------- inner FP32 block GMRES preconditioner
+@pytest.mark.parametrize("nbus,p,its,width", [(2000, 64, 8, 16), (9000, 128, 8, 16), (300, 16, 4, 0),
+                                              (2000, 48, 6, 32)])
+def test_block_gmres_inner_gmres_preconditioner(bpf, gen, oracle, nbus, p, its, width):
+    # Flexible block GMRES whose preconditioner step is `its` iterations of
+    # an FP32 ILU(0)-preconditioned block GMRES: the FP64 outer run still
+    # reaches the backward-error target and the direct solution, in fewer
+    # outer iterations than with ILU(0) alone.
+    S, A, B, Xd = _setup(bpf, gen, oracle, nbus, p, 46)
+    P = bpf.ilu0(A, p)
+    kw = dict(N=P, ldiv=True, memory=40, precision="mixed", berr_tol=1e-10, itmax=1000)
+    X, st = bpf.block_gmres(A, B, inner_its=its, inner_width=width, **kw)
+    Xh = X.to_numpy()
+    be = oracle.backward_error(S.indptr, S.indices, S.data, Xh, S.B)
+    assert st.solved and be.max() <= 1e-10, (st, be.max())
+    assert np.all(_rel(Xh, Xd) < 1e-5)
+    _, st0 = bpf.block_gmres(A, B, **kw)
+    assert st.niter < st0.niter, (st.niter, st0.niter)
+
+
+def test_inner_gmres_overlapped_is_exact(bpf, gen, oracle):
+    # The inner runs queue on the main stream only; the outer Hessenberg
+    # update overlapping the next iteration changes nothing, bit for bit.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 2000, 32, 47)
+    P = bpf.ilu0(A, 32)
+    out = {}
+    try:
+        for ovl in (0, 1):
+            bpf.set_option("overlap", ovl)
+            X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, precision="mixed", berr_tol=1e-10,
+                                    itmax=400, inner_its=6, inner_width=16, history=True)
+            out[ovl] = (X.to_numpy(), np.array(st.residuals), st.niter)
+    finally:
+        bpf.set_option("overlap", 1)
+    assert out[0][2] == out[1][2]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][0], out[0][0])
