@@ -54,7 +54,7 @@ WORKLOADS = {
 BERR = 1e-10
 PEAK_HBM = 8.0e12
 METRIC = "solved RHS/sec at 1e-10 backward error: mixed+batched vs plain FP64, 1 GPU"
-FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "dense_lu"]
+FAMILIES = ["spmm", "trsv", "proj", "gemm", "factor", "dense_lu", "proj16", "gemm16"]
 PEAK_FP64_MFMA = 78.6e12
 # family -> (kernels its timers cover, "mfma" for matrix-core kernels else
 # "hbm", stem of its counter summary profiles/<workload>_<stem>_traffic.md).
@@ -65,6 +65,9 @@ KERNELS = {
     "dense_lu": ("dlu_diag_kernel, dgemm_kernel, copy_block_kernel", "mfma", "dense_lu"),
     "proj": ("proj_partial_kernel or proj16_kernel, then proj_reduce_kernel", "mfma", "projection"),
     "gemm": ("block_gemm_kernel", "mfma", "block_gemm_kernel"),
+    "proj16": ("proj16_kernel or proj_partial_kernel (widths <= 32), then proj_reduce_kernel", "hbm",
+               "projection_narrow"),
+    "gemm16": ("block_gemm_kernel (widths <= 32)", "hbm", "block_gemm_narrow"),
 }
 
 
@@ -117,7 +120,8 @@ def roofline_from(fam, dt_prof, workload):
     for k in timed_fams:
         b, a, pk, u = family_roofline(fam, k)
         others[k] = {"kernel": KERNELS[k][0], "bound": b, "achieved": round(a, 2), "unit": u,
-                     "frac": round(a / pk, 4), "share": round(fam[k]["ms"] / (dt_prof * 1e3), 3)}
+                     "frac": round(a / pk, 4), "share": round(fam[k]["ms"] / (dt_prof * 1e3), 3),
+                     "traffic": measured_traffic(workload, KERNELS[k][2])}
     return {
         "bound": bound, "kernel": KERNELS[dom][0],
         "achieved": round(achieved, 2), "peak": peak, "unit": unit,
