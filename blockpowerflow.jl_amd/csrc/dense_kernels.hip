@@ -1610,7 +1610,8 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   // products are 2 n w^2 flops per (block, group).
   // Stacked groups: rows_sum rows of width S; column groups: n rows of ng*S.
   const double cells = a.cells >= 0 ? a.cells : a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * a.ng * S;
-  prof::end(tok, s, prof::PROJ, cells * ((a.nb) * sizeof(TVV) + sizeof(TW)), 2.0 * cells * S * a.nb);
+  prof::end(tok, s, S <= 32 ? prof::PROJ_NARROW : prof::PROJ, cells * ((a.nb) * sizeof(TVV) + sizeof(TW)),
+            2.0 * cells * S * a.nb);
 }
 template void launch_proj<double, double>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
 template void launch_proj<float, float>(hipStream_t, ProjArgs, int, double *, size_t, size_t, bool, DevBuf &);
@@ -1634,7 +1635,7 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   }
   BPF_CHECK_HIP(hipGetLastError());
   const double cells = a.cells >= 0 ? a.cells : a.rows_sum > 0 ? (double)a.rows_sum * S : (double)a.n * ng * S;
-  prof::end(tok, s, prof::GEMM,
+  prof::end(tok, s, S <= 32 ? prof::GEMM_NARROW : prof::GEMM,
             cells * (a.nb * sizeof(TVV) + (a.beta == 0.0 ? 1.0 : 2.0) * sizeof(TY)),
             2.0 * cells * S * a.nb);
 }

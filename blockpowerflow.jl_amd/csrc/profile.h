@@ -7,7 +7,12 @@
 namespace bpf {
 namespace prof {
 
-enum Family { SPMM = 0, TRSV = 1, PROJ = 2, GEMM = 3, FACTOR = 4, DENSE_LU = 5, NFAM = 6 };
+// PROJ / GEMM: block widths of 64 and more (matrix-core bound); PROJ_NARROW /
+// GEMM_NARROW: widths up to 32 (bandwidth bound), e.g. an inner solve's
+// 16-column groups.
+enum Family {
+  SPMM = 0, TRSV = 1, PROJ = 2, GEMM = 3, FACTOR = 4, DENSE_LU = 5, PROJ_NARROW = 6, GEMM_NARROW = 7, NFAM = 8
+};
 
 bool enabled();
 // Records a start event on s when profiling is on; returns a token for end().

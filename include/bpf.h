@@ -99,9 +99,12 @@ int bpf_timer_stop(bpf_ctx ctx, double *ms);
  * summed per family with their algorithmic bytes and flops:
  *   0 SpMM, 1 triangular-solve level launch, 2 block projection (V^T W),
  *   3 block update (V Psi), 4 sparse factorization, 5 dense-tail LU of the
- *   refactorization (its blocked factorization and its solves).
+ *   refactorization (its blocked factorization and its solves), 6 and 7 the
+ *   projections and updates of blocks at most 32 columns wide (2 and 3 count
+ *   widths of 64 and more).
  * Enabling (or disabling) resets the counters. */
-enum { BPF_K_SPMM = 0, BPF_K_TRSV = 1, BPF_K_PROJ = 2, BPF_K_GEMM = 3, BPF_K_FACTOR = 4, BPF_K_DENSE_LU = 5 };
+enum { BPF_K_SPMM = 0, BPF_K_TRSV = 1, BPF_K_PROJ = 2, BPF_K_GEMM = 3, BPF_K_FACTOR = 4, BPF_K_DENSE_LU = 5,
+       BPF_K_PROJ_NARROW = 6, BPF_K_GEMM_NARROW = 7 };
 int bpf_profile_enable(bpf_ctx ctx, int on);
 int bpf_profile_read(bpf_ctx ctx, int family, long long *launches, double *ms, double *bytes,
                      double *flops);
