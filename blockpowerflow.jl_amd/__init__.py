@@ -81,7 +81,7 @@ def default_context() -> Context:
     return _DEFAULT
 
 
-_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4}
+_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5}
 
 
 def set_option(name: str, value) -> None:
@@ -92,7 +92,12 @@ def set_option(name: str, value) -> None:
     "overlap" (default True) runs block GMRES's Hessenberg update of each
     iteration on a second stream beside the next iteration; "batch_threads"
     (default 4) is the number of RHS-count bins of a batched solve solved at
-    the same time, each from its own host thread on its own streams."""
+    the same time, each from its own host thread on its own streams;
+    "wide_min" (default 129) is the smallest number of right-hand sides that
+    FP64 block GMRES runs as one n x p Krylov block with its dense algebra on
+    rocBLAS / rocSOLVER, as the reference runs every p
+    (src/block_gmres.jl:46-56); narrower blocks run on the library's own
+    kernels."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbpf_amd.so")
 SOURCES = ["api.cpp", "profile.cpp", "analysis.cpp", "krylov.cpp", "poly.cpp", "solvers.cpp", "bicgstab.cpp",
-           "refactor.cpp", "ordering.cpp", "lu_host.cpp", "batch.cpp", "dense_api.cpp", "precond.cpp", "noalloc.hip", "block_jacobi.hip", "dense_lu.hip",
+           "refactor.cpp", "ordering.cpp", "lu_host.cpp", "batch.cpp", "dense_api.cpp", "precond.cpp", "noalloc.hip", "block_jacobi.hip", "dense_lu.hip", "wide.hip",
            "sparse_kernels.hip", "dense_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BPF_OFFLOAD_ARCH", "gfx950")
@@ -50,7 +50,10 @@ def build(jobs: int = 8, debug: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: _compile(s, debug), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    # rocBLAS / rocSOLVER: the dense algebra of the one-block wide solvers
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + [
+        "-L" + rocm_lib, "-lrocsolver", "-lrocblas", "-Wl,-rpath," + rocm_lib]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -149,6 +149,10 @@ int bpf_set_option(int option, long long value) {
     if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
     else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
     else if (option == BPF_OPT_OVERLAP) g_overlap_hess = value != 0;
+    else if (option == BPF_OPT_WIDE_MIN) {
+      BPF_REQUIRE(value >= 1, "wide_min must be at least 1");
+      g_wide_min = (int)std::min<long long>(value, 1 << 30);
+    }
     else if (option == BPF_OPT_BATCH_THREADS) {
       BPF_REQUIRE(value >= 1 && value <= 64, "batch threads must be in 1..64");
       g_batch_threads = (int)value;
@@ -163,6 +167,7 @@ int bpf_get_option(int option, long long *value) {
     else if (option == BPF_OPT_AUTOTUNE) *value = g_autotune ? 1 : 0;
     else if (option == BPF_OPT_OVERLAP) *value = g_overlap_hess ? 1 : 0;
     else if (option == BPF_OPT_BATCH_THREADS) *value = g_batch_threads;
+    else if (option == BPF_OPT_WIDE_MIN) *value = g_wide_min;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }
@@ -204,6 +209,7 @@ int bpf_ctx_destroy(bpf_ctx ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    blas_release(ctx);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_rn) (void)hipHostFree(ctx->h_rn);
     if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);

@@ -29,8 +29,12 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     BPF_REQUIRE(ldb >= A->n && ldx >= A->n, "leading dimension < n");
     if (d_C) BPF_REQUIRE(ldc >= A->n, "leading dimension of C < n");
     if (N) BPF_REQUIRE(N->n == A->n, "preconditioner size differs from n");
-    const int S = pad_width(p);
-    BPF_REQUIRE(S > 0, "block_bicgstab supports up to 128 right-hand sides per call (p = %d)", p);
+    // p > 128: the p x p algebra on rocBLAS / rocSOLVER, p x p matrices
+    // column-major
+    const bool wide = p > 128;
+    const int S = wide ? p : pad_width(p);
+    WideDense wd;
+    if (wide) wd.init(ctx, p);
     const double t0 = now_s();
     const int n = A->n;
     hipStream_t s = ctx->stream;
@@ -40,6 +44,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
     const size_t SS = (size_t)S * S;
     for (DevBuf *b : {&K, &D, &al, &be}) b->ensure(SS * sizeof(double));
     piv.ensure(S * sizeof(int));
+    BPF_REQUIRE(!wide || p <= n, "block_bicgstab: %d right-hand sides exceed the %d unknowns", p, n);
     dots.ensure((4 + kDotBlocks) * sizeof(double));
     std::vector<GroupDesc> hg{{0, p, 0, n, 0, 1 << 30, 1, 0}};
     upload(grp, hg, s);
@@ -61,12 +66,22 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
       return hd[0];
     };
     auto proj = [&](const double *Vm, const double *Wm, double *out) {   // out = Vm^T Wm
+      if (wide) return wd.proj(n, Vm, Wm, out);
       ProjArgs pa{nullptr, Vm, 0, 1, p, Wm, p, n, dg, 1, 0, nullptr};
       launch_proj<double, double>(s, pa, S, out, 0, SS, false, part);
     };
     auto gemm = [&](const double *Vm, const double *psi, double *Yout, double alpha, double beta) {
+      if (wide) return wd.update(n, Vm, psi, Yout, alpha, beta);
       GemmArgs ga{nullptr, Vm, 0, 1, p, psi, 0, SS, S, Yout, p, n, alpha, beta, dg};
       launch_block_gemm<double, double>(s, ga, S, 1);
+    };
+    auto lu = [&](double *Km) {
+      if (wide) return wd.lu(Km);
+      launch_lu_small(s, Km, piv.as<int>(), S, dg, 1);
+    };
+    auto lu_solve = [&](const double *Km, double *rhs) {
+      if (wide) return wd.solve(Km, rhs);
+      launch_lu_small_solve(s, Km, piv.as<int>(), rhs, S, dg, 1);
     };
     auto precond = [&](const double *in, double *out) {
       prec_apply_rm<double>(s, N, p, in, p, out, p);
@@ -92,8 +107,8 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
                           V.as<double>(), p, 1.0, 0.0);                      // :252
       proj(Cp, V.as<double>(), K.as<double>());                             // :253
       proj(Cp, R.as<double>(), al.as<double>());                            // :254-255
-      launch_lu_small(s, K.as<double>(), piv.as<int>(), S, dg, 1);          // :256
-      launch_lu_small_solve(s, K.as<double>(), piv.as<int>(), al.as<double>(), S, dg, 1);   // :257-258
+      lu(K.as<double>());                                                   // :256
+      lu_solve(K.as<double>(), al.as<double>());                            // :257-258
       BPF_CHECK_HIP(hipMemcpyAsync(U.ptr, R.ptr, mb, hipMemcpyDeviceToDevice, s));           // :259
       gemm(V.as<double>(), al.as<double>(), U.as<double>(), -1.0, 1.0);     // :260
       gemm(Yp, al.as<double>(), X.as<double>(), 1.0, 1.0);                  // :261-262
@@ -108,7 +123,7 @@ extern "C" int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p, con
       BPF_CHECK_HIP(hipMemcpyAsync(R.ptr, U.ptr, mb, hipMemcpyDeviceToDevice, s));
       axpby(s, m, -omega, Q.as<double>(), 1.0, R.as<double>());             // :267
       proj(Cp, Q.as<double>(), be.as<double>());                            // :268-269
-      launch_lu_small_solve(s, K.as<double>(), piv.as<int>(), be.as<double>(), S, dg, 1);   // :270-271
+      lu_solve(K.as<double>(), be.as<double>());                            // :270-271
       BPF_CHECK_HIP(hipMemcpyAsync(U.ptr, P.ptr, mb, hipMemcpyDeviceToDevice, s));
       axpby(s, m, -omega, V.as<double>(), 1.0, U.as<double>());             // :272
       BPF_CHECK_HIP(hipMemcpyAsync(P.ptr, R.ptr, mb, hipMemcpyDeviceToDevice, s));

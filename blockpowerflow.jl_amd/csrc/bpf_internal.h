@@ -137,6 +137,7 @@ struct bpf_ctx_s {
   double *h_rn = nullptr;
   size_t h_rn_cap = 0;
   void ensure_stream2(size_t ndouble);
+  void *blas = nullptr;       // rocblas_handle for the single-block wide solvers
 };
 
 struct bpf_csr_s {
@@ -185,6 +186,12 @@ extern std::atomic<bool> g_syncfree;
 extern std::atomic<bool> g_overlap_hess;
 // Process-wide number of host threads a batched solve runs its bins on.
 extern std::atomic<int> g_batch_threads;
+// Process-wide: block GMRES (FP64) with at least this many right-hand sides
+// in one Krylov block runs the single-block form on rocBLAS / rocSOLVER
+// (default 129: narrower blocks run on the library's own kernels).
+extern std::atomic<int> g_wide_min;
+// Releases the context's rocBLAS handle (created on first use).
+void blas_release(bpf_ctx ctx);
 // Process-wide switch for the SpMM launch-shape autotuner (default on), and
 // removal of a matrix's cached shapes (its row-pointer array is released).
 extern std::atomic<bool> g_autotune;

@@ -187,6 +187,26 @@ class BlockArnoldi {
   DevBuf Vf_, Zf_;             // FP32 copies for an FP32 preconditioner
 };
 
+// Block GMRES over one n x p Krylov block of any width (FP64; the dense
+// p x p algebra on rocBLAS / rocSOLVER), src/block_gmres.jl:170-411.
+void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const double *dB, int ldb,
+                const double *dX0, int ldx0, double *dX, int ldx, const bpf_solver_params &prm,
+                bpf_solver_stats &st, double *h_hist, int hist_cap);
+
+// p x p algebra of block BiCGSTAB for blocks wider than 128 on rocBLAS /
+// rocSOLVER: n x p blocks row-major, p x p matrices column-major.
+struct WideDense {
+  void *h = nullptr;           // rocblas_handle of the context
+  hipStream_t s = nullptr;
+  int p = 0;
+  DevBuf ipiv, info;
+  void init(bpf_ctx ctx, int p);
+  void proj(int n, const double *Vm, const double *Wm, double *out);   // out = Vm^T Wm
+  void update(int n, const double *Vm, const double *psi, double *Y, double alpha, double beta);  // Y = a Vm psi + b Y
+  void lu(double *K);                                                  // LU with partial pivoting
+  void solve(const double *K, double *rhs);                            // rhs = K^{-1} rhs
+};
+
 // Row-major FP64 residual R = B - A X.
 void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double *X, double *R);
 // Per-column backward error from row-major FP64 R, X, B; host output.

@@ -419,6 +419,9 @@ int bpf_block_gmres_ex(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int dtype
     } else if (prm->precision == BPF_PRECISION_MIXED32) {
       if (M) BPF_THROW(BPF_E_UNSUPPORTED, "left preconditioning is not available with precision MIXED32");
       gmres_mixed(ctx, A, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap);
+    } else if (p >= g_wide_min && (prm->block_width <= 0 || prm->block_width >= p)) {
+      // one n x p Krylov block, as the reference runs every width
+      gmres_wide(ctx, A, M, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap);
     } else {
       gmres_plain(ctx, A, M, N, p, B, lb, X0, lx0, X, lx, *prm, st, h_history, hist_cap);
     }

@@ -70,8 +70,18 @@ const char *bpf_version(void);
  * BPF_OPT_BATCH_THREADS: host threads that solve the bins of a batched solve
  * (systems grouped by RHS count) at the same time, each bin on its own pair
  * of streams (default 4; 1 = one bin after another).  Bins share nothing, so
- * results are identical for every value. */
-enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4 };
+ * results are identical for every value.
+ * BPF_OPT_WIDE_MIN: block GMRES in FP64 with at least this many right-hand
+ * sides (and block_width 0 or >= p) runs them as one n x p Krylov block, as
+ * the reference does for every p (src/block_gmres.jl:46-56): the sparse
+ * operator and preconditioners on the library's kernels, the p x p and
+ * 2p x p algebra (Gram-Schmidt products, Householder QR of the basis and
+ * Hessenberg blocks, reflector applications, back-substitution) on rocBLAS
+ * and rocSOLVER.  Default 129: up to 128 right-hand sides run on the
+ * library's own matrix-core kernels; block_width < p splits wider blocks
+ * into independent Krylov blocks of at most 128 columns there instead. */
+enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4,
+       BPF_OPT_WIDE_MIN = 5 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 
@@ -189,7 +199,9 @@ typedef struct {
   double berr_tol;     /* > 0: also stop when every RHS has backward error
                           ||Ax-b||inf/(||A||inf||x||inf+||b||inf) <= berr_tol   */
   int precision;       /* BPF_PRECISION_FP64 or BPF_PRECISION_MIXED             */
-  int block_width;     /* s: RHS per independent Krylov block (0 = p)           */
+  int block_width;     /* s: RHS per independent Krylov block (0 = p; blocks
+                          wider than 128 outside the one-block FP64 form are
+                          split into near-equal blocks of at most 128)         */
   int max_refine;      /* mixed: maximum FP64 refinement steps                  */
   double inner_rtol;   /* mixed: inner reduction factor per refinement step     */
   int reorthogonalization; /* second block Gram-Schmidt pass (reference kwarg) */
@@ -233,7 +245,9 @@ int bpf_block_gmres_x0(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
                        double *d_X, int ldx, const bpf_solver_params *prm,
                        bpf_solver_stats *stats, double *h_history, int hist_cap);
 /* block_bicgstab(A, B; C=B, N, atol, rtol, itmax, timemax, verbose, history)
- * (src/block_bicgstab.jl:182-299).  d_C may be NULL (C = B). */
+ * (src/block_bicgstab.jl:182-299).  d_C may be NULL (C = B).  Up to 128
+ * right-hand sides the p x p products and LU solves run on the library's
+ * kernels, beyond that on rocBLAS / rocSOLVER. */
 int bpf_block_bicgstab(bpf_ctx ctx, bpf_csr A, bpf_prec N, int p,
                        const double *d_B, int ldb, const double *d_C, int ldc,
                        double *d_X, int ldx, const bpf_solver_params *prm,
