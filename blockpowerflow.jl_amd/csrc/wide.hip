@@ -55,21 +55,9 @@ __global__ __launch_bounds__(256) void sub_kernel(int rows, int cols, const doub
 
 enum { SUB_COPY = 0, SUB_UPPER = 1, SUB_ZERO = 2, SUB_ADD = 3 };
 
-// Column 2-norms of an m x p column-major block, one workgroup per column.
-__global__ __launch_bounds__(256) void colnorm_kernel(int m, const double *Q, int ldq, double *out) {
-  const double *q = Q + (size_t)blockIdx.x * ldq;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < m; i += 256) acc += q[i] * q[i];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0] + red[1] + red[2] + red[3]);
-}
-
-// Sets *flag when the block was rank-deficient: a diagonal entry of its R at
-// rounding level against the norm of its column (r_jj^2 <= 1e-10 ||q_j||^2,
-// the rule of the CholQR2 path) or a zero column.
+// Sets *flag when the block is rank-deficient: a diagonal entry of its R at
+// rounding level against the norm of its column (r_jj^2 <= 1e-10 ||q_j||^2)
+// or a zero column.
 __global__ __launch_bounds__(256) void rank_flag_kernel(int p, const double *R, int ldr, const double *cn,
                                                         int *flag) {
   const int j = blockIdx.x * 256 + threadIdx.x;
@@ -103,21 +91,80 @@ double nrm2(rocblas_handle h, size_t count, const double *x) {
 
 // householder!(Q, R, tau) of src/block_utils.jl:46-53 on an m x p
 // column-major block: geqrf, R = its upper triangle, then orgqr unless
-// compact.  With flag set, a rank-deficient block sets *flag (cn: p
-// doubles of scratch).
+// compact.
 void householder(rocblas_handle h, hipStream_t s, int m, int p, double *Q, int ldq, double *R, int ldr, double *tau,
-                 bool compact, double *cn = nullptr, int *flag = nullptr) {
-  if (flag) {
-    hipLaunchKernelGGL(colnorm_kernel, dim3(p), dim3(256), 0, s, m, Q, ldq, cn);
-    BPF_CHECK_HIP(hipGetLastError());
-  }
+                 bool compact) {
   BPF_BLAS(rocsolver_dgeqrf(h, m, p, Q, ldq, tau));
   sub(s, p, p, Q, ldq, R, ldr, SUB_UPPER);
-  if (flag) {
-    hipLaunchKernelGGL(rank_flag_kernel, dim3((p + 255) / 256), dim3(256), 0, s, p, R, ldr, cn, flag);
-    BPF_CHECK_HIP(hipGetLastError());
-  }
   if (!compact) BPF_BLAS(rocsolver_dorgqr(h, m, p, p, Q, ldq, tau));
+}
+
+// r_jj of R1 against the column norms sqrt(G_jj) of the Gram matrix, which
+// potrf has overwritten above the diagonal only: d[j] = sqrt(G_jj) first.
+__global__ __launch_bounds__(256) void gram_diag_kernel(int p, const double *G, int ldg, double *d) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < p) d[j] = sqrt(G[(size_t)j * ldg + j]);
+}
+
+// Orthonormal basis of an n x p column-major block Q (in place) and its
+// p x p upper R (ld ldr), with Q_in = Q_out R.  CholQR2 (Gram matrix on
+// rocBLAS, Cholesky on rocSOLVER, triangular solve; twice, R = R2 R1) when
+// the block has full numerical rank: r_jj^2 > 1e-10 G_jj for every column
+// of the first Cholesky factor, as on the CholQR2 path of narrower blocks.
+// Otherwise householder! (src/block_utils.jl:46-53: geqrf, orgqr), the
+// reference's route, which completes a rank-deficient block with
+// orthonormal directions, from a copy of Q; then returns true.  Either way
+// span(Q_out) = span(Q_in) where Q_in has full rank, so the Krylov spaces and
+// the residual norms are the reference's; the factors may differ from
+// Householder's by column signs.
+struct BasisQr {
+  DevBuf save, G, G2, T, cn, tau, dinfo;
+  void init(BufPool *pool, int n, int p) {
+    for (DevBuf *b : {&save, &G, &G2, &T, &cn, &tau, &dinfo}) b->pool = pool;
+    save.ensure((size_t)n * p * sizeof(double));
+    for (DevBuf *b : {&G, &G2, &T}) b->ensure((size_t)p * p * sizeof(double));
+    cn.ensure(p * sizeof(double));
+    tau.ensure(p * sizeof(double));
+    dinfo.ensure(4 * sizeof(int));
+  }
+};
+
+bool basis_qr(rocblas_handle h, hipStream_t s, int n, int p, double *Q, double *R, int ldr, BasisQr &w) {
+  const double one = 1.0;
+  int *di = w.dinfo.as<int>();
+  int hi[2] = {0, 0};
+  BPF_CHECK_HIP(hipMemcpyAsync(w.save.ptr, Q, (size_t)n * p * sizeof(double), hipMemcpyDeviceToDevice, s));
+  BPF_CHECK_HIP(hipMemsetAsync(di, 0, 4 * sizeof(int), s));
+  gemm(h, true, false, p, p, n, 1.0, Q, n, Q, n, 0.0, w.G.as<double>(), p);
+  hipLaunchKernelGGL(gram_diag_kernel, dim3((p + 255) / 256), dim3(256), 0, s, p, w.G.as<double>(), p,
+                     w.cn.as<double>());
+  BPF_CHECK_HIP(hipGetLastError());
+  BPF_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, p, w.G.as<double>(), p, di));
+  hipLaunchKernelGGL(rank_flag_kernel, dim3((p + 255) / 256), dim3(256), 0, s, p, w.G.as<double>(), p,
+                     w.cn.as<double>(), di + 1);
+  BPF_CHECK_HIP(hipGetLastError());
+  BPF_CHECK_HIP(hipMemcpyAsync(hi, di, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  BPF_CHECK_HIP(hipStreamSynchronize(s));
+  if (hi[0] == 0 && hi[1] == 0) {
+    BPF_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, n, p, &one, w.G.as<double>(), p, Q, n));
+    gemm(h, true, false, p, p, n, 1.0, Q, n, Q, n, 0.0, w.G2.as<double>(), p);
+    BPF_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, p, w.G2.as<double>(), p, di + 2));
+    BPF_CHECK_HIP(hipMemcpyAsync(hi, di + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    BPF_CHECK_HIP(hipStreamSynchronize(s));
+    if (hi[0] == 0) {
+      BPF_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, n, p, &one, w.G2.as<double>(), p, Q, n));
+      sub(s, p, p, w.G.as<double>(), p, w.T.as<double>(), p, SUB_UPPER);
+      BPF_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, p, p, &one, w.G2.as<double>(), p, w.T.as<double>(), p, R,
+                             ldr));
+      return false;
+    }
+    BPF_CHECK_HIP(hipMemcpyAsync(Q, w.save.ptr, (size_t)n * p * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  householder(h, s, n, p, Q, n, R, ldr, w.tau.as<double>(), false);
+  return true;
 }
 
 struct HistoryWriter {
@@ -168,13 +215,13 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
   BufPool *pool = &ctx->pool;
   const size_t nb = (size_t)n * p, bb = nb * sizeof(double), pb = (size_t)p * p * sizeof(double);
   DevBuf Brm(pool), Xrm(pool), Rrm(pool), T1(pool), T2(pool), Q(pool), W0(pool), stat(pool);
-  DevBuf tauQ(pool), C(pool), D(pool), Cb(pool), Psi(pool), cn(pool), flag(pool);
-  cn.ensure(p * sizeof(double));
-  flag.ensure(sizeof(int));
+  DevBuf tauH(pool), C(pool), D(pool), Cb(pool), Psi(pool);
+  BasisQr bq;
+  bq.init(pool, n, p);
   for (DevBuf *b : {&Brm, &Xrm, &Rrm, &T1, &T2, &Q, &W0}) b->ensure(bb);
   for (DevBuf *b : {&C, &Cb, &Psi}) b->ensure(pb);
   D.ensure(2 * pb);
-  tauQ.ensure(p * sizeof(double));
+  tauH.ensure(p * sizeof(double));
   launch_cm_to_rm(s, n, p, dB, ldb, Brm.ptr, p, false);
   if (dX0) launch_cm_to_rm(s, n, p, dX0, ldx0, Xrm.ptr, p, false);
   else fill_double(s, Xrm.as<double>(), 0.0, nb);
@@ -208,7 +255,10 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
   int iter = 0, inner_itmax = itmax, npass = 0;
   bool solved = use_berr ? berr_max() <= prm.berr_tol : RNorm <= eps;
   bool tired = iter >= itmax, overtimed = false;
-  std::vector<DevBuf> V, Rc, Hq, tauH, Z;
+  // V: basis blocks; Rc: block columns of R; Om: the orthogonal 2p x 2p
+  // factors Omega_k of the Hessenberg QR, formed explicitly so that each
+  // later application is one matrix product; Z: blocks of Z.
+  std::vector<DevBuf> V, Rc, Om, Z;
   bool have_w0 = true;
   auto fresh = [&](std::vector<DevBuf> &v, size_t bytes) {
     v.emplace_back(pool);
@@ -216,15 +266,14 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
     return v.back().as<double>();
   };
   while (!(solved || tired || overtimed)) {
-    V.clear(); Rc.clear(); Hq.clear(); tauH.clear(); Z.clear();
+    V.clear(); Rc.clear(); Om.clear(); Z.clear();
     if (!have_w0) initial_residual();                                      // :266-271
     have_w0 = false;
-    BPF_CHECK_HIP(hipMemsetAsync(flag.ptr, 0, sizeof(int), s));
     // V_1 and Gamma = Z_1: householder!(V[1], Z[1], tau[1]) (:274-276)
     double *V0 = fresh(V, bb);
     BPF_CHECK_HIP(hipMemcpyAsync(V0, W0.ptr, bb, hipMemcpyDeviceToDevice, s));
     double *Z0 = fresh(Z, pb);
-    householder(h, s, n, p, V0, n, Z0, p, tauQ.as<double>(), false, cn.as<double>(), flag.as<int>());
+    bool rank_deficient = basis_qr(h, s, n, p, V0, Z0, p, bq);
     ++npass;
     int inner = 0;
     bool inner_tired = false, pass_solved = false;
@@ -262,24 +311,25 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
           sub(s, p, p, Psi.as<double>(), p, Rik, ldr, SUB_ADD);
         }
       }
-      householder(h, s, n, p, Qp, n, C.as<double>(), p, tauQ.as<double>(), false, cn.as<double>(),
-                  flag.as<int>());                                          // :316
-      for (int i = 0; i + 1 <= k; ++i)                                      // :320-326
-        BPF_BLAS(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, 2 * p, p, p, Hq[i].as<double>(),
-                                  2 * p, tauH[i].as<double>(), Rk + (size_t)i * p, ldr));
-      // Omega_k: QR of [Psi_kk; C] (:329-331)
-      double *Hk = fresh(Hq, 2 * pb);
-      double *tk = fresh(tauH, p * sizeof(double));
-      double *Rkk = Rk + (size_t)k * p;
-      sub(s, p, p, Rkk, ldr, Hk, 2 * p, SUB_COPY);
-      sub(s, p, p, C.as<double>(), p, Hk + p, 2 * p, SUB_COPY);
-      householder(h, s, 2 * p, p, Hk, 2 * p, Rkk, ldr, tk, true);
-      // [Z_k; Lambda] = Omega_k^T [Z_k; 0] (:334-337)
+      const bool hh = basis_qr(h, s, n, p, Qp, C.as<double>(), p, bq);      // :316
+      rank_deficient |= hh;
+      // previous reflections: [Psi_ik; Psi_i+1,k] = Omega_i^T [...] (:320-326)
       double *Dp = D.as<double>();
-      sub(s, p, p, Z[k].as<double>(), p, Dp, 2 * p, SUB_COPY);
-      sub(s, p, p, nullptr, 0, Dp + p, 2 * p, SUB_ZERO);
-      BPF_BLAS(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, 2 * p, p, p, Hk, 2 * p, tk, Dp,
-                                2 * p));
+      for (int i = 0; i + 1 <= k; ++i) {
+        gemm(h, true, false, 2 * p, p, 2 * p, 1.0, Om[i].as<double>(), 2 * p, Rk + (size_t)i * p, ldr, 0.0, Dp,
+             2 * p);
+        sub(s, 2 * p, p, Dp, 2 * p, Rk + (size_t)i * p, ldr, SUB_COPY);
+      }
+      // Omega_k: Householder QR of [Psi_kk; C] (:329-331), then Omega_k
+      // itself (2p x 2p) from its reflectors
+      double *Omk = fresh(Om, 4 * pb);
+      double *Rkk = Rk + (size_t)k * p;
+      sub(s, p, p, Rkk, ldr, Omk, 2 * p, SUB_COPY);
+      sub(s, p, p, C.as<double>(), p, Omk + p, 2 * p, SUB_COPY);
+      householder(h, s, 2 * p, p, Omk, 2 * p, Rkk, ldr, tauH.as<double>(), true);
+      BPF_BLAS(rocsolver_dorgqr(h, 2 * p, 2 * p, p, Omk, 2 * p, tauH.as<double>()));
+      // [Z_k; Lambda] = Omega_k^T [Z_k; 0] (:334-337)
+      gemm(h, true, false, 2 * p, p, p, 1.0, Omk, 2 * p, Z[k].as<double>(), p, 0.0, Dp, 2 * p);
       sub(s, p, p, Dp, 2 * p, Z[k].as<double>(), p, SUB_COPY);
       sub(s, p, p, Dp + p, 2 * p, Cb.as<double>(), p, SUB_COPY);
       RNorm = nrm2(h, (size_t)p * p, Cb.as<double>());                    // :341-342
@@ -288,7 +338,8 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
       inner_tired = prm.restart ? inner >= std::min(mem, inner_itmax) : inner >= inner_itmax;
       if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
       if (prm.verbose > 0 && (iter + inner) % prm.verbose == 0)
-        printf("%5d  %5d  %7.1e  %.2fs\n", npass, iter + inner, RNorm, now_s() - t0);
+        printf("%5d  %5d  %7.1e  %.2fs%s\n", npass, iter + inner, RNorm, now_s() - t0,
+               hh ? "  (rank-deficient block: Householder QR)" : "");
       if (!(pass_solved || inner_tired || overtimed)) {                    // :356-363
         V.emplace_back(std::move(Q));
         Q = DevBuf(pool);
@@ -323,13 +374,10 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
     iter += inner;
     tired = iter >= itmax;
     if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
-    int h_flag = 0;
-    BPF_CHECK_HIP(hipMemcpyAsync(&h_flag, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-    BPF_CHECK_HIP(hipStreamSynchronize(s));
     if (use_berr) {
       st.max_berr = berr_max();
       solved = st.max_berr <= prm.berr_tol;
-    } else if (h_flag) {
+    } else if (rank_deficient) {
       initial_residual();
       have_w0 = true;
       solved = nrm2(h, nb, W0.as<double>()) <= eps;
