@@ -86,7 +86,7 @@ def family_stats(bpf, ctx):
 
 
 def log_families(fam):
-    for k, v in fam.items():
+    for k, v in (fam or {}).items():
         if v["launches"]:
             log(f"[bench]   {k:6s} launches {v['launches']:7d} time {v['ms']:9.2f} ms  "
                 f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s  "
@@ -112,6 +112,8 @@ def roofline_from(fam, dt_prof, workload):
     timed steps) from the in-process kernel timers (HIP events around every
     launch on the solver's stream during the timed steps).  The other
     families' figures are listed under "families"."""
+    if fam is None:
+        return None
     timed_fams = [k for k in KERNELS if fam[k]["launches"]]
     dom = max(timed_fams, key=lambda k: fam[k]["ms"])
     d = fam[dom]
@@ -131,6 +133,9 @@ def roofline_from(fam, dt_prof, workload):
     }
 
 
+NO_KERNEL_TIMERS = False
+
+
 def timed(bpf, ctx, step, steps, warmup, label, profile=True):
     """warmup untimed steps, then `steps` timed steps with the kernel timers
     off; with profile=True one more, untimed step runs with the kernel timers
@@ -148,7 +153,7 @@ def timed(bpf, ctx, step, steps, warmup, label, profile=True):
         out.append(step())
     ctx.synchronize()
     dt = time.perf_counter() - t
-    if not profile:
+    if not profile or NO_KERNEL_TIMERS:
         return dt, out, None, None
     try:
         bpf._lib.check(bpf.L.bpf_profile_enable(ctx.handle, 1))
@@ -621,6 +626,9 @@ def bench_cases(args, bpf, gen):
                 ctx.synchronize()
                 r[key] = (time.perf_counter() - t, X, st)
             rows.append(r)
+            # one line per case: the larger cases take seconds each
+            log(f"[bench]   {c['name']}: " + ", ".join(
+                f"{k} {v[2].niter} its {v[0] * 1e3:.1f} ms" for k, v in r.items()))
         ns = sum(c["Bh"].shape[1] for c, r in zip(cases, rows) if r["ilu0"][2].solved)
         return (f"{ns} RHS solved with ILU(0)", rows)
 
@@ -753,10 +761,14 @@ def main():
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
                     help="block Gram-Schmidt: bmgs (the reference's loop, over --bmgs-blocks steps) or "
                          "bcgs2 (two classical passes)")
+    ap.add_argument("--no-kernel-timers", action="store_true",
+                    help="skip the extra kernel-timer step (no roofline families; for kernel-trace runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three device-to-device copies of 1 GiB (the known-size kernel that "
                          "scales the HBM byte counters) and exit")
     args = ap.parse_args()
+    global NO_KERNEL_TIMERS
+    NO_KERNEL_TIMERS = args.no_kernel_timers
     if args.copy_probe:
         copy_probe()
         return

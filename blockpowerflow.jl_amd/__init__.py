@@ -81,7 +81,8 @@ def default_context() -> Context:
     return _DEFAULT
 
 
-_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5}
+_OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5, "graph": 6,
+            "graph_replays": 7}
 
 
 def set_option(name: str, value) -> None:
@@ -97,7 +98,10 @@ def set_option(name: str, value) -> None:
     FP64 block GMRES runs as one n x p Krylov block with its dense algebra on
     rocBLAS / rocSOLVER, as the reference runs every p
     (src/block_gmres.jl:46-56); narrower blocks run on the library's own
-    kernels."""
+    kernels; "graph" (default True) captures the mixed form's inner FP32
+    block GMRES run (its fixed chain of inner_its iterations) into a HIP graph
+    on its first call and replays it on later outer iterations;
+    "graph_replays" counts those replays (reset with 0)."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -149,6 +149,11 @@ int bpf_set_option(int option, long long value) {
     if (option == BPF_OPT_SYNCFREE) g_syncfree = value != 0;
     else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
     else if (option == BPF_OPT_OVERLAP) g_overlap_hess = value != 0;
+    else if (option == BPF_OPT_GRAPH) g_graph = value != 0;
+    else if (option == BPF_OPT_GRAPH_REPLAYS) {
+      BPF_REQUIRE(value == 0, "graph_replays can only be reset to 0");
+      g_graph_replays = 0;
+    }
     else if (option == BPF_OPT_WIDE_MIN) {
       BPF_REQUIRE(value >= 1, "wide_min must be at least 1");
       g_wide_min = (int)std::min<long long>(value, 1 << 30);
@@ -168,6 +173,8 @@ int bpf_get_option(int option, long long *value) {
     else if (option == BPF_OPT_OVERLAP) *value = g_overlap_hess ? 1 : 0;
     else if (option == BPF_OPT_BATCH_THREADS) *value = g_batch_threads;
     else if (option == BPF_OPT_WIDE_MIN) *value = g_wide_min;
+    else if (option == BPF_OPT_GRAPH) *value = g_graph ? 1 : 0;
+    else if (option == BPF_OPT_GRAPH_REPLAYS) *value = g_graph_replays;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
 }

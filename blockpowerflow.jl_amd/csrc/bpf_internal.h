@@ -184,6 +184,11 @@ extern std::atomic<bool> g_syncfree;
 // Process-wide switch: block-Hessenberg updates on a second stream, beside
 // the next iteration (default on).
 extern std::atomic<bool> g_overlap_hess;
+// Process-wide switch: the inner runs of the mixed form's preconditioner step
+// are captured into a HIP graph and replayed (default on), and the number of
+// replays so far.
+extern std::atomic<bool> g_graph;
+extern std::atomic<long long> g_graph_replays;
 // Process-wide number of host threads a batched solve runs its bins on.
 extern std::atomic<int> g_batch_threads;
 // Process-wide: block GMRES (FP64) with at least this many right-hand sides

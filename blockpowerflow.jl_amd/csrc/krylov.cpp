@@ -12,6 +12,7 @@
 //   residual estimate ||Lambda||        (:341-342, per column here)
 //   block back substitution, X += Z Y   (:366-385)
 #include "krylov.h"
+#include "profile.h"
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +21,8 @@
 namespace bpf {
 
 std::atomic<bool> g_overlap_hess{true};
+std::atomic<bool> g_graph{true};
+std::atomic<long long> g_graph_replays{0};
 constexpr int kOverlapMaxIt = 1 << 16;
 
 void *BlockArena::block(int i) {
@@ -152,6 +155,10 @@ void BlockArnoldi::upload_groups() { upload(d_groups_, h_groups_, st_); }
 
 BlockArnoldi::~BlockArnoldi() {
   // the second stream, its events and h_rn_ belong to the context
+  if (gexec_) {
+    (void)hipStreamSynchronize(st_);
+    (void)hipGraphExecDestroy(gexec_);
+  }
   if (hs_) (void)hipStreamSynchronize(hs_);
   if (h_fl_ || own_rn_ || own_ev_) (void)hipStreamSynchronize(st_);
   if (h_fl_) (void)hipHostFree(h_fl_);
@@ -374,7 +381,7 @@ void BlockArnoldi::form(double *X, double alpha) {
     prec_apply_rm<TV>(st_, N_, p, P, p, P, p);
     Xtmp_.ensure((size_t)n * p * sizeof(double));
     copy_cast<TV, double>(st_, n, p, P, p, Xtmp_.as<double>(), p, 1.0);
-    axpby(st_, (size_t)n * p, alpha, Xtmp_.as<double>(), 1.0, X);
+    if (!staged_) axpby(st_, (size_t)n * p, alpha, Xtmp_.as<double>(), 1.0, X);
   }
 }
 
@@ -390,6 +397,61 @@ int BlockArnoldi::run(const double *R0, double *X, int maxit, const StopFn &stop
                   : run_t<double>(R0, X, maxit, stop, rnorm_out);
 }
 
+void BlockArnoldi::precondition(const double *V, double *Z, int maxit) {
+  BPF_REQUIRE(cfg_.nested, "precondition: not a nested run");
+  const int n = cfg_.n, p = cfg_.p;
+  const StopFn never = [](const double *, int) { return false; };
+  // The kernel timers record events around launches: no capture then.
+  if (!g_graph || gfail_ || prof::enabled()) {
+    fill_double(st_, Z, 0.0, (size_t)n * p);
+    run(V, Z, maxit, never, nullptr);
+    return;
+  }
+  if (cfg_.f32) copy_cast<double, float>(st_, n, p, V, p, W_.as<float>(), p, 1.0);
+  else copy_cast<double, double>(st_, n, p, V, p, W_.as<double>(), p, 1.0);
+  struct Staged {
+    bool &f;
+    explicit Staged(bool &x) : f(x) { f = true; }
+    ~Staged() { f = false; }
+  } staged(staged_);
+  if (gexec_ && gits_ == maxit) {
+    BPF_CHECK_HIP(hipGraphLaunch(gexec_, st_));
+    ++g_graph_replays;
+  } else {
+    if (gexec_) {
+      BPF_CHECK_HIP(hipStreamSynchronize(st_));
+      (void)hipGraphExecDestroy(gexec_);
+      gexec_ = nullptr;
+    }
+    // One eager run computes this call's correction and sizes every buffer,
+    // pointer table and SpMM launch shape, so the run captured next records
+    // kernel launches only.
+    run(nullptr, nullptr, maxit, never, nullptr);
+    hipGraph_t g = nullptr;
+    BPF_CHECK_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipSuccess;
+    try {
+      run(nullptr, nullptr, maxit, never, nullptr);
+    } catch (...) {
+      e = hipErrorUnknown;
+    }
+    const hipError_t ee = hipStreamEndCapture(st_, &g);
+    if (e == hipSuccess) e = ee;
+    if (e == hipSuccess) e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      // Not capturable here: eager runs from now on (this call's correction
+      // is already in Xtmp_).
+      (void)hipGetLastError();
+      gexec_ = nullptr;
+      gfail_ = true;
+    } else {
+      gits_ = maxit;
+    }
+  }
+  BPF_CHECK_HIP(hipMemcpyAsync(Z, Xtmp_.ptr, (size_t)n * p * sizeof(double), hipMemcpyDeviceToDevice, st_));
+}
+
 template <typename TV>
 int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &stop,
                         double *rnorm_out) {
@@ -401,16 +463,19 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
   k_ = 0;
   shifted_ = false;
   fill_int(st_, flags_.as<int>(), 0, 1);
-  // V_0 Gamma = R0 (CholQR2); E = [Gamma; 0].
-  copy_cast<double, TV>(st_, n, p, R0, p, W_.as<TV>(), p, 1.0);
+  // V_0 Gamma = R0 (CholQR2); E = [Gamma; 0].  (Staged: R0 is already in W_.)
+  if (!staged_) copy_cast<double, TV>(st_, n, p, R0, p, W_.as<TV>(), p, 1.0);
   fill_double(st_, E_.as<double>(), 0.0, 2 * (size_t)ng_ * SS);
   vtab_.set(0, V_.block(0), st_);
   // Gamma lands in rows 0..S of each group's E = [Gamma; 0].
   cholqr2<TV>(W_.ptr, V_.block(0), E_.as<double>(), 2 * SS, -1);
   // Snapshots of the CholQR flag word: entry j + 1 after iteration j's
   // CholQR2, entry 0 after V_0's, so the run reports the breakdowns of the
-  // iterations it keeps only (an overlapped run queues one it drops).
-  if (h_fl_cap_ < maxit + 2) {
+  // iterations it keeps only (an overlapped run queues one it drops).  A
+  // nested run reports none: its queue holds kernels only, no copies to the
+  // host and no events (so that it can be captured into a graph).
+  const bool nested = cfg_.nested;
+  if (!nested && h_fl_cap_ < maxit + 2) {
     if (h_fl_) {
       BPF_CHECK_HIP(hipStreamSynchronize(st_));
       BPF_CHECK_HIP(hipHostFree(h_fl_));
@@ -419,7 +484,7 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     BPF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_fl_), (size_t)(maxit + 2) * sizeof(int)));
     h_fl_cap_ = maxit + 2;
   }
-  BPF_CHECK_HIP(hipMemcpyAsync(h_fl_, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
+  if (!nested) BPF_CHECK_HIP(hipMemcpyAsync(h_fl_, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
   // (the context's pinned buffer may have been regrown for another solver
   // since this one was built)
   double *h_rn = own_rn_;
@@ -447,9 +512,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
       if (inner_ && flex && !f32) {
         // Z_k from inner FP32 block GMRES iterations on A z = V_k (the
         // flexible form takes any preconditioner step)
-        fill_double(st_, reinterpret_cast<double *>(Pk), 0.0, (size_t)n * p);
-        inner_->run(reinterpret_cast<const double *>(Vk), reinterpret_cast<double *>(Pk), cfg_.inner_its,
-                    [](const double *, int) { return false; }, nullptr);
+        inner_->precondition(reinterpret_cast<const double *>(Vk), reinterpret_cast<double *>(Pk),
+                             cfg_.inner_its);
       } else if (cfg_.prec32 && !f32) {
         // FP32 preconditioner on an FP64 basis: round V_k, apply, widen Z_k.
         const size_t fb = (size_t)n * p * sizeof(float);
@@ -480,7 +544,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     const size_t gstr = (size_t)(k + 2) * SS;
     // V_{k+1} C = W; C into block k+1 of each group's column.
     cholqr2<TV>(W_.ptr, V_.block(k + 1), col + (size_t)(k + 1) * SS, gstr, k);
-    BPF_CHECK_HIP(hipMemcpyAsync(h_fl_ + k + 1, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
+    if (!nested)
+      BPF_CHECK_HIP(hipMemcpyAsync(h_fl_ + k + 1, flags_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
     vtab_.set(k + 1, V_.block(k + 1), st_);
   };
   // QR update of the block Hessenberg matrix for iteration k and the copy
@@ -496,6 +561,7 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     launch_hess_apply(hs, ha, ng_);
     HessQrArgs qa{coltab_.dev<double>(), k, S_, ng_, E_.as<double>(), dg, rnorm_.as<double>()};
     launch_hess_qr(hs, qa, ng_);
+    if (nested) return;   // no host reads: the chain runs on st_ alone
     BPF_CHECK_HIP(hipMemcpyAsync(h_rn, rnorm_.ptr, (size_t)nslots_ * sizeof(double), hipMemcpyDeviceToHost, hs));
     BPF_CHECK_HIP(hipEventRecord(ev_hess_, hs));
   };
@@ -533,8 +599,8 @@ int BlockArnoldi::run_t(const double *R0, double *X, int maxit, const StopFn &st
     }
   }
   // the solution assembly below reads the last update's results
-  BPF_CHECK_HIP(hipStreamWaitEvent(st_, ev_hess_, 0));
-  if (!cfg_.nested) {
+  if (!nested) BPF_CHECK_HIP(hipStreamWaitEvent(st_, ev_hess_, 0));
+  if (!nested) {
     if (rnorm_out) std::memcpy(rnorm_out, h_rn, (size_t)nslots_ * sizeof(double));
     BPF_CHECK_HIP(hipStreamSynchronize(st_));
     shifted_ = h_fl_[k_] != 0;   // the flags after the last kept iteration

@@ -119,6 +119,15 @@ class BlockArnoldi {
   // row-major).  Returns the number of iterations; rnorm_out (host, p) gets
   // the final residual estimates.
   int run(const double *R0, double *X, int maxit, const StopFn &stop, double *rnorm_out);
+  // Preconditioner step of an outer flexible run, for a nested run: Z = the
+  // correction of maxit iterations on A z = V from z = 0 (V, Z: FP64
+  // row-major n x p).  With the graph option on, the fixed-length iteration
+  // chain is captured into a HIP graph on the first call (after one eager
+  // run that sizes every buffer and table) and replayed on later calls with
+  // the same maxit: the input is cast into the run's own block before the
+  // replay and the correction copied out after it.  Results are identical to
+  // the eager run's.
+  void precondition(const double *V, double *Z, int maxit);
   // Forms X + D for the current state without ending the run (used to
   // estimate ||x|| mid-run).
   void peek(const double *Xbase, double *Xout);
@@ -149,6 +158,12 @@ class BlockArnoldi {
 
   const double *scale_ = nullptr;
   Poly *poly_ = nullptr;
+  // Graph replay of a nested run: staged_ = the input is already in W_ and
+  // the correction stays in Xtmp_ (no R0 cast, no update of X).
+  bool staged_ = false;
+  hipGraphExec_t gexec_ = nullptr;
+  int gits_ = 0;               // iterations the captured chain runs
+  bool gfail_ = false;         // capture or instantiation failed: eager from now on
   std::unique_ptr<BlockArnoldi> inner_;   // nested FP32 block GMRES (inner_its > 0)
   hipEvent_t own_ev_ = nullptr;           // nested runs: their own event ...
   double *own_rn_ = nullptr;              // ... and pinned residual buffer

@@ -79,9 +79,17 @@ const char *bpf_version(void);
  * Hessenberg blocks, reflector applications, back-substitution) on rocBLAS
  * and rocSOLVER.  Default 129: up to 128 right-hand sides run on the
  * library's own matrix-core kernels; block_width < p splits wider blocks
- * into independent Krylov blocks of at most 128 columns there instead. */
+ * into independent Krylov blocks of at most 128 columns there instead.
+ * BPF_OPT_GRAPH: 1 (default) = in the mixed form with an inner FP32 block
+ * GMRES as the preconditioner step (inner_its > 0), the inner run's fixed
+ * chain of inner_its iterations is captured into a HIP graph on its first
+ * call and replayed on every later outer iteration (the input cast into the
+ * inner run's own block before the replay, the correction copied out after
+ * it); 0 = every launch issued from the host.  Results are identical.
+ * BPF_OPT_GRAPH_REPLAYS: read-only count of those replays in this process
+ * (set to 0 to reset). */
 enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4,
-       BPF_OPT_WIDE_MIN = 5 };
+       BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

@@ -163,3 +163,30 @@ def test_batch_errors(bpf, gen):
     with pytest.raises(bpf.BpfError):
         batch.solve([bpf.HipMatrix.from_numpy(S.B)], precision="mixed32")
 
+
+
+def test_batch_graph_replay_is_exact(bpf, gen, oracle):
+    # Each bin's inner FP32 runs replay a captured HIP graph (bins on their
+    # own threads and streams, captured concurrently): bitwise the result of
+    # issuing every launch from the host.
+    systems = _systems(gen, 12, 99, nmin=300, ps=[8, 16, 32, 64])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    out = {}
+    try:
+        for graph in (0, 1):
+            bpf.set_option("graph", graph)
+            bpf.set_option("graph_replays", 0)
+            Xs, st, solved = bpf.block_gmres_batched(As, Bs, precision="mixed", berr_tol=1e-10, memory=40,
+                                                     itmax=600, inner_its=6, bmgs_blocks=16)
+            out[graph] = ([X.to_numpy() for X in Xs], st.niter, solved.copy(), bpf.get_option("graph_replays"))
+    finally:
+        bpf.set_option("graph", 1)
+    assert out[0][3] == 0 and out[1][3] > 0, (out[0][3], out[1][3])
+    assert out[1][2].all()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for a, b in zip(out[0][0], out[1][0]):
+        np.testing.assert_array_equal(b, a)
+    for S, X in zip(systems, out[1][0]):
+        assert oracle.backward_error(S.indptr, S.indices, S.data, X, S.B).max() <= 1e-10

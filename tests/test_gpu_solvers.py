@@ -302,3 +302,30 @@ def test_inner_gmres_overlapped_is_exact(bpf, gen, oracle):
     assert out[0][2] == out[1][2]
     np.testing.assert_array_equal(out[1][1], out[0][1])
     np.testing.assert_array_equal(out[1][0], out[0][0])
+
+
+def test_inner_gmres_graph_replay_is_exact(bpf, gen, oracle):
+    # The inner FP32 runs are captured into a HIP graph on their first call
+    # and replayed on every later outer iteration: the same kernels in the
+    # same order, so the result is bitwise that of issuing every launch from
+    # the host, and the replays did happen.
+    S, A, B, Xd = _setup(bpf, gen, oracle, 2000, 32, 48)
+    P = bpf.ilu0(A, 32)
+    out = {}
+    try:
+        for graph in (0, 1):
+            bpf.set_option("graph", graph)
+            bpf.set_option("graph_replays", 0)
+            X, st = bpf.block_gmres(A, B, N=P, ldiv=True, memory=40, precision="mixed", berr_tol=1e-10,
+                                    itmax=400, inner_its=6, inner_width=16, history=True)
+            out[graph] = (X.to_numpy(), np.array(st.residuals), st.niter, bpf.get_option("graph_replays"))
+    finally:
+        bpf.set_option("graph", 1)
+    assert out[0][3] == 0
+    # one eager call (which also captures), then one replay per outer iteration
+    assert out[1][3] >= out[1][2] - 1 > 0, out[1][3]
+    assert out[0][2] == out[1][2]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    be = oracle.backward_error(S.indptr, S.indices, S.data, out[1][0], S.B)
+    assert be.max() <= 1e-10
