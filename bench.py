@@ -382,6 +382,11 @@ def bench_batch(args, bpf, gen):
 
     bpf.set_option("batch_threads", args.batch_threads)
     dt, out, _, _ = timed(bpf, ctx, step, args.steps, args.warmup, "batched", profile=False)
+    iw = batch.inner_widths()
+    widths = {int(q): int(w) for q, w in sorted(set(zip(np.asarray(ps).tolist(), iw.tolist())))}
+    widths_txt = ", ".join(f"{w} for {q} RHS" for q, w in widths.items())
+    log(f"[bench] inner Krylov block widths by RHS count: {widths_txt}"
+        + (" (chosen by timing on the first solve)" if args.inner_width < 0 else ""))
     solved = sum(r[2] for r in out)
     value = solved / dt
     st_last = out[-1][1]
@@ -430,8 +435,12 @@ def bench_batch(args, bpf, gen):
                      + (f" over {args.bmgs_blocks}-block steps" if args.orth == "bmgs" else "")
                      + "), FP32 ILU0 solves"
                      f"{f' in a degree-{args.poly_degree} GMRES polynomial' if args.poly_degree else ''}, "
-                     f"per-system stopping, to per-RHS backward error 1e-10; the four RHS-count bins "
+                     + (f"; preconditioner step: {args.inner_its} iterations of an inner FP32 block GMRES "
+                        f"(each system's columns in inner Krylov blocks of {widths_txt} columns; the inner "
+                        "chain captured in a HIP graph and replayed)" if args.inner_its > 0 else "")
+                     + f"; per-system stopping, to per-RHS backward error 1e-10; the four RHS-count bins "
                      f"solved {args.batch_threads} at a time from separate host threads"),
+        "inner_its": args.inner_its, "inner_width": args.inner_width, "inner_widths_by_rhs": widths,
         "nsys": nsys, "sum_n": ntot, "sum_nnz": nnz, "nrhs": nrhs, "orth": args.orth,
         "bmgs_blocks": args.bmgs_blocks, "batch_threads": args.batch_threads,
         "poly_degree": args.poly_degree,
@@ -754,8 +763,9 @@ def main():
     ap.add_argument("--inner-its", type=int, default=None,
                     help="mixed form: iterations of the inner FP32 block GMRES that is the preconditioner "
                          "step (default 8 for c2/c3, 6 for c4)")
-    ap.add_argument("--inner-width", type=int, default=16,
-                    help="columns per independent Krylov block of the inner solve (default 16)")
+    ap.add_argument("--inner-width", type=int, default=None,
+                    help="columns per independent Krylov block of the inner solve (default 16 for c2/c3; "
+                         "c4: -1 = chosen per RHS-count bin by timing on the first, untimed solve)")
     ap.add_argument("--batch-threads", type=int, default=4,
                     help="c4: RHS-count bins solved at the same time, each from its own host thread")
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
@@ -779,6 +789,8 @@ def main():
         args.poly_degree = 0
     if args.bmgs_blocks is None:
         args.bmgs_blocks = 16 if batched else 8
+    if args.inner_width is None:
+        args.inner_width = -1 if batched else 16
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)

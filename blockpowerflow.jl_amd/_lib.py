@@ -104,6 +104,7 @@ SIGNATURES = {
     "bpf_block_hessenberg_lsq": [vp, i, i, dp, i, dp, i, dp, i, dp],
     "bpf_batch_create": [vp, i, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), ip, C.POINTER(vp)],
     "bpf_batch_solve": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolverParams), C.POINTER(SolverStats), ip],
+    "bpf_batch_inner_widths": [vp, ip],
     "bpf_batch_destroy": [vp],
     "bpf_rf_create": [vp, vp, i, d, C.POINTER(vp)],
     "bpf_rf_refactor": [vp, vp],

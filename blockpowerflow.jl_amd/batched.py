@@ -65,6 +65,8 @@ class BatchedSystems:
                 raise ValueError(f"system {i}: B must be float64 of shape {(int(self.n[i]), int(self.p[i]))}")
         if Xs is None:
             Xs = [HipMatrix(B.shape, np.float64, self.ctx) for B in Bs]
+        if kw.get("inner_width") == "auto":
+            kw = dict(kw, inner_width=-1)
         prm = _params(**kw)
         st = _lib.SolverStats()
         solved = np.zeros(self.nsys, np.int32)
@@ -74,6 +76,14 @@ class BatchedSystems:
         stats = SimpleStats()
         _fill_stats(stats, st, np.zeros(0), False)
         return Xs, stats, solved.astype(bool)
+
+    def inner_widths(self):
+        """Columns per inner Krylov block the last solve used for each system
+        (0 = no inner solve); with inner_width="auto" the width each bin's
+        first solve chose by timing."""
+        out = np.zeros(self.nsys, np.int32)
+        check(L.bpf_batch_inner_widths(self.handle, out.ctypes.data_as(_lib.ip)))
+        return out
 
     def destroy(self):
         # The context must still be alive: its stream and device own the
@@ -94,7 +104,8 @@ def block_gmres_batched(As, Bs, *, precision="mixed", **kw):
     HipSparseMatrixCSR or scipy sparse matrices, Bs are HipMatrix blocks
     (n_i x p_i).  Keywords as block_gmres (memory, restart, atol, rtol, itmax,
     timemax, berr_tol, precision "fp64" | "mixed", reorthogonalization,
-    poly_degree)."""
+    poly_degree, bmgs_blocks, inner_its, inner_width: columns per inner
+    Krylov block or "auto" to time the candidates per RHS-count bin)."""
     for i, B in enumerate(Bs):
         if np.dtype(B.dtype) != np.float64:
             raise TypeError(f"system {i}: the batched solve takes float64 blocks, not {B.dtype}")

@@ -46,7 +46,10 @@ struct Bin {
   std::vector<double> anorm;            // ||A_j||_inf per member
   DevBuf d_groups, B, X, R, Xt, scale, stat, cmtab, ldtab;
   std::unique_ptr<BlockArnoldi> arn;
-  int arn_mode = -1;
+  long long arn_mode = -1;
+  // Inner-solve width chosen by the autotuner (0 = not tuned yet) for
+  // tuned_its inner iterations, and the width the last solve used.
+  int tuned_width = 0, tuned_its = -1, last_width = 0;
   ~Bin() {
     arn.reset();
     if (N) bpf_prec_destroy(N);
@@ -119,7 +122,10 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
   const bool mixed = prm.precision == BPF_PRECISION_MIXED;
   const bool use_berr = prm.berr_tol > 0.0;
   const int inner_its = mixed ? std::max(0, prm.inner_its) : 0;
-  const int mode = mixed ? 1 + 2 * inner_its : 0;
+  // columns per inner Krylov block (the whole system's p when not split)
+  const int iw = inner_its > 0 && prm.inner_width > 0 && prm.inner_width < p ? prm.inner_width : p;
+  const long long mode = mixed ? 1 + 2LL * inner_its + 4096LL * iw : 0;
+  b.last_width = inner_its > 0 ? iw : 0;
   if (!b.arn || b.arn_mode != mode) {
     ArnoldiConfig cfg;
     cfg.n = rows;
@@ -132,6 +138,7 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
     cfg.bmgs_blocks = std::max(1, prm.bmgs_blocks);
     cfg.groups = b.groups;
     cfg.inner_its = inner_its;
+    cfg.inner_s = iw < p ? iw : 0;
     b.arn.reset();
     b.arn.reset(new BlockArnoldi(ctx, b.A, b.N, cfg));
     b.arn_mode = mode;
@@ -311,6 +318,57 @@ BinResult solve_bin(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const
   return res;
 }
 
+// solve_bin with the inner-solve width chosen by timing (inner_width < 0):
+// on a bin's first such solve each candidate width (8, 16, 32, 64 up to the
+// bin's RHS count) solves the bin completely, the fastest to solve every
+// system is kept for this bin and number of inner iterations, and the
+// bin is solved once more with it if it was not the last candidate.  Every
+// candidate run is a full solve (the right-hand sides are read again), so
+// the answer is the chosen width's.
+BinResult solve_bin_tuned(bpf_ctx ctx, Bin &b, const double *const *d_B, double *const *d_X,
+                          const bpf_solver_params &prm, double t0, std::vector<char> &done_sys,
+                          std::vector<double> &berr_sys) {
+  const bool mixed = prm.precision == BPF_PRECISION_MIXED;
+  if (prm.inner_width >= 0 || !mixed || prm.inner_its <= 0) {
+    bpf_solver_params q = prm;
+    q.inner_width = std::max(0, prm.inner_width);
+    return solve_bin(ctx, b, d_B, d_X, q, t0, done_sys, berr_sys);
+  }
+  bpf_solver_params q = prm;
+  if (b.tuned_width > 0 && b.tuned_its == prm.inner_its) {
+    q.inner_width = b.tuned_width;
+    return solve_bin(ctx, b, d_B, d_X, q, t0, done_sys, berr_sys);
+  }
+  std::vector<int> cand;
+  for (int w : {8, 16, 32, 64})
+    if (w <= b.p) cand.push_back(w);
+  if (cand.empty() || cand.back() < b.p) cand.push_back(b.p);
+  int best = cand.back();
+  double best_t = INFINITY;
+  BinResult res;
+  for (int w : cand) {
+    q.inner_width = w;
+    BPF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    const double ts = now_s();
+    res = solve_bin(ctx, b, d_B, d_X, q, t0, done_sys, berr_sys);
+    BPF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    const double dt = now_s() - ts;
+    bool all = true;
+    for (int i : b.sys) all = all && done_sys[i];
+    if (all && dt < best_t) {
+      best_t = dt;
+      best = w;
+    }
+  }
+  b.tuned_width = best;
+  b.tuned_its = prm.inner_its;
+  if (best != cand.back()) {
+    q.inner_width = best;
+    res = solve_bin(ctx, b, d_B, d_X, q, t0, done_sys, berr_sys);
+  }
+  return res;
+}
+
 }  // namespace
 
 extern "C" {
@@ -422,7 +480,7 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
       // Refresh the factor: the caller may have changed nothing, but a bin is
       // factored once per solve so repeated solves time the same work.
       rethrow(bpf_prec_factor(b.N, b.A));
-      res[i] = solve_bin(b.cctx, b, d_B, d_X, *prm, t0, done, berr);
+      res[i] = solve_bin_tuned(b.cctx, b, d_B, d_X, *prm, t0, done, berr);
       BPF_CHECK_HIP(hipStreamSynchronize(b.cctx->stream));
       if (b.cctx->stream2) BPF_CHECK_HIP(hipStreamSynchronize(b.cctx->stream2));
     };
@@ -469,6 +527,13 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
     st.refine_steps = std::max(0, max_cycles - 1);
     st.timer = now_s() - t0;
     if (stats) *stats = st;
+  });
+}
+
+int bpf_batch_inner_widths(bpf_batch bt, int32_t *h_width) {
+  return guard([&] {
+    BPF_REQUIRE(bt && h_width, "null argument");
+    for (int i = 0; i < bt->nsys; ++i) h_width[i] = bt->bins[bt->bin_of[i]]->last_width;
   });
 }
 

@@ -141,7 +141,15 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
     ic.bmgs_blocks = std::max(8, cfg.inner_its);
     ic.cholqr_passes = 1;
     ic.nested = true;
-    ic.groups = cfg.groups;
+    if (!cfg.groups.empty() && cfg.inner_s > 0) {
+      // stacked systems: each system's columns split into inner groups of
+      // at most inner_s columns (independent Krylov blocks over its rows)
+      for (const auto &g : cfg.groups)
+        for (int c = 0; c < g.w; c += cfg.inner_s)
+          ic.groups.push_back({g.c0 + c, std::min(cfg.inner_s, g.w - c), g.r0, g.nr, g.slot + c, g.K, g.active, 0});
+    } else {
+      ic.groups = cfg.groups;
+    }
     inner_ = std::make_unique<BlockArnoldi>(ctx, A, N, ic);
   }
 }

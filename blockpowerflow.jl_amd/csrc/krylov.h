@@ -94,7 +94,8 @@ struct ArnoldiConfig {
   std::vector<GroupDesc> groups;
   // Flexible runs with an FP64 basis: the preconditioner step is inner_its
   // iterations of an FP32 block GMRES (ILU(0)-right-preconditioned, groups
-  // of inner_s columns, 0 = p) on A z = V_k, from z = 0 (0 = off).
+  // of inner_s columns, 0 = the run's own groups) on A z = V_k, from z = 0
+  // (0 = off).  With explicit groups each group's columns are split.
   int inner_its = 0;
   int inner_s = 0;
   // A run nested inside another one's preconditioner step: one stream, its

@@ -223,7 +223,9 @@ typedef struct {
                           many iterations of an FP32 ILU(0)-preconditioned block
                           GMRES on A z = V_k (in place of the polynomial)       */
   int inner_width;     /* mixed, inner_its > 0: columns per independent Krylov
-                          block of the inner solve (0 = all p)                  */
+                          block of the inner solve (0 = all p; batched solves:
+                          < 0 = chosen per bin by timing, see
+                          bpf_batch_inner_widths)                               */
 } bpf_solver_params;
 
 typedef struct {
@@ -329,6 +331,12 @@ int bpf_batch_create(bpf_ctx ctx, int nsys, const int32_t *n, const int32_t *nnz
 int bpf_batch_solve(bpf_batch batch, const double *const *d_B, double *const *d_X,
                     const bpf_solver_params *prm, bpf_solver_stats *stats,
                     int32_t *h_solved_per_system);
+/* Columns per inner Krylov block that the last bpf_batch_solve used for each
+ * system's bin (0 = no inner solve).  With inner_width < 0 (mixed,
+ * inner_its > 0) a bin's first solve times each width of 8, 16, 32, 64 (up
+ * to its RHS count) on full solves of the bin and keeps the fastest for
+ * later solves with the same inner_its. */
+int bpf_batch_inner_widths(bpf_batch batch, int32_t *h_width);
 int bpf_batch_destroy(bpf_batch batch);
 
 /* ---- fixed-pattern LU refactorization (CUSOLVERRF replacement) -------------- */

@@ -190,3 +190,61 @@ def test_batch_graph_replay_is_exact(bpf, gen, oracle):
         np.testing.assert_array_equal(b, a)
     for S, X in zip(systems, out[1][0]):
         assert oracle.backward_error(S.indptr, S.indices, S.data, X, S.B).max() <= 1e-10
+
+
+@pytest.mark.parametrize("width", [8, 16])
+def test_batch_inner_groups_split(bpf, gen, oracle, width):
+    # Each system's columns split into inner Krylov blocks of `width` columns
+    # (independent FP32 block GMRES runs over the system's rows): still the
+    # backward-error target and the direct solution for every system.
+    systems = _systems(gen, 8, 101, nmin=300, ps=[8, 16, 32, 64])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    batch = bpf.BatchedSystems(As, [S.B.shape[1] for S in systems])
+    try:
+        Xs, st, solved = batch.solve(Bs, precision="mixed", berr_tol=1e-10, memory=40, itmax=600, inner_its=6,
+                                     inner_width=width, bmgs_blocks=16)
+        assert solved.all(), (st, solved)
+        np.testing.assert_array_equal(batch.inner_widths(),
+                                      [min(width, S.B.shape[1]) for S in systems])
+        for S, X in zip(systems, Xs):
+            Xh = X.to_numpy()
+            assert _berr(oracle, S, Xh).max() <= 1e-10
+            Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+            assert np.all(np.abs(Xh - Xd).max(axis=0) / np.abs(Xd).max(axis=0) < 1e-5)
+    finally:
+        batch.destroy()
+
+
+def test_batch_inner_width_autotuned(bpf, gen, oracle):
+    # inner_width="auto": each bin's first solve times the candidate widths
+    # and keeps the fastest; the next solve uses it and equals, bit for bit,
+    # a solve with that width given explicitly.
+    systems = _systems(gen, 8, 103, nmin=300, ps=[8, 16, 32, 64])
+    As = [bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data) for S in systems]
+    Bs = [bpf.HipMatrix.from_numpy(S.B) for S in systems]
+    kw = dict(precision="mixed", berr_tol=1e-10, memory=40, itmax=600, inner_its=6, bmgs_blocks=16)
+    batch = bpf.BatchedSystems(As, [S.B.shape[1] for S in systems])
+    try:
+        Xs, st, solved = batch.solve(Bs, inner_width="auto", **kw)
+        assert solved.all(), (st, solved)
+        w = batch.inner_widths()
+        for S, wi, X in zip(systems, w, Xs):
+            p = S.B.shape[1]
+            assert wi in [c for c in (8, 16, 32, 64) if c <= p], (p, wi)
+            assert oracle.backward_error(S.indptr, S.indices, S.data, X.to_numpy(), S.B).max() <= 1e-10
+        Xa, _, solved_a = batch.solve(Bs, inner_width="auto", **kw)
+        np.testing.assert_array_equal(batch.inner_widths(), w)
+        # bins of one RHS count share a width: solving with those widths
+        # given explicitly, one count at a time, reproduces the tuned solve
+        for p in sorted({S.B.shape[1] for S in systems}):
+            idx = [i for i, S in enumerate(systems) if S.B.shape[1] == p]
+            sub = bpf.BatchedSystems([As[i] for i in idx], [p] * len(idx))
+            try:
+                Xe, _, _ = sub.solve([Bs[i] for i in idx], inner_width=int(w[idx[0]]), **kw)
+                for j, i in enumerate(idx):
+                    np.testing.assert_array_equal(Xe[j].to_numpy(), Xa[i].to_numpy())
+            finally:
+                sub.destroy()
+    finally:
+        batch.destroy()
