@@ -7,6 +7,37 @@
 
 namespace bpf {
 
+void build_elim_plan(int n, const std::vector<int> &rowptr, const std::vector<int> &colidx,
+                     const std::vector<int> &diagpos, hipStream_t s, ElimPlan &out) {
+  const int nnz = rowptr[n];
+  std::vector<PivInfo> pinfo(std::max(1, nnz), PivInfo{0, 0, 0, 0});
+  size_t total = 0;
+  for (int i = 0; i < n; ++i)
+    for (int e = rowptr[i]; e < diagpos[i]; ++e) {
+      const int k = colidx[e];
+      total += (size_t)(rowptr[k + 1] - diagpos[k] - 1);
+    }
+  BPF_REQUIRE(total < (size_t)INT32_MAX - 256, "elimination plan of %zu updates too large", total);
+  std::vector<int> map;
+  map.reserve(total + 1);
+  for (int i = 0; i < n; ++i) {
+    const int r0 = rowptr[i], r1 = rowptr[i + 1];
+    for (int e = r0; e < diagpos[i]; ++e) {
+      const int k = colidx[e];
+      pinfo[e] = PivInfo{diagpos[k], rowptr[k + 1], (int)map.size(), 0};
+      // row i's columns after position e are the ones above column k
+      const int *b = colidx.data() + e + 1, *end = colidx.data() + r1;
+      for (int j = diagpos[k] + 1; j < rowptr[k + 1]; ++j) {
+        const int *f = std::lower_bound(b, end, colidx[j]);
+        map.push_back(f != end && *f == colidx[j] ? (int)(f - colidx.data()) - r0 : -1);
+      }
+    }
+  }
+  map.push_back(-1);   // a pivot row without upper entries still reads one entry
+  upload(out.pinfo, pinfo, s);
+  upload(out.map, map, s);
+}
+
 std::vector<Seg> make_plan(const TriOp &op, int tail_rows) {
   std::vector<Seg> plan;
   int l = 0;

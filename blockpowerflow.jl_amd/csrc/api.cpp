@@ -443,7 +443,8 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A) {
     BPF_CHECK_HIP(hipMemcpyAsync(d_status, &init, sizeof(int), hipMemcpyHostToDevice, s));
     if (P->kind == BPF_PREC_ILU0) {
       launch_ilu_factor(s, P->lo, P->n, P->rowptr.as<int>(), P->colidx.as<int>(), P->diagpos.as<int>(),
-                        P->fac.as<double>(), P->max_row_nnz, d_status, factor_bytes(P->n, P->nnz));
+                        P->plan, P->fac.as<double>(), P->max_row_nnz, d_status,
+                        factor_bytes(P->n, P->nnz));
     } else {
       // IC(0) eliminates the strictly lower part and the diagonal of each row
       double entries = P->n;
@@ -488,6 +489,7 @@ int bpf_prec_create(bpf_ctx ctx, bpf_csr A, int kind, bpf_prec *out) {
       upload(P->diagpos, P->h_diagpos, ctx->stream);
       P->fac.ensure(std::max(1, P->nnz) * sizeof(double));
       if (kind == BPF_PREC_ILU0) {
+        build_elim_plan(P->n, P->h_rowptr, P->h_colidx, P->h_diagpos, ctx->stream, P->plan);
         build_triop(P->lo, P->n, P->h_rowptr, P->h_colidx, 0, ctx->stream);
         build_triop(P->up, P->n, P->h_rowptr, P->h_colidx, 1, ctx->stream);
       } else {

@@ -178,6 +178,21 @@ struct TriOp {
   int checked = 0;             // epoch of the last status check
 };
 
+// Elimination plan of an IKJ factorization on a fixed pattern (ILU(0) on
+// A's pattern, LU on the fill pattern), built once per pattern on the host:
+// for every strictly lower entry e = (i, k), PivInfo at e holds pivot row
+// k's diagonal position and end and where its update positions start in
+// `map`: for each upper entry (k, j) of row k, in order, the position of
+// (i, j) relative to row i's start, or -1 where row i has no entry (a fill
+// ILU(0) drops).  So no pivot step searches row i's pattern and the loads of
+// a pivot depend on nothing computed during the elimination.
+struct PivInfo {
+  int diag, end, moff, pad_;
+};
+struct ElimPlan {
+  DevBuf pinfo, map;
+  bool ready() const { return pinfo.ptr && map.ptr; }
+};
 // Process-wide switch between the sync-free single-launch triangular
 // solves / factorizations and the level-by-level launches (default).
 extern std::atomic<bool> g_syncfree;
@@ -235,6 +250,7 @@ struct bpf_prec_s {
   int kind = BPF_PREC_ILU0;
   int n = 0, nnz = 0;
   bpf::DevBuf rowptr, colidx, diagpos, fac;   // combined factor on A's pattern
+  bpf::ElimPlan plan;                         // ILU(0): elimination plan of the pattern
   std::vector<int> h_rowptr, h_colidx, h_diagpos;
   bpf::TriOp lo, up;                          // solve operators
   bpf::DevBuf tmp;                            // n x p scratch for apply
@@ -292,12 +308,14 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
 // row pointer / diagonal position / order entries (factor_bytes()).
 // Rows [r0, r1), independent of each other, eliminated against the (final)
 // pivot rows k < kmax only.
+void build_elim_plan(int n, const std::vector<int> &rowptr, const std::vector<int> &colidx,
+                     const std::vector<int> &diagpos, hipStream_t s, ElimPlan &out);
 void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
-                           const int *diagpos, double *fac, int max_row_nnz, int *d_status,
-                           double bytes);
+                           const int *diagpos, const ElimPlan &plan, double *fac, int max_row_nnz,
+                           int *d_status, double bytes);
 void launch_ilu_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
-                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                       int *d_status, double bytes);
+                       const int *colidx, const int *diagpos, const ElimPlan &plan, double *fac,
+                       int max_row_nnz, int *d_status, double bytes);
 void launch_ic_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                       int *d_status, double bytes);

@@ -40,6 +40,7 @@ struct bpf_rf_s {
   std::vector<int> h_rowptr, h_colidx; // pattern of A
   HostLU lu;                           // P, Q, pattern (and first values) of L\U
   DevBuf dP, dQ, frow, fcol, fdiag, fac, amap;
+  ElimPlan plan;                       // elimination plan of the fill pattern
   std::vector<int> h_fdiag;
   int max_row = 0;
   TriOp lo, up;
@@ -78,11 +79,12 @@ void refactor_into(bpf_rf rf, const double *d_aval, double *fac) {
   fill_int(s, rf->status.as<int>(), rf->n, 1);
   const int head = rf->dense() ? rf->t : rf->n;
   const std::vector<int> &frow = rf->lu.frow;
-  launch_ilu_factor(s, rf->lop(), rf->n, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(), fac,
+  launch_ilu_factor(s, rf->lop(), rf->n, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(),
+                    rf->plan, fac,
                     rf->max_row, rf->status.as<int>(), factor_bytes(head, frow[head] - frow[0]));
   if (rf->dense()) {
     launch_partial_factor(s, rf->t, rf->n, rf->t, rf->frow.as<int>(), rf->fcol.as<int>(), rf->fdiag.as<int>(),
-                          fac, rf->max_row, rf->status.as<int>(),
+                          rf->plan, fac, rf->max_row, rf->status.as<int>(),
                           factor_bytes(rf->n - rf->t, frow[rf->n] - frow[rf->t]));
     double *S = rf->S.as<double>();
     fill_double(s, S, 0.0, (size_t)rf->m * rf->m);
@@ -174,6 +176,7 @@ void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool all
   upload(rf->fcol, lu.fcol, s);
   upload(rf->fdiag, rf->h_fdiag, s);
   upload(rf->amap, amap, s);
+  build_elim_plan(n, lu.frow, lu.fcol, rf->h_fdiag, s, rf->plan);
   rf->fac.ensure(std::max(1, rf->nnzF) * sizeof(double));
   // Dense tail for large single systems: the last m rows (m a multiple of
   // the dense block width, at most 12 blocks and about n/16).  On the

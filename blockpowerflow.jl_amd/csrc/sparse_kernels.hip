@@ -409,6 +409,8 @@ __device__ __forceinline__ int lds_find(const int *s_col, int lo, int hi, int j)
 
 struct FacDev {
   const int *rowptr, *colidx, *diagpos, *order, *lvl;
+  const PivInfo *pinfo;   // ILU / LU elimination plan: pivot rows per lower
+  const int *map;         // entry, and the update positions of each pivot
   double *fac;
   int *status;
   int cap;  // LDS entries per wave
@@ -419,56 +421,94 @@ struct FacDev {
 // kmax < i: only pivot rows k < kmax are eliminated (the rest of the row is
 // left as the Schur complement of the leading kmax rows).
 __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_val, int kmax = 0x7fffffff) {
-  const int base = f.rowptr[i];
-  const int len = f.rowptr[i + 1] - base;
-  const int dloc = f.diagpos[i] - base;
+  // (row scalars are wave-uniform: one row per wavefront)
+  i = __builtin_amdgcn_readfirstlane(i);
+  const int base = __builtin_amdgcn_readfirstlane(f.rowptr[i]);
+  const int len = __builtin_amdgcn_readfirstlane(f.rowptr[i + 1]) - base;
+  const int dloc = __builtin_amdgcn_readfirstlane(f.diagpos[i]) - base;
   for (int q = lane; q < len; q += 64) {
     s_col[q] = f.colidx[base + q];
     s_val[q] = f.fac[base + q];
   }
   wave_sync();
-  // Pivot rows are final, so pivot kk+1's diagonal, extent and first 64
-  // entries are loaded while pivot kk is applied (the loads of one step no
-  // longer wait on each other); the arithmetic and its order are unchanged.
-  struct Piv {
-    int k, j0, j1, col;
-    double ukk, val;
-  };
-  auto fetch = [&](int kk) {
-    Piv q;
-    q.k = s_col[kk];
-    const int d = f.diagpos[q.k];
-    q.ukk = f.fac[d];
-    q.j0 = d + 1;
-    q.j1 = f.rowptr[q.k + 1];
-    const int jj = q.j0 + lane;
-    q.col = jj < q.j1 ? f.colidx[jj] : -1;
-    q.val = jj < q.j1 ? f.fac[jj] : 0.0;
-    return q;
-  };
-  Piv cur = dloc > 0 ? fetch(0) : Piv{0, 0, 0, -1, 1.0, 0.0};
-  for (int kk = 0; kk < dloc; ++kk) {
-    const int k = cur.k;
-    if (k >= kmax) break;
-    const Piv nxt = kk + 1 < dloc ? fetch(kk + 1) : cur;
-    const double ukk = cur.ukk;
-    const double l = s_val[kk] / ukk;
-    const int j0 = cur.j0, j1 = cur.j1;
-    if (cur.col >= 0) {
-      const int pos = lds_find(s_col, kk + 1, len, cur.col);
-      if (pos >= 0) s_val[pos] -= l * cur.val;
+  // Pivot rows are final, and the plan (pinfo: where pivot row k starts and
+  // ends and where its update positions start in map; read 64 pivots at a
+  // time, one entry per lane) comes from the pattern alone: the loads of the
+  // next four pivots (diagonal, first 128 upper entries and their positions
+  // in this row) are in flight while the current one is applied, and no
+  // step searches this row's pattern.  The loop is unrolled by the four
+  // slots, each refilled in place once applied, with every load issued at a
+  // clamped address.  Each position of the row is updated at most once per
+  // pivot, in pivot order, so the result is that of the IKJ recurrence.
+  if (dloc > 0) {
+    struct Piv {
+      int ulen, moff, d, pos0, pos1;
+      double ukk, val0, val1;
+    };
+    const PivInfo *pinfo = f.pinfo + base;
+    int wbase = 0;
+    PivInfo wpi = pinfo[min(lane, dloc - 1)];
+    auto fetch = [&](int kk) {
+      kk = min(kk, dloc - 1);
+      int d, end, moff;
+      if (kk >= wbase + 64) {
+        // next 64 pivots (reached exactly at the window's end: kk == wbase)
+        wbase += 64;
+        wpi = pinfo[min(wbase + lane, dloc - 1)];
+        d = __builtin_amdgcn_readlane(wpi.diag, 0);
+        end = __builtin_amdgcn_readlane(wpi.end, 0);
+        moff = __builtin_amdgcn_readlane(wpi.moff, 0);
+      } else {
+        d = __builtin_amdgcn_readlane(wpi.diag, kk - wbase);
+        end = __builtin_amdgcn_readlane(wpi.end, kk - wbase);
+        moff = __builtin_amdgcn_readlane(wpi.moff, kk - wbase);
+      }
+      Piv q;
+      q.d = d;
+      q.ulen = end - d - 1;
+      q.moff = moff;
+      q.ukk = f.fac[d];
+      // (an empty upper part reads entry 0: row k + 1's first value and the
+      // next pivot's first position, or the map's final padding entry)
+      const int last = max(q.ulen - 1, 0);
+      const int u0 = min(lane, last), u1 = min(lane + 64, last);
+      q.pos0 = f.map[moff + u0];
+      q.val0 = f.fac[d + 1 + u0];
+      q.pos1 = f.map[moff + u1];
+      q.val1 = f.fac[d + 1 + u1];
+      return q;
+    };
+    // Applies pivot kk (held in cur); false when k >= kmax ends the row.
+    auto apply = [&](int kk, const Piv &cur) {
+      const int k = s_col[kk];
+      if (k >= kmax) return false;
+      const double ukk = cur.ukk;
+      const double l = s_val[kk] / ukk;
+      if (lane < cur.ulen && cur.pos0 >= 0) s_val[cur.pos0] -= l * cur.val0;
+      if (lane + 64 < cur.ulen && cur.pos1 >= 0) s_val[cur.pos1] -= l * cur.val1;
+      for (int u = 128 + lane; u < cur.ulen; u += 64) {
+        const int pos = f.map[cur.moff + u];
+        if (pos >= 0) s_val[pos] -= l * f.fac[cur.d + 1 + u];
+      }
+      wave_sync();
+      if (lane == 0) {
+        s_val[kk] = l;
+        if (ukk == 0.0) atomicMin(f.status, k);
+      }
+      wave_sync();
+      return true;
+    };
+    Piv p0 = fetch(0), p1 = fetch(1), p2 = fetch(2), p3 = fetch(3);
+    for (int kk = 0; kk < dloc; kk += 4) {
+      if (!apply(kk, p0)) break;
+      p0 = fetch(kk + 4);
+      if (kk + 1 >= dloc || !apply(kk + 1, p1)) break;
+      p1 = fetch(kk + 5);
+      if (kk + 2 >= dloc || !apply(kk + 2, p2)) break;
+      p2 = fetch(kk + 6);
+      if (kk + 3 >= dloc || !apply(kk + 3, p3)) break;
+      p3 = fetch(kk + 7);
     }
-    for (int jj = j0 + 64 + lane; jj < j1; jj += 64) {
-      const int pos = lds_find(s_col, kk + 1, len, f.colidx[jj]);
-      if (pos >= 0) s_val[pos] -= l * f.fac[jj];
-    }
-    cur = nxt;
-    wave_sync();
-    if (lane == 0) {
-      s_val[kk] = l;
-      if (ukk == 0.0) atomicMin(f.status, k);
-    }
-    wave_sync();
   }
   for (int q = lane; q < len; q += 64) f.fac[base + q] = s_val[q];
   if (lane == 0 && kmax > i && s_val[dloc] == 0.0) atomicMin(f.status, i);
@@ -890,14 +930,17 @@ template void trisolve<double>(hipStream_t, TriOp &, int, const double *, int, d
 template void trisolve<float>(hipStream_t, TriOp &, int, const float *, int, float *, int);
 
 static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowptr,
-                          const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                          int *d_status, double bytes) {
+                          const int *colidx, const int *diagpos, const ElimPlan *eplan, double *fac,
+                          int max_row_nnz, int *d_status, double bytes) {
+  BPF_REQUIRE(ic || (eplan && eplan->ready()), "LU elimination without its elimination plan");
+  const PivInfo *pinfo = eplan ? eplan->pinfo.as<PivInfo>() : nullptr;
+  const int *map = eplan ? eplan->map.as<int>() : nullptr;
   const int cap = std::max(64, max_row_nnz);
   const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
   if (per_wave > 64 * 1024)
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
-  FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), fac, d_status, cap};
+  FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), pinfo, map, fac, d_status, cap};
   const int tok = prof::begin(s);
   if (g_syncfree && lower.n > 0) {
     const size_t shm = per_wave * wpb;
@@ -941,15 +984,17 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
 }
 
 void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *rowptr, const int *colidx,
-                           const int *diagpos, double *fac, int max_row_nnz, int *d_status,
-                           double bytes) {
+                           const int *diagpos, const ElimPlan &plan, double *fac, int max_row_nnz,
+                           int *d_status, double bytes) {
   if (r1 <= r0) return;
+  BPF_REQUIRE(plan.ready(), "LU elimination without its elimination plan");
   const int cap = std::max(64, max_row_nnz);
   const size_t per_wave = (size_t)cap * (sizeof(int) + sizeof(double));
   if (per_wave > 64 * 1024)
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   const int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
-  FacDev f{rowptr, colidx, diagpos, nullptr, nullptr, fac, d_status, cap};
+  FacDev f{rowptr, colidx, diagpos, nullptr, nullptr, plan.pinfo.as<PivInfo>(), plan.map.as<int>(), fac, d_status,
+           cap};
   const int tok = prof::begin(s);
   hipLaunchKernelGGL(fac_partial_kernel, dim3((r1 - r0 + wpb - 1) / wpb), dim3(64 * wpb), per_wave * wpb, s, f,
                      r0, r1, kmax);
@@ -958,15 +1003,15 @@ void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *r
 }
 
 void launch_ilu_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
-                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
-                       int *d_status, double bytes) {
-  launch_factor(false, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status, bytes);
+                       const int *colidx, const int *diagpos, const ElimPlan &plan, double *fac,
+                       int max_row_nnz, int *d_status, double bytes) {
+  launch_factor(false, s, lower, rowptr, colidx, diagpos, &plan, fac, max_row_nnz, d_status, bytes);
 }
 
 void launch_ic_factor(hipStream_t s, TriOp &lower, int, const int *rowptr,
                       const int *colidx, const int *diagpos, double *fac, int max_row_nnz,
                       int *d_status, double bytes) {
-  launch_factor(true, s, lower, rowptr, colidx, diagpos, fac, max_row_nnz, d_status, bytes);
+  launch_factor(true, s, lower, rowptr, colidx, diagpos, nullptr, fac, max_row_nnz, d_status, bytes);
 }
 
 void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst, int ldd,
