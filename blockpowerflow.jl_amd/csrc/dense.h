@@ -63,6 +63,10 @@ struct GemmArgs {
   int keep_skipped = 0;
   int rows_sum = 0;    // stacked groups: sum of heights (traffic accounting)
   double cells = -1.0; // rows x width of the groups that take part (accounting; negative = all)
+  // 1: every group's c0 and width, ldv and ldy are multiples of 4 and the
+  // blocks and Y are 16-byte aligned (the FP32 width-16 update then loads a
+  // row's four columns per lane as one vector).
+  int vec4 = 0;
 };
 
 struct CholArgs {

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace bpf {
 
@@ -473,6 +474,118 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
         TY *yp = Y + (size_t)r * a.ldy + c;
         const double v = a.alpha * acc[y][q];
         *yp = static_cast<TY>(a.beta == 0.0 ? v : a.beta * to_d(*yp) + v);
+      }
+    }
+  }
+}
+
+// Y[r, c0+b] = beta*Y + alpha * sum_i sum_a V_i[r, c0+a] * Psi_i[a][b] for
+// width-16 groups of an FP32 basis (the inner solve's updates): FP32 matrix
+// cores (v_mfma_f32_16x16x4f32) accumulating the 16 nb products of each
+// output in FP32, Psi rounded to FP32.  Lane l owns row l%16 of a 16-row
+// tile and loads its columns 4(l/16)..4(l/16)+3 as one 16-byte vector, so a
+// tile is one load per block with no LDS staging: MFMA step m takes column
+// 4(l/16)+m of the loaded row as A[i = l%16][k = l/16] and Psi[4(l/16)+m][l%16]
+// as B[k][j = l%16], which sums over every column once.  The result lands as
+// D[4(l/16)+q][l%16] in acc[q].  A workgroup (4 wavefronts, 4 tiles each)
+// owns 256 rows of one group, and consecutive workgroups take the groups of
+// one row chunk, so each 512-byte row of the block is read while its cache
+// lines are fresh.  The next block's loads are in flight during the current
+// block's MFMAs.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+constexpr int kG16Tiles = 4;
+constexpr int kG16Rows = 4 * 16 * kG16Tiles;
+
+// VEC: every group's width and column offset and both leading dimensions are
+// multiples of 4 and the blocks 16-byte aligned (GemmArgs::vec4); otherwise
+// the same lanes load their four columns one by one.  All loads are issued
+// at clamped addresses (rows past the group re-read its last row, columns
+// past its width read column 0): rows past the group only feed outputs that
+// are not stored, and columns past the width meet zero coefficients.
+template <bool VEC>
+__global__ __launch_bounds__(256) void gemm16f_kernel(GemmArgs a, int ng) {
+  const int g = blockIdx.x % ng;
+  const int chunk = blockIdx.x / ng;
+  const GroupDesc gd = a.groups[g];
+  if (chunk * kG16Rows >= gd.nr) return;
+  const int r0 = gd.r0 + chunk * kG16Rows, rend = gd.r0 + gd.nr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  float *Y = static_cast<float *>(a.Y) + gd.c0;
+  const bool takes_part = a.use_k ? gd.K > 0 : gd.active != 0;
+  if (!takes_part) {
+    // Skipped group: a fresh block (beta == 0) is zeroed, anything else kept.
+    if (a.beta == 0.0 && !a.keep_skipped) {
+      for (int e = threadIdx.x; e < kG16Rows * gd.w; e += 256) {
+        const int r = r0 + e / gd.w, c = e % gd.w;
+        if (r < rend) Y[(size_t)r * a.ldy + c] = 0.0f;
+      }
+    }
+    return;
+  }
+  const int nb = a.use_k ? min(a.nb, gd.K) : a.nb;
+  const int w = gd.w;
+  // element offsets of this lane's four columns (clamped into the group)
+  const int cq = 4 * kq < w ? 4 * kq : 0;
+  int ce[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) ce[m] = min(4 * kq + m, w - 1);
+  unsigned rowoff[kG16Tiles];
+#pragma unroll
+  for (int t = 0; t < kG16Tiles; ++t) {
+    const int r = min(r0 + wave * 16 * kG16Tiles + 16 * t + i16, rend - 1);
+    rowoff[t] = static_cast<unsigned>(r) * static_cast<unsigned>(a.ldv);
+  }
+  const int jc = min(i16, w - 1);
+  auto load = [&](int i, flt4 (&v)[kG16Tiles], double (&b)[4]) {
+    const auto *V = global_ptr<float>(static_cast<const float *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0);
+    const auto *P = global_ptr<double>(a.psi + (size_t)i * a.psi_istride + (size_t)g * a.psi_gstride);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) b[m] = P[ce[m] * a.ldpsi + jc];
+#pragma unroll
+    for (int t = 0; t < kG16Tiles; ++t) {
+      if (VEC) {
+        v[t] = *reinterpret_cast<const __attribute__((address_space(1))) flt4 *>(V + rowoff[t] + cq);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) v[t][m] = V[rowoff[t] + ce[m]];
+      }
+    }
+  };
+  flt4 acc[kG16Tiles];
+#pragma unroll
+  for (int t = 0; t < kG16Tiles; ++t) acc[t] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+  flt4 cv[kG16Tiles];
+  double cb[4];
+  if (nb > 0) load(0, cv, cb);
+  for (int i = 0; i < nb; ++i) {
+    flt4 nv[kG16Tiles];
+    double nbv[4];
+    if (i + 1 < nb) load(i + 1, nv, nbv);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      // coefficient rows past the group's width are zero
+      const float bm = 4 * kq + m < w ? static_cast<float>(cb[m]) : 0.0f;
+#pragma unroll
+      for (int t = 0; t < kG16Tiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv[t][m], bm, acc[t], 0, 0, 0);
+    }
+    if (i + 1 < nb) {
+#pragma unroll
+      for (int t = 0; t < kG16Tiles; ++t) cv[t] = nv[t];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) cb[m] = nbv[m];
+    }
+  }
+  if (i16 >= w) return;
+#pragma unroll
+  for (int t = 0; t < kG16Tiles; ++t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + wave * 16 * kG16Tiles + 16 * t + 4 * kq + q;
+      if (r < rend) {
+        float *yp = Y + (size_t)r * a.ldy + i16;
+        const double v = a.alpha * static_cast<double>(acc[t][q]);
+        *yp = static_cast<float>(a.beta == 0.0 ? v : a.beta * static_cast<double>(*yp) + v);
       }
     }
   }
@@ -1626,7 +1739,13 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   }
   const int tok = prof::begin(s);
   const dim3 blk(kGemmThreads);
-  switch (S) {
+  if (S == 16 && std::is_same<TVV, float>::value && std::is_same<TY, float>::value) {
+    // FP32 basis, width 16: the inner solve's updates on FP32 matrix cores
+    const long nwg = (long)((a.n + kG16Rows - 1) / kG16Rows) * ng;
+    BPF_REQUIRE(nwg < (1L << 31), "update grid too large");
+    if (a.vec4) hipLaunchKernelGGL(gemm16f_kernel<true>, dim3((unsigned)nwg), dim3(256), 0, s, a, ng);
+    else hipLaunchKernelGGL(gemm16f_kernel<false>, dim3((unsigned)nwg), dim3(256), 0, s, a, ng);
+  } else switch (S) {
     case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, blk, 0, s, a); break;
     case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, blk, 0, s, a); break;
     case 64: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 64>), grid, blk, 0, s, a); break;

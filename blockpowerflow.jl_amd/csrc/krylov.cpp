@@ -97,6 +97,11 @@ BlockArnoldi::BlockArnoldi(bpf_ctx ctx, bpf_csr A, bpf_prec N, const ArnoldiConf
     gmax_rows_ = cfg.n;
   }
   ng_ = (int)h_groups_.size();
+  // Vector loads of four columns: every group's offset and width and the
+  // row length are multiples of 4 (blocks are then 16-byte aligned: each
+  // starts n p elements after an aligned allocation).
+  vec4_ = cfg.p % 4 == 0;
+  for (const auto &g : h_groups_) vec4_ = vec4_ && g.c0 % 4 == 0 && g.w % 4 == 0;
   // per-solve work buffers come from the context's pool
   for (DevBuf *b : {&d_groups_, &W_, &P_, &E_, &rnorm_, &Y_, &Zb_, &Rinvb_, &Xtmp_, &Vf_, &Zf_, &tmpcol_,
                     &cq_.G, &cq_.R1, &cq_.R1i, &cq_.R2, &cq_.R2i, &cq_.R3, &cq_.R3i, &cq_.flags, &cq_.part,
@@ -196,7 +201,7 @@ void CholQrWork::init(int ng, int S) {
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
                     int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
-                    const ReorthFn &reorth, bool host_check, int passes) {
+                    const ReorthFn &reorth, bool host_check, int passes, int vec4) {
   // Two CholQR passes; C = R2 R1 (upper), Vout = W R1^{-1} R2^{-1}.
   const size_t SS = (size_t)S * S;
   int *fl = w.flags.as<int>();
@@ -216,6 +221,7 @@ void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, in
     ca.gflags = w.gflags.as<int>();
     launch_chol(s, ca, ng);
     GemmArgs ga{nullptr, src, 0, 1, p, Ris[i]->as<double>(), 0, SS, S, dst, p, gmax_rows, 1.0, 0.0, grp};
+    ga.vec4 = vec4;
     ga.rows_sum = rows_sum;
     ga.cells = cells;
     ga.keep_skipped = grp != dg;   // a subset pass leaves the other groups alone
@@ -267,9 +273,9 @@ void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, in
   launch_trmm_upper(s, w.G.as<double>(), w.R1.as<double>(), Cout, S, SS, cstride, brk, ng);
 }
 template void cholqr2_groups<double>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                     void *, double *, size_t, CholQrWork &, const ReorthFn &, bool, int);
+                                     void *, double *, size_t, CholQrWork &, const ReorthFn &, bool, int, int);
 template void cholqr2_groups<float>(hipStream_t, const GroupDesc *, int, int, int, int, int, int, void *,
-                                    void *, double *, size_t, CholQrWork &, const ReorthFn &, bool, int);
+                                    void *, double *, size_t, CholQrWork &, const ReorthFn &, bool, int, int);
 
 template <typename TV>
 void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride, int kprev) {
@@ -291,6 +297,7 @@ void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride, in
         pa.cells = 0.0;
         launch_proj<TV, TV>(st_, pa, S_, t, SS, gstr, false, part_);
         GemmArgs ga{vt, nullptr, i, nb, cfg_.p, t, SS, gstr, S_, Wp, cfg_.p, gmax_rows_, -1.0, 1.0, dg};
+        ga.vec4 = vec4_;
         ga.rows_sum = rows_sum_;
         ga.cells = 0.0;
         launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
@@ -299,7 +306,7 @@ void BlockArnoldi::cholqr2(void *W, void *Vout, double *Cout, size_t cstride, in
   }
   // a nested run (a preconditioner step) never waits on the host
   cholqr2_groups<TV>(st_, d_groups_.as<GroupDesc>(), ng_, S_, cfg_.p, gmax_rows_, rows_sum_, cfg_.n, W, Vout,
-                     Cout, cstride, cq_, reorth, !cfg_.nested, cfg_.cholqr_passes);
+                     Cout, cstride, cq_, reorth, !cfg_.nested, cfg_.cholqr_passes, vec4_);
 }
 
 // Psi_{i,k} = V_i^T W and W -= V_i Psi_{i,k}, i = 0..k, into block column k.
@@ -327,6 +334,7 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
       pa.cells = cells;
       launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, SS, gstr, false, part_);
       GemmArgs ga{vt, nullptr, i, nb, p, col + (size_t)i * SS, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
+      ga.vec4 = vec4_;
       ga.rows_sum = rows_sum_;
       ga.cells = cells;
       launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
@@ -337,6 +345,7 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
   pa.cells = cells;
   launch_proj<TV, TV>(st_, pa, S_, col, SS, gstr, false, part_);
   GemmArgs ga{vt, nullptr, 0, k + 1, p, col, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
+  ga.vec4 = vec4_;
   ga.rows_sum = rows_sum_;
   ga.cells = cells;
   launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
@@ -378,12 +387,14 @@ void BlockArnoldi::form(double *X, double alpha) {
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
                 S_, X, p, gmax_rows_, alpha, 1.0, dg, 1};
     ga.rows_sum = rows_sum_;
+    ga.vec4 = vec4_;
     launch_block_gemm<TV, double>(st_, ga, S_, ng_);
   } else {
     // P = sum_i V_i Y_i (TV), then X += alpha * N^{-1} P.
     GemmArgs ga{tab, nullptr, 0, K, p, Y_.as<double>(), (size_t)ng_ * SS, SS,
                 S_, P_.ptr, p, gmax_rows_, 1.0, 0.0, dg, 1};
     ga.rows_sum = rows_sum_;
+    ga.vec4 = vec4_;
     launch_block_gemm<TV, TV>(st_, ga, S_, ng_);
     TV *P = P_.as<TV>();
     prec_apply_rm<TV>(st_, N_, p, P, p, P, p);

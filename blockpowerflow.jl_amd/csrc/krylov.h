@@ -75,7 +75,7 @@ using ReorthFn = std::function<void(void *, const GroupDesc *)>;
 template <typename TV>
 void cholqr2_groups(hipStream_t s, const GroupDesc *dg, int ng, int S, int p, int gmax_rows, int rows_sum,
                     int n_shift, void *W, void *Vout, double *Cout, size_t cstride, CholQrWork &w,
-                    const ReorthFn &reorth = {}, bool host_check = true, int passes = 2);
+                    const ReorthFn &reorth = {}, bool host_check = true, int passes = 2, int vec4 = 0);
 
 // Host-visible description of one Krylov run.
 struct ArnoldiConfig {
@@ -173,6 +173,7 @@ class BlockArnoldi {
   bpf_prec N_;
   ArnoldiConfig cfg_;
   int S_ = 0, ng_ = 0, nslots_ = 0, gmax_rows_ = 0;
+  int vec4_ = 0;               // GemmArgs::vec4 holds for this run's groups and blocks
   int rows_sum_ = 0;           // > 0: groups stacked by rows (sum of heights)
   int k_ = 0;                  // completed block columns
   bool shifted_ = false;
