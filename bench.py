@@ -771,6 +771,10 @@ def main():
     ap.add_argument("--orth", choices=["bmgs", "bcgs2"], default="bmgs",
                     help="block Gram-Schmidt: bmgs (the reference's loop, over --bmgs-blocks steps) or "
                          "bcgs2 (two classical passes)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the inner solves' launches from the host instead of replaying a HIP graph "
+                         "(the same kernels; for rocprofv3 runs, whose kernel tracing does not survive graph "
+                         "instantiation here)")
     ap.add_argument("--no-kernel-timers", action="store_true",
                     help="skip the extra kernel-timer step (no roofline families; for kernel-trace runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -798,6 +802,8 @@ def main():
 
     import __graft_entry__ as ge
     bpf = ge.package()
+    if args.no_graph:
+        bpf.set_option("graph", 0)
     gen = ge._load("bpf_generator", os.path.join(ge.PKG_DIR, "generator.py"))
     if args.workload == "c4":
         res = bench_batch(args, bpf, gen)

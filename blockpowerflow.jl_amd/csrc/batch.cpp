@@ -485,8 +485,13 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
       if (b.cctx->stream2) BPF_CHECK_HIP(hipStreamSynchronize(b.cctx->stream2));
     };
     // Bins write disjoint systems' entries of done / berr and their own
-    // blocks, so they run on up to g_batch_threads host threads at once.
-    const int nthr = std::max(1, std::min<int>(nbin, g_batch_threads));
+    // blocks, so they run on up to g_batch_threads host threads at once;
+    // a solve in which some bin times its inner-width candidates runs the
+    // bins one after another, so that no bin's timings include another's.
+    bool tuning = false;
+    if (prm->precision == BPF_PRECISION_MIXED && prm->inner_width < 0 && prm->inner_its > 0)
+      for (const auto &b : bt->bins) tuning = tuning || !(b->tuned_width > 0 && b->tuned_its == prm->inner_its);
+    const int nthr = tuning ? 1 : std::max(1, std::min<int>(nbin, g_batch_threads));
     if (nthr == 1) {
       for (int i = 0; i < nbin; ++i) run(i);
     } else {

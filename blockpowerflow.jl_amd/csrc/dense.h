@@ -40,6 +40,9 @@ struct ProjArgs {
   // Rows x width of the groups that take part (traffic accounting only;
   // negative = all groups).
   double cells = -1.0;
+  // 1: width-16 FP32 blocks accumulate each row chunk's sums in FP32 on
+  // FP32 matrix cores (the chunks are still added in FP64).
+  int f32acc = 0;
 };
 
 // Y[:, g] = beta*Y[:, g] + alpha * sum_{i<nb} V_{i0+i}[:, g] * Psi_{i,g}

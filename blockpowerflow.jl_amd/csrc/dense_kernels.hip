@@ -73,6 +73,23 @@ __global__ void csr_norm_cast_kernel(int n, const int *rowptr, const double *val
 // output tiles are spread over the 4 wavefronts (a 2x2 grid of tile blocks);
 // for S < 64 the wavefronts split the rows instead and are summed in LDS.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Workgroups are handed to the 8 XCDs of an MI355X in turn (workgroup w runs
+// on XCD w % 8, each with its own L2).  Narrow groups share the cache lines
+// of a row (a 128-byte line holds 32 FP32 or 16 FP64 columns), so all groups
+// of one row chunk are placed on one XCD: chunk c on XCD c % 8, its groups
+// in consecutive turns.  Grids are padded to a multiple of 8 chunks.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_chunk_group(int ng, int &chunk, int &g) {
+  const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  g = j % ng;
+  chunk = (j / ng) * kXcds + xcd;
+}
+inline unsigned xcd_grid(long nchunk, int ng) {
+  const long padded = (nchunk + kXcds - 1) / kXcds * kXcds;
+  return (unsigned)(padded * ng);
+}
 
 __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
   // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
@@ -246,13 +263,65 @@ __device__ __forceinline__ void proj16_rows(const __attribute__((address_space(1
   }
 }
 
+// The same on FP32 matrix cores for an FP32 basis (ProjArgs::f32acc): each
+// chunk's sums accumulate in FP32 (v_mfma_f32_16x16x4f32, result in
+// acc[q] = D[4 (l/16) + q][l%16]) and the chunks are added in FP64.
+template <int NV>
+__device__ __forceinline__ void proj16_rows_f32(const __attribute__((address_space(1))) float *const (&V)[kNarrowBatch],
+                                                const __attribute__((address_space(1))) float *W, int ldv, int ldw,
+                                                int r0, int r1, int lr, int lc, bool colok,
+                                                flt4 (&acc)[kNarrowBatch]) {
+  constexpr int RU = 4;
+  const int lcl = colok ? lc : 0;
+  // The next round's loads are in flight during this round's MFMAs (masks
+  // applied at use, loads at clamped rows).
+  auto load = [&](int rr, float (&b)[RU], float (&v)[RU][NV]) {
+#pragma unroll
+    for (int t = 0; t < RU; ++t) {
+      const int rc = min(rr + 4 * t + lr, r1 - 1);
+      const unsigned ow = static_cast<unsigned>(rc) * static_cast<unsigned>(ldw) + lcl;
+      const unsigned ov = static_cast<unsigned>(rc) * static_cast<unsigned>(ldv) + lcl;
+      b[t] = W[ow];
+#pragma unroll
+      for (int u = 0; u < NV; ++u) v[t][u] = V[u][ov];
+    }
+  };
+  float bw[RU], av[RU][NV];
+  if (r0 < r1) load(r0, bw, av);
+  for (int rr = r0; rr < r1; rr += 4 * RU) {
+    const bool more = rr + 4 * RU < r1;
+    float nbw[RU], nav[RU][NV];
+    if (more) load(rr + 4 * RU, nbw, nav);
+#pragma unroll
+    for (int t = 0; t < RU; ++t) {
+      const float b = (colok && rr + 4 * t + lr < r1) ? bw[t] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < NV; ++u)
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(colok ? av[t][u] : 0.0f, b, acc[u], 0, 0, 0);
+    }
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < RU; ++t) {
+        bw[t] = nbw[t];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) av[t][u] = nav[t][u];
+      }
+    }
+  }
+}
+
 // The occupancy hint keeps the kernel at 61-67 VGPRs and no AGPRs (7-8
 // wavefronts per SIMD); without it the compiler parks the accumulators in
 // AGPRs and settles at 4.
-template <typename TVV, typename TW>
+template <typename TVV, typename TW, bool F32 = false>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(6))) void proj16_kernel(ProjArgs a) {
-  constexpr int NW = kB / 64, NB = kNarrowBatch;
-  const int chunk = blockIdx.x, g = blockIdx.y;
+  // one wavefront per block slot (up to 4; fewer blocks, fewer wavefronts
+  // and shorter chunks, so that every call keeps as many rows in flight)
+  constexpr int NB = kNarrowBatch;
+  const int NW = blockDim.x >> 6;
+  int g, chunk;
+  xcd_chunk_group(a.ng, chunk, g);
+  if (chunk * a.chunk_rows >= a.n) return;   // grid padding
   const GroupDesc gd = a.groups[g];
   const int r0 = gd.r0 + chunk * a.chunk_rows;
   const int r1 = gd.active ? min(gd.r0 + gd.nr, r0 + a.chunk_rows) : r0;
@@ -270,21 +339,40 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(6))) void pr
       const int i = min(ib + u * NW, a.nb - 1);
       V[u] = global_ptr<TVV>(static_cast<const TVV *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0);
     }
-    dbl4 acc[NB];
+    if constexpr (F32) {
+      flt4 acc[NB];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-    switch (nv) {
-      case 4: proj16_rows<4>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
-      case 3: proj16_rows<3>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
-      case 2: proj16_rows<2>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
-      default: proj16_rows<1>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
-    }
+      for (int u = 0; u < NB; ++u) acc[u] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+      switch (nv) {
+        case 4: proj16_rows_f32<4>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        case 3: proj16_rows_f32<3>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        case 2: proj16_rows_f32<2>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        default: proj16_rows_f32<1>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+      }
 #pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      if (u >= nv) break;
-      double *out = a.part + ((size_t)(chunk * a.nb + ib + u * NW) * a.ng + g) * 256;
+      for (int u = 0; u < NB; ++u) {
+        if (u >= nv) break;
+        double *out = a.part + ((size_t)(chunk * a.nb + ib + u * NW) * a.ng + g) * 256;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) out[(lr + 4 * q) * 16 + lc] = acc[u][q];
+        for (int q = 0; q < 4; ++q) out[(4 * lr + q) * 16 + lc] = acc[u][q];
+      }
+    } else {
+      dbl4 acc[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+      switch (nv) {
+        case 4: proj16_rows<4>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        case 3: proj16_rows<3>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        case 2: proj16_rows<2>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+        default: proj16_rows<1>(V, W, a.ldv, a.ldw, r0, r1, lr, lc, colok, acc); break;
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        if (u >= nv) break;
+        double *out = a.part + ((size_t)(chunk * a.nb + ib + u * NW) * a.ng + g) * 256;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[(lr + 4 * q) * 16 + lc] = acc[u][q];
+      }
     }
   }
 }
@@ -488,11 +576,9 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
 // 4(l/16)+m of the loaded row as A[i = l%16][k = l/16] and Psi[4(l/16)+m][l%16]
 // as B[k][j = l%16], which sums over every column once.  The result lands as
 // D[4(l/16)+q][l%16] in acc[q].  A workgroup (4 wavefronts, 4 tiles each)
-// owns 256 rows of one group, and consecutive workgroups take the groups of
-// one row chunk, so each 512-byte row of the block is read while its cache
-// lines are fresh.  The next block's loads are in flight during the current
+// owns 256 rows of one group; all groups of a row chunk run on one XCD
+// (xcd_chunk_group), so the lines of a row are fetched into one L2 only.  The next block's loads are in flight during the current
 // block's MFMAs.
-typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int kG16Tiles = 4;
 constexpr int kG16Rows = 4 * 16 * kG16Tiles;
 
@@ -504,8 +590,8 @@ constexpr int kG16Rows = 4 * 16 * kG16Tiles;
 // are not stored, and columns past the width meet zero coefficients.
 template <bool VEC>
 __global__ __launch_bounds__(256) void gemm16f_kernel(GemmArgs a, int ng) {
-  const int g = blockIdx.x % ng;
-  const int chunk = blockIdx.x / ng;
+  int g, chunk;
+  xcd_chunk_group(ng, chunk, g);
   const GroupDesc gd = a.groups[g];
   if (chunk * kG16Rows >= gd.nr) return;
   const int r0 = gd.r0 + chunk * kG16Rows, rend = gd.r0 + gd.nr;
@@ -1678,10 +1764,13 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
                  bool accumulate, DevBuf &part) {
   if (a.nb <= 0 || a.ng <= 0) return;
   int nchunk;
+  const int nw16 = std::min(4, a.nb);   // S = 16: wavefronts per workgroup
   if (S == 16) {
-    // One workgroup per row chunk and group (all blocks): ~2048 of them.
+    // One workgroup per row chunk and group (all blocks): ~8192 wavefronts
+    // over all of them whatever the number of blocks.
     const long rows = a.rows_sum > 0 ? a.rows_sum : (long)a.n * a.ng;
-    const int cr = (int)std::max<long>(64, (rows + 2047) / 2048);
+    const long target = 2048L * 4 / nw16;
+    const int cr = (int)std::max<long>(64, (rows + target - 1) / target);
     a.chunk_rows = ((cr + 15) / 16) * 16;
   } else if (a.rows_sum > 0) {
     // Stacked groups of uneven height: ~1024 non-empty workgroups over all
@@ -1703,7 +1792,15 @@ void launch_proj(hipStream_t s, ProjArgs a, int S, double *out, size_t o_istride
   dim3 grid(a.nb, nchunk, a.ng);
   const int tok = prof::begin(s);
   switch (S) {
-    case 16: hipLaunchKernelGGL((proj16_kernel<TVV, TW>), dim3(nchunk, a.ng), dim3(kB), 0, s, a); break;
+    case 16:
+      if constexpr (std::is_same<TVV, float>::value && std::is_same<TW, float>::value) {
+        if (a.f32acc) {
+          hipLaunchKernelGGL((proj16_kernel<TVV, TW, true>), dim3(xcd_grid(nchunk, a.ng)), dim3(64 * nw16), 0, s, a);
+          break;
+        }
+      }
+      hipLaunchKernelGGL((proj16_kernel<TVV, TW, false>), dim3(xcd_grid(nchunk, a.ng)), dim3(64 * nw16), 0, s, a);
+      break;
     case 32: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 32>), grid, dim3(kB), 0, s, a); break;
     case 64: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 64>), grid, dim3(kB), 0, s, a); break;
     case 128: hipLaunchKernelGGL((proj_partial_kernel<TVV, TW, 128>), grid, dim3(kB), 0, s, a); break;
@@ -1741,10 +1838,11 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
   const dim3 blk(kGemmThreads);
   if (S == 16 && std::is_same<TVV, float>::value && std::is_same<TY, float>::value) {
     // FP32 basis, width 16: the inner solve's updates on FP32 matrix cores
-    const long nwg = (long)((a.n + kG16Rows - 1) / kG16Rows) * ng;
-    BPF_REQUIRE(nwg < (1L << 31), "update grid too large");
-    if (a.vec4) hipLaunchKernelGGL(gemm16f_kernel<true>, dim3((unsigned)nwg), dim3(256), 0, s, a, ng);
-    else hipLaunchKernelGGL(gemm16f_kernel<false>, dim3((unsigned)nwg), dim3(256), 0, s, a, ng);
+    const long nchunk = (a.n + kG16Rows - 1) / kG16Rows;
+    BPF_REQUIRE((nchunk + kXcds) * ng < (1L << 31), "update grid too large");
+    const dim3 grid16(xcd_grid(nchunk, ng));
+    if (a.vec4) hipLaunchKernelGGL(gemm16f_kernel<true>, grid16, dim3(256), 0, s, a, ng);
+    else hipLaunchKernelGGL(gemm16f_kernel<false>, grid16, dim3(256), 0, s, a, ng);
   } else switch (S) {
     case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, blk, 0, s, a); break;
     case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, blk, 0, s, a); break;

@@ -332,6 +332,9 @@ void BlockArnoldi::orthogonalize(int k, double *col) {
       const int nb = std::min(g, k + 1 - i);
       ProjArgs pa{vt, nullptr, i, nb, p, W_.ptr, p, gmax_rows_, dg, ng_, 0, nullptr, rows_sum_};
       pa.cells = cells;
+      // an inner (preconditioner) run's projection coefficients: FP32 sums
+      // per row chunk (its CholQR Gram matrices stay FP64)
+      pa.f32acc = cfg_.nested ? 1 : 0;
       launch_proj<TV, TV>(st_, pa, S_, col + (size_t)i * SS, SS, gstr, false, part_);
       GemmArgs ga{vt, nullptr, i, nb, p, col + (size_t)i * SS, SS, gstr, S_, W_.ptr, p, gmax_rows_, -1.0, 1.0, dg};
       ga.vec4 = vec4_;
