@@ -82,7 +82,7 @@ def default_context() -> Context:
 
 
 _OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5, "graph": 6,
-            "graph_replays": 7}
+            "graph_replays": 7, "relabel_min": 8}
 
 
 def set_option(name: str, value) -> None:
@@ -101,7 +101,10 @@ def set_option(name: str, value) -> None:
     kernels; "graph" (default True) captures the mixed form's inner FP32
     block GMRES run (its fixed chain of inner_its iterations) into a HIP graph
     on its first call and replays it on later outer iterations;
-    "graph_replays" counts those replays (reset with 0)."""
+    "graph_replays" counts those replays (reset with 0); "relabel_min"
+    (default 32768, 0 = never) is the smallest system that block GMRES
+    solves in a reverse Cuthill-McKee relabelling of A's pattern, for the
+    locality of the rows its products and triangular solves gather."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -137,6 +137,15 @@ void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
     for (int i = 0; i < n; ++i)
       if (member[i]) op.h_order[f[level[i]]++] = i;
   }
+  const std::vector<int> *rl = win.relabel;
+  if (rl) {
+    BPF_REQUIRE((int)rl->size() == n, "relabelling of %zu rows for an operator on %d", rl->size(), n);
+    // each level's rows in storage order, so neighbouring rows (and the rows
+    // they gather) are near each other in memory
+    for (int l = 0; l < nlev; ++l)
+      std::sort(op.h_order.begin() + op.h_lvl[l], op.h_order.begin() + op.h_lvl[l + 1],
+                [&](int a, int b) { return (*rl)[a] < (*rl)[b]; });
+  }
   n = nrows;
   op.h_ptr.assign(n + 1, 0);
   op.max_row_nnz = 0;
@@ -153,11 +162,13 @@ void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
     const int i = op.h_order[t];
     int o = op.h_ptr[t];
     for (int k = optr[i]; k < optr[i + 1]; ++k, ++o) {
-      op.h_col[o] = ocol[k];
+      op.h_col[o] = rl ? (*rl)[ocol[k]] : ocol[k];
       op.h_gather[o] = opos[k];
     }
     op.h_dgather[t] = dpos[i] < 0 ? 0 : dpos[i];
   }
+  if (rl)
+    for (int t = 0; t < n; ++t) op.h_order[t] = (*rl)[op.h_order[t]];
   upload(op.order, op.h_order, s);
   upload(op.ptr, op.h_ptr, s);
   upload(op.col, op.h_col, s);

@@ -137,6 +137,7 @@ void bpf_csr_s::refresh_derived() {
   BPF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   norm_inf = ctx->h_small[0];
   val32_valid = true;
+  ++version;
 }
 
 extern "C" {
@@ -150,6 +151,10 @@ int bpf_set_option(int option, long long value) {
     else if (option == BPF_OPT_AUTOTUNE) g_autotune = value != 0;
     else if (option == BPF_OPT_OVERLAP) g_overlap_hess = value != 0;
     else if (option == BPF_OPT_GRAPH) g_graph = value != 0;
+    else if (option == BPF_OPT_RELABEL_MIN) {
+      BPF_REQUIRE(value >= 0, "relabel_min must be non-negative");
+      g_relabel_min = (int)std::min<long long>(value, 1 << 30);
+    }
     else if (option == BPF_OPT_GRAPH_REPLAYS) {
       BPF_REQUIRE(value == 0, "graph_replays can only be reset to 0");
       g_graph_replays = 0;
@@ -174,6 +179,7 @@ int bpf_get_option(int option, long long *value) {
     else if (option == BPF_OPT_BATCH_THREADS) *value = g_batch_threads;
     else if (option == BPF_OPT_WIDE_MIN) *value = g_wide_min;
     else if (option == BPF_OPT_GRAPH) *value = g_graph ? 1 : 0;
+    else if (option == BPF_OPT_RELABEL_MIN) *value = g_relabel_min;
     else if (option == BPF_OPT_GRAPH_REPLAYS) *value = g_graph_replays;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });
@@ -389,6 +395,7 @@ int bpf_csr_destroy(bpf_csr A) {
     if (!A) return;
     (void)hipStreamSynchronize(A->ctx->stream);
     spmm_forget(A->rowptr.as<int>());
+    relabel_free(A);
     delete A;
   });
 }
@@ -458,6 +465,7 @@ int bpf_prec_factor(bpf_prec P, bpf_csr A) {
     triop_check(P->lo, s);
     triop_refresh(P->lo, P->fac.as<double>(), s);
     triop_refresh(P->up, P->fac.as<double>(), s);
+    ++P->fac_version;
     if (status < P->n)
       BPF_THROW(BPF_E_SINGULAR, "%s: zero or non-positive pivot at row %d",
                 P->kind == BPF_PREC_ILU0 ? "ilu0" : "ic0", status);
@@ -571,6 +579,7 @@ int bpf_prec_destroy(bpf_prec P) {
   return guard([&] {
     if (!P) return;
     (void)hipStreamSynchronize(P->ctx->stream);
+    relabel_free(P);
     delete P;
   });
 }

@@ -120,6 +120,11 @@ void upload(DevBuf &b, const std::vector<T> &h, hipStream_t s) {
 }  // namespace bpf
 
 // ---------------------------------------------------------------- handles
+namespace bpf {
+struct CsrRelabel;
+struct PrecRelabel;
+}  // namespace bpf
+
 struct bpf_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -148,6 +153,8 @@ struct bpf_csr_s {
   bpf::DevBuf rowlen_order;    // rows sorted for SpMM (unused: natural order)
   double norm_inf = 0.0;       // ||A||_inf of the current values
   bool val32_valid = false;
+  unsigned long long version = 0;   // bumped by every value change
+  bpf::CsrRelabel *rl = nullptr;    // locality-relabelled copy for the solvers (lazy)
   void refresh_derived();      // after a value change: norm, FP32 copy
 };
 
@@ -236,6 +243,11 @@ std::vector<Seg> make_plan(const TriOp &op, int tail_rows);
 struct TriWindow {
   int row_lo = 0, row_hi = -1, col_lo = 0, col_hi = -1;
   bool unit = false, skip_empty = false;
+  // Relabelling of the vectors the operator acts on (original row i is
+  // stored at row relabel[i]): the same levels, rows and arithmetic, with
+  // row and dependency indices mapped and each level's rows in storage
+  // order.
+  const std::vector<int> *relabel = nullptr;
 };
 void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
                  const std::vector<int> &colidx, int kind, hipStream_t s,
@@ -251,6 +263,8 @@ struct bpf_prec_s {
   int n = 0, nnz = 0;
   bpf::DevBuf rowptr, colidx, diagpos, fac;   // combined factor on A's pattern
   bpf::ElimPlan plan;                         // ILU(0): elimination plan of the pattern
+  unsigned long long fac_version = 0;         // bumped by every factorization
+  bpf::PrecRelabel *rl = nullptr;             // relabelled triangular operators (lazy)
   std::vector<int> h_rowptr, h_colidx, h_diagpos;
   bpf::TriOp lo, up;                          // solve operators
   bpf::DevBuf tmp;                            // n x p scratch for apply
@@ -323,6 +337,44 @@ void launch_ic_factor(hipStream_t s, TriOp &lower, int n, const int *rowptr,
 inline double factor_bytes(double rows, double entries) {
   return entries * (2 * sizeof(double) + sizeof(int)) + rows * 3 * sizeof(int);
 }
+
+// Row-permuted layout conversions: dst (row-major) row k = src (column-major)
+// row rowmap[k]; dst (column-major) row rowmap[k] = src (row-major) row k.
+template <typename TD>
+void launch_perm_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, const int *rowmap, TD *dst,
+                          int ldd);
+template <typename TS>
+void launch_perm_rm_to_cm(hipStream_t s, int n, int p, const TS *src, int lds, const int *rowmap, double *dst,
+                          int ldd);
+
+// Locality relabelling for the Krylov solvers (csrc/relabel.cpp).  Systems
+// with at least g_relabel_min unknowns are solved in a reverse Cuthill-McKee
+// relabelling of A's (symmetrized) pattern, so that the rows an SpMM or a
+// triangular-solve row gathers lie near it in memory: A's relabelled copy
+// (each row's entries in the original order, so each product is summed as
+// before) and relabelled views of an ILU(0)/IC(0) handle's triangular
+// operators (the same levels and rows).  Both are built once per pattern and
+// refreshed when A's values or the factor change.
+extern std::atomic<int> g_relabel_min;
+struct CsrRelabel {
+  std::vector<int> perm;       // original row i -> storage row perm[i]
+  DevBuf order;                // storage row k holds original row order[k] (device)
+  DevBuf emap;                 // original entry e -> entry of the relabelled copy
+  bpf_csr_s *Ar = nullptr;
+  unsigned long long version = ~0ull;
+};
+struct PrecRelabel {
+  const CsrRelabel *key = nullptr;
+  bpf_prec_s *Nr = nullptr;
+  unsigned long long fac_version = ~0ull;
+};
+// The relabelled copy of A with current values (built on first use).
+CsrRelabel *relabel_csr(bpf_csr A, hipStream_t s);
+// The relabelled view of P for A's relabelling (null for kinds it does not
+// cover), its values current.
+bpf_prec relabel_prec(bpf_prec P, const CsrRelabel *rl, hipStream_t s);
+void relabel_free(bpf_csr A);
+void relabel_free(bpf_prec P);
 
 // Layout conversions (column-major <-> row-major) with type conversion.
 void launch_cm_to_rm(hipStream_t s, int n, int p, const double *src, int lds, void *dst,

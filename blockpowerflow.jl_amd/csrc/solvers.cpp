@@ -97,11 +97,26 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const do
   const double t0 = now_s();
   const int n = A->n;
   hipStream_t s = ctx->stream;
+  // Large systems run in A's locality relabelling (the products and the
+  // preconditioner's triangular solves on relabelled copies; B, X0 and X
+  // permuted at the layout conversion).
+  const int *order = nullptr;
+  if (g_relabel_min > 0 && n >= g_relabel_min && !M &&
+      (!N || N->kind == BPF_PREC_ILU0 || N->kind == BPF_PREC_IC0)) {
+    CsrRelabel *rl = relabel_csr(A, s);
+    if (N) N = relabel_prec(N, rl, s);
+    A = rl->Ar;
+    order = rl->order.as<int>();
+  }
+  auto to_rm = [&](const double *src, int ld, double *dst) {
+    if (order) launch_perm_cm_to_rm<double>(s, n, p, src, ld, order, dst, p);
+    else launch_cm_to_rm(s, n, p, src, ld, dst, p, false);
+  };
   Blocks blk;
   blk.init(n, p, &ctx->pool);
-  launch_cm_to_rm(s, n, p, dB, ldb, blk.B.ptr, p, false);
+  to_rm(dB, ldb, blk.B.as<double>());
   if (dX0) {
-    launch_cm_to_rm(s, n, p, dX0, ldx0, blk.X.ptr, p, false);
+    to_rm(dX0, ldx0, blk.X.as<double>());
     residual_rm(s, A, p, blk.B.as<double>(), blk.X.as<double>(), blk.R.as<double>());
   } else {
     fill_double(s, blk.X.as<double>(), 0.0, (size_t)n * p);
@@ -227,7 +242,8 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const do
     if (prm.timemax > 0 && now_s() - t0 > prm.timemax) overtimed = true;
     if (its == 0) break;
   }
-  launch_rm_to_cm(s, n, p, blk.X.ptr, p, false, dX, ldx);
+  if (order) launch_perm_rm_to_cm<double>(s, n, p, blk.X.as<double>(), p, order, dX, ldx);
+  else launch_rm_to_cm(s, n, p, blk.X.ptr, p, false, dX, ldx);
   BPF_CHECK_HIP(hipStreamSynchronize(s));
   st.niter = iter;
   st.solved = solved;

@@ -313,6 +313,25 @@ void gmres_wide(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const dou
       }
       const bool hh = basis_qr(h, s, n, p, Qp, C.as<double>(), p, bq);      // :316
       rank_deficient |= hh;
+      if (hh) {
+        // A rank-deficient block (the Krylov space exhausted to rounding, or
+        // dependent directions): Householder QR completed it with directions
+        // that are orthonormal among themselves but not to the earlier basis,
+        // and blocks built on them would break the Arnoldi relation (the
+        // least-squares solution then grows without bound).  The block is
+        // projected off V_0..V_k once more, Phi_i = V_i^T Q, Q -= V_i Phi_i,
+        // and orthonormalized again, Q = Q2 C2; then A P = sum_i V_i
+        // (Psi_ik + Phi_i C) + Q2 (C2 C), so Psi_ik += Phi_i C and C <- C2 C.
+        double *Cp = C.as<double>(), *Ph = Psi.as<double>(), *T2 = D.as<double>();
+        for (int i = 0; i <= k; ++i) {
+          gemm(h, true, false, p, p, n, 1.0, V[i].as<double>(), n, Qp, n, 0.0, Ph, p);
+          gemm(h, false, false, n, p, p, -1.0, V[i].as<double>(), n, Ph, p, 1.0, Qp, n);
+          gemm(h, false, false, p, p, p, 1.0, Ph, p, Cp, p, 1.0, Rk + (size_t)i * p, ldr);
+        }
+        basis_qr(h, s, n, p, Qp, T2, p, bq);
+        gemm(h, false, false, p, p, p, 1.0, T2, p, Cp, p, 0.0, T2 + (size_t)p * p, p);
+        sub(s, p, p, T2 + (size_t)p * p, p, Cp, p, SUB_COPY);
+      }
       // previous reflections: [Psi_ik; Psi_i+1,k] = Omega_i^T [...] (:320-326)
       double *Dp = D.as<double>();
       for (int i = 0; i + 1 <= k; ++i) {

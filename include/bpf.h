@@ -87,9 +87,17 @@ const char *bpf_version(void);
  * inner run's own block before the replay, the correction copied out after
  * it); 0 = every launch issued from the host.  Results are identical.
  * BPF_OPT_GRAPH_REPLAYS: read-only count of those replays in this process
- * (set to 0 to reset). */
+ * (set to 0 to reset).
+ * BPF_OPT_RELABEL_MIN: block GMRES on a system of at least this many
+ * unknowns (default 32768; 0 = never), with no left preconditioner and an
+ * ILU(0)/IC(0) or no right one, runs in a reverse Cuthill-McKee
+ * relabelling of A's pattern (built once per pattern and kept with the
+ * handles): A's relabelled copy for the products and relabelled views of the
+ * preconditioner's triangular operators (the same factor, levels and
+ * arithmetic per row), the blocks permuted at the layout conversion, so that
+ * gathered rows lie near the rows gathering them. */
 enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4,
-       BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7 };
+       BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7, BPF_OPT_RELABEL_MIN = 8 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

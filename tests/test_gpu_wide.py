@@ -162,8 +162,8 @@ def test_wide_itmax_status_and_p_above_n(bpf, gen, oracle):
 
 # block iterations of the one-block run to ||B - A X||_F <= 1e-10 with
 # ILU(0) on one MI355X (all passes; the bound allows two more)
-ITS_WIDE = {"case300": 8, "case1354pegase": 6, "case_ACTIVSg2000": 16, "case2869pegase": 10,
-            "case9241pegase": 13, "case_ACTIVSg10k": 12}
+ITS_WIDE = {"case300": 4, "case1354pegase": 5, "case_ACTIVSg2000": 9, "case2869pegase": 5,
+            "case9241pegase": 9, "case_ACTIVSg10k": 10}
 
 
 @pytest.mark.parametrize("name", ["case300", "case1354pegase", "case_ACTIVSg2000", "case2869pegase",
@@ -194,6 +194,35 @@ def test_reference_sweep_wide_cases(bpf, oracle, name):
     np.testing.assert_allclose(Xg, Xd, rtol=0, atol=1e-6 * max(1.0, np.abs(Xd).max()))
     if name in ITS_WIDE:
         assert st.niter <= ITS_WIDE[name] + 2
+
+
+@pytest.mark.parametrize("name,precond", [("case1354pegase", "bj"), ("case_ACTIVSg2000", "none"),
+                                          ("case2869pegase", "bj")])
+def test_reference_sweep_wide_weak_legs(bpf, oracle, name, precond):
+    # The sweep's block-Jacobi (blocks of 32) and unpreconditioned legs on
+    # cases where the Krylov space of the one wide block is exhausted before
+    # the residual reaches atol: the rank-deficient blocks that follow end the
+    # pass, and passes restart from the true residual, so the result never
+    # moves away from the solution: ||B - A X||_F <= ||B||_F (X0 = 0, the
+    # residual-minimizing property), finite, and at the tolerance whenever
+    # the run reports solved.  (Completing such blocks and iterating on them
+    # gave ||B - A X||_F of 1e93 to 1e135.)
+    Ah = bpf.mmread(os.path.join(MTX, f"Gx_{name}.mtx.gz"))
+    Bh = bpf.mmread_dense(os.path.join(MTX, f"Gu_{name}.mtx.gz"))
+    A = bpf.HipSparseMatrixCSR(Ah.indptr, Ah.indices, Ah.data)
+    B = bpf.HipMatrix.from_numpy(Bh)
+    kw = dict(atol=1e-10, rtol=0.0, memory=1, restart=False, itmax=50)
+    if precond == "bj":
+        BJ = bpf.BlockJacobiPreconditioner(A, max(1, Ah.shape[0] // 32))
+        X, st = bpf.block_gmres(A, B, N=BJ, **kw)
+    else:
+        X, st = bpf.block_gmres(A, B, **kw)
+    As = sp.csr_matrix((Ah.data, Ah.indices, Ah.indptr), shape=Ah.shape)
+    rn = np.linalg.norm(Bh - As @ X.to_numpy())
+    print(f"{name} {precond}: {st.niter} its, solved {st.solved}, ||B-AX||_F={rn:.2e}", flush=True)
+    assert np.isfinite(rn) and rn <= np.linalg.norm(Bh) * (1 + 1e-8)
+    if st.solved:
+        assert rn <= 1e-10 * 10
 
 
 @pytest.mark.parametrize("p", [136, 200])
