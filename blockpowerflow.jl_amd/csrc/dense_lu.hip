@@ -100,54 +100,79 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
   for (int p0 = 0; p0 < w; p0 += 16) {
     const int pw = min(16, w - p0);
     if (wave == 0) {
-      for (int kk = 0; kk < pw; ++kk) {
-        const int k = p0 + kk;
-        double d = G[k * kLd + k];
-        if (d == 0.0) {
-          if (lane == 0) atomicMin(status, base + k);
-          d = 1.0;
-          if (lane == 0) G[k * kLd + k] = 1.0;
-        }
-        if (lane > kk && lane < pw) G[(p0 + lane) * kLd + k] /= d;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        for (int e = lane; e < 256; e += 64) {
-          const int i = e >> 4, j = e & 15;
-          if (i > kk && j > kk && i < pw && j < pw)
-            G[(p0 + i) * kLd + p0 + j] -= G[(p0 + i) * kLd + k] * G[k * kLd + p0 + j];
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      // The 16 x 16 diagonal block in registers: lane l holds column
+      // c = l % 16, rows 4 (l / 16) + q in x[q]; pivot rows, multipliers and
+      // solved rows move between lanes by shuffles.  LU without pivoting
+      // (the same division and fused update per entry as an elimination in
+      // place), then U_PP^{-1} and L_PP^{-1} by column-oriented substitution
+      // on the identity; outside the pw x pw block both are the identity.
+      const int c = lane & 15, g = lane >> 4;
+      double x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = 4 * g + q;
+        x[q] = (r < pw && c < pw) ? G[(p0 + r) * kLd + p0 + c] : (r == c ? 1.0 : 0.0);
       }
-    }
-    // 16 x 16 triangular inverses (identity outside the pw x pw block).
-    if (wave == 0 && lane < 32) {
-      const int j = lane & 15;
-      double x[16];
-      if (lane < 16) {
-        // column j of U_PP^{-1}: x_i = (delta_ij - sum_{i<m<=j} U_im x_m) / U_ii
 #pragma unroll
-        for (int i = 15; i >= 0; --i) {
-          double v = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-          for (int m = i + 1; m < 16; ++m)
-            if (m <= j && m < pw) v -= G[(p0 + i) * kLd + p0 + m] * x[m];
-          x[i] = (i <= j && i < pw) ? v / G[(p0 + i) * kLd + p0 + i] : (i == j ? 1.0 : 0.0);
+      for (int kk = 0; kk < 16; ++kk) {
+        if (kk >= pw) break;
+        const int rowl = (kk >> 2) * 16;   // lanes holding row kk (register kk & 3)
+        double d = __shfl(x[kk & 3], rowl + kk);
+        if (d == 0.0) {
+          if (lane == 0) atomicMin(status, base + p0 + kk);
+          d = 1.0;
+          if (lane == rowl + kk) x[kk & 3] = 1.0;
         }
+        const double u = __shfl(x[kk & 3], rowl + c);   // G[kk][c]
 #pragma unroll
-        for (int i = 0; i < 16; ++i) sU[i][j] = x[i];
-      } else {
-        // column j of L_PP^{-1} (unit): x_i = delta_ij - sum_{j<=m<i} L_im x_m
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          double v = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-          for (int m = 0; m < i; ++m)
-            if (m >= j && i < pw) v -= G[(p0 + i) * kLd + p0 + m] * x[m];
-          x[i] = v;
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * g + q;
+          const double m = __shfl(x[q], g * 16 + kk) / d;   // G[r][kk] / d
+          if (r > kk && r < pw) {
+            if (c == kk) x[q] = m;
+            else if (c > kk && c < pw) x[q] = fma(-m, u, x[q]);
+          }
         }
+      }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) sL[i][j] = x[i];
+      for (int q = 0; q < 4; ++q) {
+        const int r = 4 * g + q;
+        if (r < pw && c < pw) G[(p0 + r) * kLd + p0 + c] = x[q];
+      }
+      double bu[4], bl[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bu[q] = bl[q] = (4 * g + q == c) ? 1.0 : 0.0;
+      // U X = I, rows kk = pw-1 .. 0
+#pragma unroll
+      for (int kk = 15; kk >= 0; --kk) {
+        if (kk >= pw) continue;
+        const int rowl = (kk >> 2) * 16;
+        const double ukk = __shfl(x[kk & 3], rowl + kk);
+        if (g == (kk >> 2)) bu[kk & 3] = bu[kk & 3] / ukk;
+        const double xk = __shfl(bu[kk & 3], rowl + c);   // X[kk][c]
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double urk = __shfl(x[q], g * 16 + kk);   // U[r][kk]
+          if (4 * g + q < kk) bu[q] = fma(-urk, xk, bu[q]);
+        }
+      }
+      // L X = I (unit diagonal), rows kk = 0 .. pw-1
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) {
+        if (kk >= pw) break;
+        const int rowl = (kk >> 2) * 16;
+        const double xk = __shfl(bl[kk & 3], rowl + c);   // X[kk][c]
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * g + q;
+          const double lrk = __shfl(x[q], g * 16 + kk);   // L[r][kk]
+          if (r > kk && r < pw) bl[q] = fma(-lrk, xk, bl[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        sU[4 * g + q][c] = bu[q];
+        sL[4 * g + q][c] = bl[q];
       }
     }
     __syncthreads();
