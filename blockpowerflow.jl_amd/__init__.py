@@ -102,9 +102,10 @@ def set_option(name: str, value) -> None:
     block GMRES run (its fixed chain of inner_its iterations) into a HIP graph
     on its first call and replays it on later outer iterations;
     "graph_replays" counts those replays (reset with 0); "relabel_min"
-    (default 32768, 0 = never) is the smallest system that block GMRES
-    solves in a reverse Cuthill-McKee relabelling of A's pattern, for the
-    locality of the rows its products and triangular solves gather."""
+    (default 32768, 0 = never) is the smallest system that block GMRES in
+    the "mixed" precision solves in a reverse Cuthill-McKee relabelling of
+    A's pattern, for the locality of the rows its products and triangular
+    solves gather (the "fp64" form keeps A's row order)."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbpf_amd.so")
-SOURCES = ["api.cpp", "profile.cpp", "analysis.cpp", "krylov.cpp", "poly.cpp", "solvers.cpp", "bicgstab.cpp",
+SOURCES = ["api.cpp", "host_common.cpp", "profile.cpp", "analysis.cpp", "krylov.cpp", "poly.cpp", "solvers.cpp", "bicgstab.cpp",
            "refactor.cpp", "ordering.cpp", "lu_host.cpp", "batch.cpp", "dense_api.cpp", "precond.cpp", "relabel.cpp", "noalloc.hip", "block_jacobi.hip", "dense_lu.hip", "wide.hip",
            "sparse_kernels.hip", "dense_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -10,15 +10,6 @@
 
 namespace bpf {
 
-static thread_local char g_err[1024] = "";
-
-void set_error(const char *fmt, ...) {
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(g_err, sizeof(g_err), fmt, ap);
-  va_end(ap);
-}
-
 void *BufPool::take(size_t nbytes, size_t *got) {
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -142,7 +133,7 @@ void bpf_csr_s::refresh_derived() {
 
 extern "C" {
 
-const char *bpf_last_error(void) { return g_err; }
+const char *bpf_last_error(void) { return bpf::last_error(); }
 const char *bpf_version(void) { return "bpf_amd 0.1 (gfx950)"; }
 
 int bpf_set_option(int option, long long value) {

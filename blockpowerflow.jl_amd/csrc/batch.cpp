@@ -492,30 +492,8 @@ int bpf_batch_solve(bpf_batch bt, const double *const *d_B, double *const *d_X,
     if (prm->precision == BPF_PRECISION_MIXED && prm->inner_width < 0 && prm->inner_its > 0)
       for (const auto &b : bt->bins) tuning = tuning || !(b->tuned_width > 0 && b->tuned_its == prm->inner_its);
     const int nthr = tuning ? 1 : std::max(1, std::min<int>(nbin, g_batch_threads));
-    if (nthr == 1) {
-      for (int i = 0; i < nbin; ++i) run(i);
-    } else {
-      std::atomic<int> next{0};
-      std::vector<std::exception_ptr> err(nthr);
-      std::vector<std::string> msg(nthr);
-      std::vector<std::thread> pool;
-      for (int t = 0; t < nthr; ++t)
-        pool.emplace_back([&, t] {
-          try {
-            BPF_CHECK_HIP(hipSetDevice(bt->ctx->device));
-            for (int i; (i = next.fetch_add(1)) < nbin;) run(i);
-          } catch (...) {
-            err[t] = std::current_exception();
-            msg[t] = bpf_last_error();   // the message is per thread
-          }
-        });
-      for (auto &th : pool) th.join();
-      for (int t = 0; t < nthr; ++t)
-        if (err[t]) {
-          set_error("%s", msg[t].c_str());
-          std::rethrow_exception(err[t]);
-        }
-    }
+    const int dev = bt->ctx->device;
+    run_pool(nbin, nthr, run, [dev] { BPF_CHECK_HIP(hipSetDevice(dev)); });
     for (const BinResult &r : res) {
       st.niter += r.iters;
       max_cycles = std::max(max_cycles, r.cycles);

@@ -14,21 +14,9 @@
 #include <vector>
 
 #include "../../include/bpf.h"
+#include "host_common.h"
 
 namespace bpf {
-
-// ---------------------------------------------------------------- errors
-void set_error(const char *fmt, ...);
-
-struct Error {
-  int code;
-};
-
-#define BPF_THROW(code, ...)          \
-  do {                                \
-    ::bpf::set_error(__VA_ARGS__);    \
-    throw ::bpf::Error{code};         \
-  } while (0)
 
 #define BPF_CHECK_HIP(expr)                                                    \
   do {                                                                         \
@@ -36,11 +24,6 @@ struct Error {
     if (e_ != hipSuccess)                                                      \
       BPF_THROW(BPF_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                 __FILE__, __LINE__);                                           \
-  } while (0)
-
-#define BPF_REQUIRE(cond, ...)                 \
-  do {                                         \
-    if (!(cond)) BPF_THROW(BPF_E_INVALID, __VA_ARGS__); \
   } while (0)
 
 // Runs f() and converts exceptions into status codes.
@@ -160,23 +143,11 @@ struct bpf_csr_s {
 
 namespace bpf {
 
-// A triangular operator in level-scheduled form.  Rows are stored in level
-// order; row t of the stored CSR is original row `order[t]`.  Off-diagonal
-// entries only; the diagonal (if not unit) is stored separately.  `gather`
-// maps stored entry -> position in the source value array (combined factor),
-// `dgather` maps stored row -> position of its diagonal.
-struct TriOp {
-  int n = 0;                   // rows of the operator (entries of order)
-  int ndim = 0;                // dimension of the vectors it acts on
-  bool unit = true;            // implicit unit diagonal
-  bool lower = true;           // dependencies on smaller (true) / larger indices
-  int nlevels = 0;
-  std::vector<int> h_lvl;      // level start offsets into order (nlevels+1)
-  std::vector<int> h_order;
-  std::vector<int> h_ptr, h_col, h_gather, h_dgather;
+// A triangular operator in level-scheduled form: the host level sets and
+// their device copies.
+struct TriOp : TriLevels {
   DevBuf order, ptr, col, gather, dgather, lvl;
   DevBuf val, val32, diag, diag32;   // refreshed from the factor values
-  int max_row_nnz = 0;
   // Sync-free execution: per-row completion flags (original row numbering)
   // tagged with the epoch of the launch that produced them, and a status
   // word set when a bounded dependency wait ran out.
@@ -185,17 +156,7 @@ struct TriOp {
   int checked = 0;             // epoch of the last status check
 };
 
-// Elimination plan of an IKJ factorization on a fixed pattern (ILU(0) on
-// A's pattern, LU on the fill pattern), built once per pattern on the host:
-// for every strictly lower entry e = (i, k), PivInfo at e holds pivot row
-// k's diagonal position and end and where its update positions start in
-// `map`: for each upper entry (k, j) of row k, in order, the position of
-// (i, j) relative to row i's start, or -1 where row i has no entry (a fill
-// ILU(0) drops).  So no pivot step searches row i's pattern and the loads of
-// a pivot depend on nothing computed during the elimination.
-struct PivInfo {
-  int diag, end, moff, pad_;
-};
+// Device copy of an elimination plan (elim_plan_host).
 struct ElimPlan {
   DevBuf pinfo, map;
   bool ready() const { return pinfo.ptr && map.ptr; }
@@ -233,22 +194,7 @@ void triop_check(TriOp &op, hipStream_t s);
 struct Seg { int l0, l1; bool wide; };
 std::vector<Seg> make_plan(const TriOp &op, int tail_rows);
 
-// Builds a TriOp from a CSR pattern.  kind: 0 = strict lower part (unit diag),
-// 1 = upper part (diag separate), 2 = lower part with diag separate,
-// 3 = transpose of the lower part (upper operator, diag separate).
-// A window restricts the operator to rows [row_lo, row_hi) and entries with
-// columns in [col_lo, col_hi) (-1 = n); dependencies on rows outside it are
-// taken as solved.  unit: no diagonal scaling; skip_empty: rows without
-// entries are left out (their y is not written).
-struct TriWindow {
-  int row_lo = 0, row_hi = -1, col_lo = 0, col_hi = -1;
-  bool unit = false, skip_empty = false;
-  // Relabelling of the vectors the operator acts on (original row i is
-  // stored at row relabel[i]): the same levels, rows and arithmetic, with
-  // row and dependency indices mapped and each level's rows in storage
-  // order.
-  const std::vector<int> *relabel = nullptr;
-};
+// Builds a TriOp from a CSR pattern (analyze_triop) and uploads it.
 void build_triop(TriOp &op, int n, const std::vector<int> &rowptr,
                  const std::vector<int> &colidx, int kind, hipStream_t s,
                  const TriWindow &win = TriWindow());

@@ -7,7 +7,7 @@
 #include <algorithm>
 #include <cmath>
 
-#include "bpf_internal.h"
+#include "host_common.h"
 #include "ordering.h"
 
 namespace bpf {

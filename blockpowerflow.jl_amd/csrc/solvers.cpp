@@ -97,11 +97,15 @@ void gmres_plain(bpf_ctx ctx, bpf_csr A, bpf_prec M, bpf_prec N, int p, const do
   const double t0 = now_s();
   const int n = A->n;
   hipStream_t s = ctx->stream;
-  // Large systems run in A's locality relabelling (the products and the
-  // preconditioner's triangular solves on relabelled copies; B, X0 and X
-  // permuted at the layout conversion).
+  // Large systems run the mixed form in A's locality relabelling (the
+  // products and the preconditioner's triangular solves on relabelled
+  // copies; B, X0 and X permuted at the layout conversion).  The FP64 form
+  // keeps A's row order, so its Gram sums run in the reference's order: the
+  // single-pass block MGS amplifies a change of summation order (70,000 buses:
+  // the first cycle stops with the true backward error just above its target
+  // in the relabelled order, and a second cycle doubles the iterations).
   const int *order = nullptr;
-  if (g_relabel_min > 0 && n >= g_relabel_min && !M &&
+  if (mixed && g_relabel_min > 0 && n >= g_relabel_min && !M &&
       (!N || N->kind == BPF_PREC_ILU0 || N->kind == BPF_PREC_IC0)) {
     CsrRelabel *rl = relabel_csr(A, s);
     if (N) N = relabel_prec(N, rl, s);

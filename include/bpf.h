@@ -88,9 +88,10 @@ const char *bpf_version(void);
  * it); 0 = every launch issued from the host.  Results are identical.
  * BPF_OPT_GRAPH_REPLAYS: read-only count of those replays in this process
  * (set to 0 to reset).
- * BPF_OPT_RELABEL_MIN: block GMRES on a system of at least this many
- * unknowns (default 32768; 0 = never), with no left preconditioner and an
- * ILU(0)/IC(0) or no right one, runs in a reverse Cuthill-McKee
+ * BPF_OPT_RELABEL_MIN: block GMRES in the MIXED precision on a system of at
+ * least this many unknowns (default 32768; 0 = never), with no left
+ * preconditioner and an ILU(0)/IC(0) or no right one, runs in a reverse
+ * Cuthill-McKee
  * relabelling of A's pattern (built once per pattern and kept with the
  * handles): A's relabelled copy for the products and relabelled views of the
  * preconditioner's triangular operators (the same factor, levels and

@@ -332,32 +332,35 @@ def test_inner_gmres_graph_replay_is_exact(bpf, gen, oracle):
 
 
 def test_relabelled_solve(bpf, gen, oracle):
-    # Systems of at least `relabel_min` unknowns are solved in a reverse
-    # Cuthill-McKee relabelling (products and ILU(0) triangular solves on
-    # relabelled copies of A and of the factor's operators, blocks permuted at
-    # the layout conversion): the same solution, in the same number of block
-    # iterations to within one (the Gram sums run over the rows in another
-    # order), for the plain and the mixed form; and the relabelled copies
-    # follow new values of A and a new factor.
+    # The mixed form on systems of at least `relabel_min` unknowns runs in a
+    # reverse Cuthill-McKee relabelling (products and ILU(0) triangular solves
+    # on relabelled copies of A and of the factor's operators, blocks permuted
+    # at the layout conversion): the same solution, in the same number of
+    # block iterations to within one (the Gram sums run over the rows in
+    # another order); the relabelled copies follow new values of A and a new
+    # factor.  The FP64 form keeps A's order: bitwise the same result.
     S, A, B, Xd = _setup(bpf, gen, oracle, 2000, 32, 49)
     P = bpf.ilu0(A, 32)
     kw = dict(N=P, ldiv=True, memory=40, berr_tol=1e-10, itmax=400)
+    mixed = dict(precision="mixed", inner_its=6, inner_width=16)
     old = bpf.get_option("relabel_min")
     try:
-        for prec, extra in (("fp64", {}), ("mixed", dict(inner_its=6, inner_width=16))):
-            out = {}
-            for rl in (0, 1000):
-                bpf.set_option("relabel_min", rl)
-                X, st = bpf.block_gmres(A, B, precision=prec, **kw, **extra)
-                out[rl] = (X.to_numpy(), st.niter, st.solved)
-            assert out[1000][2] and abs(out[1000][1] - out[0][1]) <= 1, (prec, out[0][1], out[1000][1])
-            be = oracle.backward_error(S.indptr, S.indices, S.data, out[1000][0], S.B)
-            assert be.max() <= 1e-10
-            assert np.all(_rel(out[1000][0], Xd) < 1e-5)
+        out = {}
+        for rl in (0, 1000):
+            bpf.set_option("relabel_min", rl)
+            X, st = bpf.block_gmres(A, B, **kw, **mixed)
+            out[rl] = (X.to_numpy(), st.niter, st.solved)
+            Xp, _ = bpf.block_gmres(A, B, precision="fp64", **kw)
+            out[rl, "fp64"] = Xp.to_numpy()
+        assert out[1000][2] and abs(out[1000][1] - out[0][1]) <= 1, (out[0][1], out[1000][1])
+        be = oracle.backward_error(S.indptr, S.indices, S.data, out[1000][0], S.B)
+        assert be.max() <= 1e-10
+        assert np.all(_rel(out[1000][0], Xd) < 1e-5)
+        np.testing.assert_array_equal(out[1000, "fp64"], out[0, "fp64"])
         bpf.set_option("relabel_min", 1000)
         A.set_values(2.0 * S.data)
         P.factor_(A)
-        X2, st2 = bpf.block_gmres(A, B, precision="fp64", **kw)
+        X2, st2 = bpf.block_gmres(A, B, **kw, **mixed)
         assert st2.solved
         np.testing.assert_allclose(X2.to_numpy(), 0.5 * Xd, rtol=0, atol=1e-7 * np.abs(Xd).max())
     finally:
