@@ -568,6 +568,108 @@ __global__ __launch_bounds__(kGemmThreads, 4) void block_gemm_kernel(GemmArgs a)
 }
 
 // Y[r, c0+b] = beta*Y + alpha * sum_i sum_a V_i[r, c0+a] * Psi_i[a][b] for
+// FP64 blocks of width 128 (groups with widths and offsets that are
+// multiples of 4, GemmArgs::vec4): the outer block-MGS update.  A workgroup
+// of 16 wavefronts owns 256 rows, one 16-row tile per wavefront, and keeps
+// one whole Psi_i (128 x 128, padded rows) in LDS, loaded once per basis
+// block, so its wavefronts meet at a barrier once per block instead of once
+// per 16-column chunk.  The basis rows go straight from memory into the
+// matrix-core operands: in each 16-column chunk lane l loads columns
+// 4(l/16)..4(l/16)+3 of row l%16 of its tile as two 16-byte vectors, and
+// MFMA step m takes column 4(l/16)+m as A[i = l%16][k = l/16] with
+// Psi[4(l/16)+m][16y + l%16] as B[k][j] for output tile y, which sums over
+// every column once.  The next chunk's loads are in flight during the
+// current chunk's 32 MFMAs.  Rows past the group re-read its last row and
+// column groups past its width read columns 0..3 (their Psi rows are zero);
+// each wavefront writes only the rows it read, so Y may alias the (single)
+// V block.
+constexpr int kG128Waves = 16;
+constexpr int kG128Rows = 16 * kG128Waves;
+constexpr int kG128Ld = 130;   // Psi row stride in LDS (doubles)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(64 * kG128Waves, 1) void gemm128d_kernel(GemmArgs a) {
+  __shared__ double sp[128 * kG128Ld];
+  const int g = blockIdx.y;
+  const GroupDesc gd = a.groups[g];
+  if (blockIdx.x * kG128Rows >= gd.nr) return;
+  const int r0 = gd.r0 + blockIdx.x * kG128Rows;
+  const int rend = gd.r0 + gd.nr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int w = gd.w;
+  const bool takes_part = a.use_k ? gd.K > 0 : gd.active != 0;
+  double *Y = static_cast<double *>(a.Y) + gd.c0;
+  if (!takes_part) {
+    if (a.beta == 0.0 && !a.keep_skipped)
+      for (int e = tid; e < kG128Rows * w; e += blockDim.x) {
+        const int r = r0 + e / w, c = e % w;
+        if (r < rend) Y[(size_t)r * a.ldy + c] = 0.0;
+      }
+    return;
+  }
+  const int nblk = a.use_k ? min(a.nb, gd.K) : a.nb;
+  const int kch = (w + 15) / 16;
+  const int nchunks = nblk * kch;
+  const int row = min(r0 + wave * 16 + i16, rend - 1);
+  auto vrow = [&](int c) {
+    const int i = c / kch, k = (c % kch) * 16 + 4 * kq;
+    const double *V = static_cast<const double *>(a.Vp ? a.Vp[a.i0 + i] : a.V1) + gd.c0;
+    return reinterpret_cast<const dbl2 *>(V + (size_t)row * a.ldv + (k < w ? k : 0));
+  };
+  dbl4 acc[8];
+#pragma unroll
+  for (int y = 0; y < 8; ++y) acc[y] = dbl4{0.0, 0.0, 0.0, 0.0};
+  dbl2 n0 = dbl2{0.0, 0.0}, n1 = dbl2{0.0, 0.0};
+  if (nchunks > 0) {
+    const dbl2 *vp = vrow(0);
+    n0 = __builtin_nontemporal_load(vp);
+    n1 = __builtin_nontemporal_load(vp + 1);
+  }
+  for (int c = 0; c < nchunks; ++c) {
+    const int ch = c % kch;
+    if (ch == 0) {
+      // Psi of the next basis block into LDS (zero past the group's width)
+      const double *P = a.psi + (size_t)(c / kch) * a.psi_istride + (size_t)g * a.psi_gstride;
+      __syncthreads();
+      for (int e = tid; e < 128 * 128; e += 64 * kG128Waves) {
+        const int k = e >> 7, j = e & 127;
+        sp[k * kG128Ld + j] = (k < w && j < w) ? P[(size_t)k * a.ldpsi + j] : 0.0;
+      }
+      __syncthreads();
+    }
+    const double av[4] = {n0.x, n0.y, n1.x, n1.y};
+    if (c + 1 < nchunks) {
+      const dbl2 *vp = vrow(c + 1);
+      n0 = __builtin_nontemporal_load(vp);
+      n1 = __builtin_nontemporal_load(vp + 1);
+    }
+    const double *spk = sp + (ch * 16 + 4 * kq) * kG128Ld + i16;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double fb[8];
+#pragma unroll
+      for (int y = 0; y < 8; ++y) fb[y] = spk[m * kG128Ld + y * 16];
+#pragma unroll
+      for (int y = 0; y < 8; ++y) acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], fb[y], acc[y], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int y = 0; y < 8; ++y) {
+    const int c = y * 16 + i16;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + wave * 16 + kq + 4 * q;
+      if (r < rend && c < w) {
+        double *yp = Y + (size_t)r * a.ldy + c;
+        const double v = a.alpha * acc[y][q];
+        *yp = a.beta == 0.0 ? v : a.beta * *yp + v;
+      }
+    }
+  }
+}
+
+// Y[r, c0+b] = beta*Y + alpha * sum_i sum_a V_i[r, c0+a] * Psi_i[a][b] for
 // width-16 groups of an FP32 basis (the inner solve's updates): FP32 matrix
 // cores (v_mfma_f32_16x16x4f32) accumulating the 16 nb products of each
 // output in FP32, Psi rounded to FP32.  Lane l owns row l%16 of a 16-row
@@ -1843,6 +1945,9 @@ void launch_block_gemm(hipStream_t s, GemmArgs a, int S, int ng) {
     const dim3 grid16(xcd_grid(nchunk, ng));
     if (a.vec4) hipLaunchKernelGGL(gemm16f_kernel<true>, grid16, dim3(256), 0, s, a, ng);
     else hipLaunchKernelGGL(gemm16f_kernel<false>, grid16, dim3(256), 0, s, a, ng);
+  } else if (S == 128 && a.vec4 && std::is_same<TVV, double>::value && std::is_same<TY, double>::value) {
+    BPF_REQUIRE(((a.n + kG128Rows - 1) / kG128Rows) < (1L << 31), "update grid too large");
+    hipLaunchKernelGGL(gemm128d_kernel, dim3((a.n + kG128Rows - 1) / kG128Rows, ng), dim3(64 * kG128Waves), 0, s, a);
   } else switch (S) {
     case 16: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 16>), grid, blk, 0, s, a); break;
     case 32: hipLaunchKernelGGL((block_gemm_kernel<TVV, TY, 32>), grid, blk, 0, s, a); break;
