@@ -64,10 +64,10 @@ KERNELS = {
     "factor": ("fac_level_kernel, fac_tail_kernel, fac_partial_kernel", "hbm", "fac_level_kernel"),
     "dense_lu": ("dlu_diag_kernel, dgemm_kernel, copy_block_kernel", "mfma", "dense_lu"),
     "proj": ("proj_partial_kernel or proj16_kernel, then proj_reduce_kernel", "mfma", "projection"),
-    "gemm": ("block_gemm_kernel", "mfma", "block_gemm_kernel"),
+    "gemm": ("gemm128d_kernel (FP64 128-wide) or block_gemm_kernel", "mfma", "block_gemm_kernel"),
     "proj16": ("proj16_kernel or proj_partial_kernel (widths <= 32), then proj_reduce_kernel", "hbm",
                "projection_narrow"),
-    "gemm16": ("block_gemm_kernel (widths <= 32)", "hbm", "block_gemm_narrow"),
+    "gemm16": ("gemm16f_kernel or block_gemm_kernel (widths <= 32)", "hbm", "block_gemm_narrow"),
 }
 
 

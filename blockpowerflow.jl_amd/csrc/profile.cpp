@@ -64,6 +64,22 @@ int begin(hipStream_t s) {
   return (int)g.open.size() - 1;
 }
 
+ExtPair ext_begin() {
+  ExtPair p;
+  if (!g.on) return p;
+  std::lock_guard<std::mutex> lk(mu);
+  p.a = take();
+  p.b = take();
+  return p;
+}
+
+void ext_end(const ExtPair &p, Family f, double bytes, double flops) {
+  if (!p.a || !g.on) return;
+  std::lock_guard<std::mutex> lk(mu);
+  g.done.push_back(Rec{p.a, p.b, f, bytes, flops});
+  if (g.done.size() > 4096) drain();
+}
+
 void end(int token, hipStream_t s, Family f, double bytes, double flops) {
   if (token < 0 || !g.on) return;
   std::lock_guard<std::mutex> lk(mu);

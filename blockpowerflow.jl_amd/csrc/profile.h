@@ -18,6 +18,15 @@ bool enabled();
 // Records a start event on s when profiling is on; returns a token for end().
 int begin(hipStream_t s);
 void end(int token, hipStream_t s, Family f, double bytes, double flops);
+// A start / stop event pair for hipExtLaunchKernelGGL, which stamps them
+// with the kernel's own start and end (no event packets of their own, whose
+// dispatch a separate pair would add to a short launch); both null when
+// profiling is off.  ext_end() registers the pair like end().
+struct ExtPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+ExtPair ext_begin();
+void ext_end(const ExtPair &p, Family f, double bytes, double flops);
 
 }  // namespace prof
 }  // namespace bpf

@@ -6,6 +6,8 @@
 // so a row gather is one coalesced load of p elements instead of p strided
 // loads.  A group of G lanes (G = power of two <= 64) owns one matrix row and
 // spreads the p columns over its lanes (CPL columns per lane).
+#include <hip/hip_ext.h>
+
 #include "bpf_internal.h"
 #include "profile.h"
 
@@ -881,15 +883,21 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
       for (int l = seg.l0; l < seg.l1; ++l) {
         const int t0 = op.h_lvl[l], t1 = op.h_lvl[l + 1];
         dim3 grid((t1 - t0 + rpb - 1) / rpb);
-        const int tok = prof::begin(s);
+        // short launches: kernel-stamped timer events (no packets of their own)
+        const prof::ExtPair ep = prof::ext_begin();
+#define BPF_TRI1(U, C, VV)                                                                              \
+  do {                                                                                                   \
+    if (ep.a)                                                                                            \
+      hipExtLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, VV>), grid, dim3(kBlock), 0, s, ep.a, ep.b,   \
+                            0, a, t0, t1, x, ldx, y, ldy, p, glog);                                      \
+    else                                                                                                 \
+      hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, VV>), grid, dim3(kBlock), 0, s, a, t0, t1, x,    \
+                         ldx, y, ldy, p, glog);                                                          \
+  } while (0)
 #define BPF_TRI(U, C)                                                                                   \
   do {                                                                                                   \
-    if (vec)                                                                                             \
-      hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, true>), grid, dim3(kBlock), 0, s, a, t0, t1, x,  \
-                         ldx, y, ldy, p, glog);                                                          \
-    else                                                                                                 \
-      hipLaunchKernelGGL((tri_level_kernel<TV, TV, U, C, false>), grid, dim3(kBlock), 0, s, a, t0, t1, x, \
-                         ldx, y, ldy, p, glog);                                                          \
+    if (vec) BPF_TRI1(U, C, true);                                                                       \
+    else BPF_TRI1(U, C, false);                                                                          \
   } while (0)
         if (op.unit) {
           if (cpl == 1) BPF_TRI(true, 1); else if (cpl == 2) BPF_TRI(true, 2); else BPF_TRI(true, 4);
@@ -897,20 +905,26 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
           if (cpl == 1) BPF_TRI(false, 1); else if (cpl == 2) BPF_TRI(false, 2); else BPF_TRI(false, 4);
         }
 #undef BPF_TRI
+#undef BPF_TRI1
         const double rows = t1 - t0, nz = op.h_ptr[t1] - op.h_ptr[t0], w = sizeof(TV);
-        prof::end(tok, s, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
-                  2.0 * nz * p);
+        prof::ext_end(ep, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
+                      2.0 * nz * p);
       }
     } else {
-      const int tok = prof::begin(s);
+      const prof::ExtPair ep = prof::ext_begin();
+#define BPF_TAIL1(U, C, VV)                                                                                 \
+  do {                                                                                                       \
+    if (ep.a)                                                                                                \
+      hipExtLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(1), dim3(kTailBlock), 0, s, ep.a, ep.b, \
+                            0, a, seg.l0, seg.l1, x, ldx, y, ldy, p, glog);                                  \
+    else                                                                                                     \
+      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,     \
+                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
+  } while (0)
 #define BPF_TAIL(U, C)                                                                                      \
   do {                                                                                                       \
-    if (vec)                                                                                                 \
-      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, true>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,   \
-                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
-    else                                                                                                     \
-      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, false>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,  \
-                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
+    if (vec) BPF_TAIL1(U, C, true);                                                                          \
+    else BPF_TAIL1(U, C, false);                                                                             \
   } while (0)
       if (op.unit) {
         if (cpl == 1) BPF_TAIL(true, 1); else if (cpl == 2) BPF_TAIL(true, 2); else BPF_TAIL(true, 4);
@@ -918,10 +932,11 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
         if (cpl == 1) BPF_TAIL(false, 1); else if (cpl == 2) BPF_TAIL(false, 2); else BPF_TAIL(false, 4);
       }
 #undef BPF_TAIL
+#undef BPF_TAIL1
       const int t0 = op.h_lvl[seg.l0], t1 = op.h_lvl[seg.l1];
       const double rows = t1 - t0, nz = op.h_ptr[t1] - op.h_ptr[t0], w = sizeof(TV);
-      prof::end(tok, s, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
-                2.0 * nz * p);
+      prof::ext_end(ep, prof::TRSV, nz * (w + 4.0) + rows * (12.0 + (op.unit ? 0.0 : w)) + 2.0 * rows * p * w,
+                    2.0 * nz * p);
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
