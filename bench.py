@@ -798,6 +798,8 @@ def main():
                          "instantiation here)")
     ap.add_argument("--no-kernel-timers", action="store_true",
                     help="skip the extra kernel-timer step (no roofline families; for kernel-trace runs)")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="process-wide library option (bpf.set_option) set before the run; repeatable")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three device-to-device copies of 1 GiB (the known-size kernel that "
                          "scales the HBM byte counters) and exit")
@@ -825,6 +827,9 @@ def main():
     bpf = ge.package()
     if args.no_graph:
         bpf.set_option("graph", 0)
+    for opt in args.option:
+        name, _, value = opt.partition("=")
+        bpf.set_option(name, int(value))
     gen = ge._load("bpf_generator", os.path.join(ge.PKG_DIR, "generator.py"))
     if args.workload == "c4":
         res = bench_batch(args, bpf, gen)

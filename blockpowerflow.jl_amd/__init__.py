@@ -82,7 +82,8 @@ def default_context() -> Context:
 
 
 _OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5, "graph": 6,
-            "graph_replays": 7, "relabel_min": 8}
+            "graph_replays": 7, "relabel_min": 8, "tail_slab": 9,
+            "dense_inverse": 10}
 
 
 def set_option(name: str, value) -> None:
@@ -105,7 +106,15 @@ def set_option(name: str, value) -> None:
     (default 32768, 0 = never) is the smallest system that block GMRES in
     the "mixed" precision solves in a reverse Cuthill-McKee relabelling of
     A's pattern, for the locality of the rows its products and triangular
-    solves gather (the "fp64" form keeps A's row order)."""
+    solves gather (the "fp64" form keeps A's row order); "tail_slab" (default
+    64, or 0) is the column width of the slabs that the narrow-level part of
+    a triangular solve over more right-hand sides than that is split into, one
+    workgroup per slab (0 = one workgroup for all columns; results are
+    identical); "dense_inverse" (default 1) solves the dense tail of a
+    refactorization handle by explicit inverses of its triangular factors,
+    built by the first solve that uses them after each refactorization: 0 =
+    never (blocked substitution), 1 = in the FP32 solves that refinement
+    follows, 2 = in every solve."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -154,6 +154,14 @@ int bpf_set_option(int option, long long value) {
       BPF_REQUIRE(value >= 1, "wide_min must be at least 1");
       g_wide_min = (int)std::min<long long>(value, 1 << 30);
     }
+    else if (option == BPF_OPT_DENSE_INVERSE) {
+      BPF_REQUIRE(value >= 0 && value <= 2, "dense_inverse must be 0, 1 or 2");
+      g_dense_inverse = (int)value;
+    }
+    else if (option == BPF_OPT_TAIL_SLAB) {
+      BPF_REQUIRE(value == 0 || value == 64, "tail_slab must be 0 or 64");
+      g_tail_slab = (int)value;
+    }
     else if (option == BPF_OPT_BATCH_THREADS) {
       BPF_REQUIRE(value >= 1 && value <= 64, "batch threads must be in 1..64");
       g_batch_threads = (int)value;
@@ -171,6 +179,8 @@ int bpf_get_option(int option, long long *value) {
     else if (option == BPF_OPT_WIDE_MIN) *value = g_wide_min;
     else if (option == BPF_OPT_GRAPH) *value = g_graph ? 1 : 0;
     else if (option == BPF_OPT_RELABEL_MIN) *value = g_relabel_min;
+    else if (option == BPF_OPT_TAIL_SLAB) *value = g_tail_slab;
+    else if (option == BPF_OPT_DENSE_INVERSE) *value = g_dense_inverse;
     else if (option == BPF_OPT_GRAPH_REPLAYS) *value = g_graph_replays;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });

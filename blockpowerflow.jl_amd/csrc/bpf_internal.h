@@ -172,6 +172,12 @@ extern std::atomic<bool> g_overlap_hess;
 // replays so far.
 extern std::atomic<bool> g_graph;
 extern std::atomic<long long> g_graph_replays;
+// Process-wide column width (64, or 0 = one workgroup) of the slabs the
+// narrow-level part of a triangular solve is split into, one workgroup each.
+extern std::atomic<int> g_tail_slab;
+// Process-wide: solves of a refactorization handle's dense tail by explicit
+// inverse factors (0 never, 1 FP32 solves, 2 all solves).
+extern std::atomic<int> g_dense_inverse;
 // Process-wide number of host threads a batched solve runs its bins on.
 extern std::atomic<int> g_batch_threads;
 // Process-wide: block GMRES (FP64) with at least this many right-hand sides

@@ -8,18 +8,23 @@
 // cores, and the matching triangular solves are blocked the same way:
 //
 //   for each nb-column panel j0 of S:
-//     dlu_diag_kernel   LU of the nb x nb diagonal block in LDS (16-column
-//                       sub-panels, MFMA updates) + inverses of its L and U
-//     dgemm_kernel      P = S21 U11^{-1},  Q = L11^{-1} S12   (scratch)
-//     copy              S21 <- P, S12 <- Q
-//     dgemm_kernel      S22 -= P Q
+//     dlu_diag_kernel     LU of the nb x nb diagonal block in LDS (16-column
+//                         sub-panels, MFMA updates) + inverses of its L and U
+//     dgemm_batch_kernel  P = S21 U11^{-1},  Q = L11^{-1} S12   (scratch)
+//     copy                S21 <- P, S12 <- Q
+//     dgemm_batch_kernel  S22 -= P Q
 //
-//   solves (m x p right-hand sides, FP64): Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b,
-//   then Z_b = U_bb^{-1} Y_b, Y_{<b} -= U_{<b,b} Z_b.
+//   solves (m x p right-hand sides, FP64), blocked substitution:
+//     Y_b = L_bb^{-1} X_b, X_{>b} -= L_{>b,b} Y_b, then Z_b = U_bb^{-1} Y_b,
+//     Y_{<b} -= U_{<b,b} Z_b;
+//   or by explicit inverse factors L^{-1}, U^{-1} (built from the diagonal-
+//   block inverses by recursive doubling): two block-triangular products.
 #include "dense_lu.h"
 #include "profile.h"
 
 #include <algorithm>
+#include <cstdint>
+#include <vector>
 
 namespace bpf {
 namespace {
@@ -30,50 +35,6 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
   // D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and
   // B[l/16][l%16] and holds D[l/16 + 4q][l%16] in acc[q].
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-}
-
-// C (M x N) = alpha A (M x K) B (K x N) (+ C when ACCUM), row-major FP64.
-// 64 x 64 output tile per workgroup (4 wavefronts of 16 x 64), K in chunks
-// of 16 staged in LDS.  C must not overlap A or B.
-template <bool ACCUM>
-__global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double alpha, const double *A,
-                                                    int lda, const double *B, int ldb, double *C, int ldc) {
-  __shared__ double As[64][17];
-  __shared__ double Bs[16][65];
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane >> 4, lc = lane & 15;
-  dbl4 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + q * 256;
-      const int r = e >> 4, k = e & 15;
-      As[r][k] = (m0 + r < M && k0 + k < K) ? A[(size_t)(m0 + r) * lda + k0 + k] : 0.0;
-      const int rb = e >> 6, c = e & 63;
-      Bs[rb][c] = (k0 + rb < K && n0 + c < N) ? B[(size_t)(k0 + rb) * ldb + n0 + c] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k4 = 0; k4 < 4; ++k4) {
-      const double a = As[wave * 16 + lc][k4 * 4 + lr];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc[t] = mfma4(a, Bs[k4 * 4 + lr][t * 16 + lc], acc[t]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = m0 + wave * 16 + lr + 4 * q, c = n0 + t * 16 + lc;
-      if (r < M && c < N) {
-        double *cp = C + (size_t)r * ldc + c;
-        *cp = ACCUM ? *cp + alpha * acc[t][q] : alpha * acc[t][q];
-      }
-    }
 }
 
 constexpr int kLd = kDluNb + 1;
@@ -373,14 +334,6 @@ __global__ __launch_bounds__(256) void tail_move_kernel(int cnt, const int *pos,
   else fac[pos[e]] = S[didx[e]];
 }
 
-void gemm(hipStream_t s, bool accum, int M, int N, int K, double alpha, const double *A, int lda,
-          const double *B, int ldb, double *C, int ldc) {
-  if (M <= 0 || N <= 0 || K <= 0) return;
-  const dim3 grid((N + 63) / 64, (M + 63) / 64);
-  if (accum) hipLaunchKernelGGL(dgemm_kernel<true>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc);
-  else hipLaunchKernelGGL(dgemm_kernel<false>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc);
-}
-
 void copy_block(hipStream_t s, int rows, int cols, const double *src, int lds, double *dst, int ldd) {
   const size_t m = (size_t)rows * cols;
   if (m == 0) return;
@@ -388,7 +341,239 @@ void copy_block(hipStream_t s, int rows, int cols, const double *src, int lds, d
                      lds, dst, ldd);
 }
 
+// ------------------------------------------------ inverse triangular factors
+// One GEMM of a batch: C = alpha A B + beta C, row-major FP64.  M and K are
+// multiples of 16, N any; tri = 1: A is block lower triangular (a row tile in
+// kDluNb block b takes K < (b + 1) kDluNb), tri = 2: block upper
+// (K >= b kDluNb).  C must not overlap A or B.
+struct GemmDesc {
+  const double *A, *B;
+  double *C;
+  int M, N, K, lda, ldb, ldc;
+  double alpha, beta;
+  int tri;
+  int tiles_n;   // output tiles across N
+  int tile0;     // first output tile of this GEMM in the batch
+};
+constexpr int kMaxGemm = 8;
+struct GemmBatch {
+  GemmDesc d[kMaxGemm];
+  int nd, ntiles;
+};
+
+// One workgroup per (16 TM) x (16 TN) output tile of one GEMM of the batch;
+// its four wavefronts split the k range (wavefront w takes the 16-wide
+// chunks w, w + 4, ...) and their partial tiles are summed in LDS in a fixed
+// order (bitwise reproducible).  Operands come straight from global memory:
+// per chunk, lane group g = lane / 16 owns k = k0 + 4g .. k0 + 4g + 3 (a
+// 32-byte run of each of its A rows and four B rows), so MFMA step s pairs A
+// and B on k = k0 + 4g + s.  The next chunk's operands are loaded before the
+// current chunk's products.
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void dgemm_batch_kernel(GemmBatch b) {
+  __shared__ dbl4 red[3][TM * TN][64];
+  const int t = blockIdx.x;
+  int id = 0;
+  while (id + 1 < b.nd && t >= b.d[id + 1].tile0) ++id;
+  const GemmDesc g = b.d[id];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g4 = lane >> 4, l16 = lane & 15;
+  const int tt = t - g.tile0;
+  const int m0 = (tt / g.tiles_n) * 16 * TM, n0 = (tt % g.tiles_n) * 16 * TN;
+  int kb = 0, ke = g.K;
+  if (g.tri == 1) ke = min(g.K, (m0 / kDluNb + 1) * kDluNb);
+  else if (g.tri == 2) kb = (m0 / kDluNb) * kDluNb;
+  const double *ar[TM];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) ar[i] = g.A + (size_t)(m0 + 16 * i + l16) * g.lda + 4 * g4;
+  const double *bc[TN];
+  bool ok[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int c = n0 + 16 * j + l16;
+    ok[j] = c < g.N;
+    bc[j] = g.B + (size_t)(4 * g4) * g.ldb + (ok[j] ? c : 0);
+  }
+  dbl4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+  double2 a[TM][2], an[TM][2];
+  double bv[TN][4], bn[TN][4];
+  auto load = [&](int k, double2 (&aa)[TM][2], double (&bb)[TN][4]) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      aa[i][0] = *reinterpret_cast<const double2 *>(ar[i] + k);
+      aa[i][1] = *reinterpret_cast<const double2 *>(ar[i] + k + 2);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bb[j][q] = ok[j] ? bc[j][(size_t)(k + q) * g.ldb] : 0.0;
+  };
+  const int k1 = kb + 16 * wave;
+  if (k1 < ke) load(k1, a, bv);
+  for (int k = k1; k < ke; k += 64) {
+    if (k + 64 < ke) load(k + 64, an, bn);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const double av = q == 0 ? a[i][0].x : q == 1 ? a[i][0].y : q == 2 ? a[i][1].x : a[i][1].y;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = mfma4(av, bv[j][q], acc[i][j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      a[i][0] = an[i][0];
+      a[i][1] = an[i][1];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bv[j][q] = bn[j][q];
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) red[wave - 1][i * TN + j][lane] = acc[i][j];
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] += red[w][i * TN + j][lane];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    if (!ok[j]) continue;
+    const int c = n0 + 16 * j + l16;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double *cp = g.C + (size_t)(m0 + 16 * i + g4 + 4 * q) * g.ldc + c;
+        *cp = g.beta == 0.0 ? g.alpha * acc[i][j][q] : fma(g.beta, *cp, g.alpha * acc[i][j][q]);
+      }
+  }
+}
+
+// The inverses of the diagonal blocks into the diagonal blocks of Zl / Zu.
+__global__ __launch_bounds__(256) void diag_inv_copy_kernel(int nblk, int m, const double *Linv,
+                                                            const double *Uinv, double *Zl, double *Zu) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)nblk * kDluNb * kDluNb) return;
+  const int b = (int)(e / (kDluNb * kDluNb)), r = (int)(e / kDluNb % kDluNb), c = (int)(e % kDluNb);
+  const size_t d = (size_t)(b * kDluNb + r) * m + b * kDluNb + c;
+  Zl[d] = Linv[e];
+  Zu[d] = Uinv[e];
+}
+
+void add_gemm(std::vector<GemmDesc> &v, int M, int N, int K, double alpha, const double *A, int lda,
+              const double *B, int ldb, double beta, double *C, int ldc, int tri) {
+  BPF_REQUIRE(M % 16 == 0 && K % 16 == 0 && lda % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0,
+              "dense-tail GEMM operand shape");
+  if (M == 0 || N <= 0) return;
+  GemmDesc d{A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tri, 0, 0};
+  v.push_back(d);
+}
+
+// Runs the GEMMs of v (independent of each other), up to kMaxGemm per launch.
+template <int TM, int TN>
+void run_gemms(hipStream_t s, const std::vector<GemmDesc> &v) {
+  for (size_t i0 = 0; i0 < v.size(); i0 += kMaxGemm) {
+    GemmBatch b{};
+    b.nd = (int)std::min<size_t>(kMaxGemm, v.size() - i0);
+    int tiles = 0;
+    for (int i = 0; i < b.nd; ++i) {
+      GemmDesc d = v[i0 + i];
+      d.tiles_n = (d.N + 16 * TN - 1) / (16 * TN);
+      d.tile0 = tiles;
+      tiles += d.M / (16 * TM) * d.tiles_n;
+      b.d[i] = d;
+    }
+    b.ntiles = tiles;
+    if (tiles == 0) continue;
+    hipLaunchKernelGGL((dgemm_batch_kernel<TM, TN>), dim3(tiles), dim3(256), 0, s, b);
+  }
+}
+
+struct Comb {
+  int lo, mid, hi;   // block ranges [lo, mid) and [mid, hi)
+};
+
+void split_blocks(int lo, int hi, int depth, std::vector<std::vector<Comb>> &lv) {
+  if (hi - lo <= 1) return;
+  const int mid = lo + (hi - lo + 1) / 2;
+  split_blocks(lo, mid, depth + 1, lv);
+  split_blocks(mid, hi, depth + 1, lv);
+  if ((int)lv.size() <= depth) lv.resize(depth + 1);
+  lv[depth].push_back({lo, mid, hi});
+}
+
 }  // namespace
+
+void dense_lu_invert(hipStream_t s, int m, const double *S, const double *Linv, const double *Uinv,
+                     double *Zl, double *Zu, double *T) {
+  BPF_REQUIRE(m % kDluNb == 0, "dense tail of %d rows is not a multiple of %d", m, kDluNb);
+  const int nb = kDluNb, nblk = m / nb;
+  const size_t cnt = (size_t)nblk * nb * nb;
+  hipLaunchKernelGGL(diag_inv_copy_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, nblk, m, Linv,
+                     Uinv, Zl, Zu);
+  // Recursive doubling, deepest combines first.  For ranges A = [lo, mid),
+  // C = [mid, hi) of a unit lower L and an upper U:
+  //   (L^{-1})_CA = -(L^{-1})_CC (L_CA (L^{-1})_AA)
+  //   (U^{-1})_AC = -(U^{-1})_AA (U_AC (U^{-1})_CC)
+  // with the inner products staged in T at the block's own position (the
+  // combines of one depth cover disjoint blocks).
+  std::vector<std::vector<Comb>> lv;
+  split_blocks(0, nblk, 0, lv);
+  for (int d = (int)lv.size() - 1; d >= 0; --d) {
+    std::vector<GemmDesc> first, second;
+    for (const Comb &c : lv[d]) {
+      const int a0 = c.lo * nb, c0 = c.mid * nb, a = (c.mid - c.lo) * nb, cc = (c.hi - c.mid) * nb;
+      add_gemm(first, cc, a, a, 1.0, S + (size_t)c0 * m + a0, m, Zl + (size_t)a0 * m + a0, m, 0.0,
+               T + (size_t)c0 * m + a0, m, 0);
+      add_gemm(first, a, cc, cc, 1.0, S + (size_t)a0 * m + c0, m, Zu + (size_t)c0 * m + c0, m, 0.0,
+               T + (size_t)a0 * m + c0, m, 0);
+      add_gemm(second, cc, a, cc, -1.0, Zl + (size_t)c0 * m + c0, m, T + (size_t)c0 * m + a0, m, 0.0,
+               Zl + (size_t)c0 * m + a0, m, 1);
+      add_gemm(second, a, cc, a, -1.0, Zu + (size_t)a0 * m + a0, m, T + (size_t)a0 * m + c0, m, 0.0,
+               Zu + (size_t)a0 * m + c0, m, 2);
+    }
+    run_gemms<2, 2>(s, first);
+    run_gemms<2, 2>(s, second);
+  }
+  BPF_CHECK_HIP(hipGetLastError());
+}
+
+void dense_lu_solve_inv(hipStream_t s, int m, int p, const double *S, const double *Linv, const double *Uinv,
+                        bool build, double *Zl, double *Zu, double *T, double *X, double *Y) {
+  const int tok = prof::begin(s);
+  double bytes = 0.0, flops = 0.0;
+  if (build) {
+    dense_lu_invert(s, m, S, Linv, Uinv, Zl, Zu, T);
+    // two triangular inverses from their diagonal blocks (m^3 / 3 each): S
+    // read, both written once
+    flops += 2.0 / 3.0 * m * (double)m * m;
+    bytes += 2.0 * m * (double)m * sizeof(double);
+  }
+  std::vector<GemmDesc> v;
+  add_gemm(v, m, p, m, 1.0, Zl, m, X, p, 0.0, Y, p, 1);   // Y = L^{-1} X
+  if (p >= 32) run_gemms<1, 2>(s, v); else run_gemms<1, 1>(s, v);
+  v.clear();
+  add_gemm(v, m, p, m, 1.0, Zu, m, Y, p, 0.0, X, p, 2);   // X = U^{-1} Y
+  if (p >= 32) run_gemms<1, 2>(s, v); else run_gemms<1, 1>(s, v);
+  BPF_CHECK_HIP(hipGetLastError());
+  // two triangular m x m products (m^2 p flops each); compulsory traffic:
+  // both inverse factors' triangles once, X read and written twice
+  const double mp = (double)m * p;
+  prof::end(tok, s, prof::DENSE_LU, bytes + (double)m * m * sizeof(double) + 4.0 * mp * sizeof(double),
+            flops + 2.0 * m * mp);
+}
 
 void dense_tail_move(hipStream_t s, int cnt, const int *pos, const int *didx, double *fac, double *S,
                      bool to_dense) {
@@ -413,11 +598,15 @@ void dense_lu_factor(hipStream_t s, int m, double *S, double *Linv, double *Uinv
     hipLaunchKernelGGL(dlu_diag_kernel, dim3(1), dim3(1024), 0, s, S11, m, w, Li, Ui, status, base + j0);
     if (rest > 0) {
       double *S21 = S + (size_t)(j0 + w) * m + j0, *S12 = S11 + w, *S22 = S21 + w;
-      gemm(s, false, rest, w, w, 1.0, S21, m, Ui, nb, P, nb);    // P = S21 U11^{-1}
-      gemm(s, false, w, rest, w, 1.0, Li, nb, S12, m, Q, m);     // Q = L11^{-1} S12
+      std::vector<GemmDesc> v;
+      add_gemm(v, rest, w, w, 1.0, S21, m, Ui, nb, 0.0, P, nb, 0);    // P = S21 U11^{-1}
+      add_gemm(v, w, rest, w, 1.0, Li, nb, S12, m, 0.0, Q, m, 0);     // Q = L11^{-1} S12
+      run_gemms<2, 2>(s, v);
       copy_block(s, rest, w, P, nb, S21, m);
       copy_block(s, w, rest, Q, m, S12, m);
-      gemm(s, true, rest, rest, w, -1.0, P, nb, Q, m, S22, m);   // S22 -= P Q
+      v.clear();
+      add_gemm(v, rest, rest, w, -1.0, P, nb, Q, m, 1.0, S22, m, 0);  // S22 -= P Q
+      run_gemms<2, 2>(s, v);
     }
   }
   BPF_CHECK_HIP(hipGetLastError());
@@ -431,19 +620,30 @@ void dense_lu_solve(hipStream_t s, int m, int p, const double *S, const double *
   const int nb = kDluNb;
   const int nblk = (m + nb - 1) / nb;
   const int tok = prof::begin(s);
+  std::vector<GemmDesc> v;
+  auto run = [&]() {
+    if (p >= 32) run_gemms<1, 2>(s, v); else run_gemms<1, 1>(s, v);
+    v.clear();
+  };
   for (int b = 0; b < nblk; ++b) {
     const int j0 = b * nb, w = std::min(nb, m - j0), rest = m - j0 - w;
-    gemm(s, false, w, p, w, 1.0, Linv + (size_t)b * nb * nb, nb, X + (size_t)j0 * p, p, Y + (size_t)j0 * p, p);
-    if (rest > 0)
-      gemm(s, true, rest, p, w, -1.0, S + (size_t)(j0 + w) * m + j0, m, Y + (size_t)j0 * p, p,
-           X + (size_t)(j0 + w) * p, p);
+    add_gemm(v, w, p, w, 1.0, Linv + (size_t)b * nb * nb, nb, X + (size_t)j0 * p, p, 0.0, Y + (size_t)j0 * p, p, 0);
+    run();
+    if (rest > 0) {
+      add_gemm(v, rest, p, w, -1.0, S + (size_t)(j0 + w) * m + j0, m, Y + (size_t)j0 * p, p, 1.0,
+               X + (size_t)(j0 + w) * p, p, 0);
+      run();
+    }
   }
   // Backward: X_b = U_bb^{-1} Y_b, Y_{<b} -= U_{<b,b} X_b.
   for (int b = nblk - 1; b >= 0; --b) {
     const int j0 = b * nb, w = std::min(nb, m - j0);
-    gemm(s, false, w, p, w, 1.0, Uinv + (size_t)b * nb * nb, nb, Y + (size_t)j0 * p, p, X + (size_t)j0 * p, p);
-    if (j0 > 0)
-      gemm(s, true, j0, p, w, -1.0, S + j0, m, X + (size_t)j0 * p, p, Y, p);
+    add_gemm(v, w, p, w, 1.0, Uinv + (size_t)b * nb * nb, nb, Y + (size_t)j0 * p, p, 0.0, X + (size_t)j0 * p, p, 0);
+    run();
+    if (j0 > 0) {
+      add_gemm(v, j0, p, w, -1.0, S + j0, m, X + (size_t)j0 * p, p, 1.0, Y, p, 0);
+      run();
+    }
   }
   BPF_CHECK_HIP(hipGetLastError());
   // two sweeps of m x m triangles plus the full diagonal-block inverses:

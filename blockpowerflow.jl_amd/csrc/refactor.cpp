@@ -55,6 +55,8 @@ struct bpf_rf_s {
   DevBuf S, Linv, Uinv, P, Q;          // dense tail factor, diagonal-block inverses, scratch
   DevBuf tpos, tdidx;                  // tail entries: factor position, dense index
   DevBuf Xd, Yd;                       // FP64 m x p solve blocks
+  DevBuf Zl, Zu, Zt;                   // explicit L^{-1}, U^{-1} of the tail (m x m) and their scratch
+  bool zinv = false;                   // Zl, Zu hold the inverses of the current factor values
   bool dense() const { return m > 0; }
   TriOp &lop() { return dense() ? lo_h : lo; }
   TriOp &uop() { return dense() ? up_h : up; }
@@ -89,6 +91,7 @@ void refactor_into(bpf_rf rf, const double *d_aval, double *fac) {
     double *S = rf->S.as<double>();
     fill_double(s, S, 0.0, (size_t)rf->m * rf->m);
     dense_tail_move(s, rf->ntail, rf->tpos.as<int>(), rf->tdidx.as<int>(), fac, S, true);
+    rf->zinv = false;   // rebuilt by the first solve that uses them
     dense_lu_factor(s, rf->m, S, rf->Linv.as<double>(), rf->Uinv.as<double>(), rf->P.as<double>(),
                     rf->Q.as<double>(), rf->status.as<int>(), rf->t);
     dense_tail_move(s, rf->ntail, rf->tpos.as<int>(), rf->tdidx.as<int>(), fac, S, false);
@@ -122,6 +125,12 @@ void check_ops(bpf_rf rf) {
 }
 
 constexpr int kDenseMinN = 16384;
+
+}  // namespace
+
+std::atomic<int> bpf::g_dense_inverse{1};
+
+namespace {
 
 void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool allow_dense) {
   rf->ctx = ctx;
@@ -247,8 +256,26 @@ void tri_solve(bpf_rf rf, int p, TV *x) {
   rf->Yd.ensure((size_t)m * p * sizeof(double));
   TV *xt = x + (size_t)t * p;
   copy_cast<TV, double>(s, m, p, xt, p, rf->Xd.as<double>(), p, 1.0);
-  dense_lu_solve(s, m, p, rf->S.as<double>(), rf->Linv.as<double>(), rf->Uinv.as<double>(), rf->Xd.as<double>(),
-                 rf->Yd.as<double>());
+  // FP32 solves (refined afterwards) by the explicit inverse factors, FP64
+  // solves too with the option at 2; else blocked substitution.  The
+  // inverses are built by the first such solve after a refactorization.
+  const bool inv = g_dense_inverse == 2 || (g_dense_inverse == 1 && std::is_same<TV, float>::value);
+  if (inv) {
+    const size_t mm = (size_t)m * m;
+    if (!rf->Zl.ptr) {
+      rf->Zl.ensure(mm * sizeof(double));
+      rf->Zu.ensure(mm * sizeof(double));
+      rf->Zt.ensure(mm * sizeof(double));
+      fill_double(s, rf->Zl.as<double>(), 0.0, mm);   // strictly upper blocks stay zero
+      fill_double(s, rf->Zu.as<double>(), 0.0, mm);   // strictly lower blocks stay zero
+    }
+    dense_lu_solve_inv(s, m, p, rf->S.as<double>(), rf->Linv.as<double>(), rf->Uinv.as<double>(), !rf->zinv,
+                       rf->Zl.as<double>(), rf->Zu.as<double>(), rf->Zt.as<double>(), rf->Xd.as<double>(),
+                       rf->Yd.as<double>());
+    rf->zinv = true;
+  } else
+    dense_lu_solve(s, m, p, rf->S.as<double>(), rf->Linv.as<double>(), rf->Uinv.as<double>(), rf->Xd.as<double>(),
+                   rf->Yd.as<double>());
   copy_cast<double, TV>(s, m, p, rf->Xd.as<double>(), p, xt, p, 1.0);
   trisolve<TV>(s, rf->u12, p, x, p, x, p);
   trisolve<TV>(s, rf->up_h, p, x, p, x, p);

@@ -299,8 +299,10 @@ __global__ __launch_bounds__(kBlock) void tri_syncfree_kernel(TriDev a, int n, c
   sf_publish(flags, a.order[t], epoch, lane);
 }
 
-// One single-workgroup launch walks many narrow levels [l0, l1), with a
-// workgroup barrier between levels.
+// One launch walks many narrow levels [l0, l1), with a workgroup barrier
+// between levels.  Its workgroups split the columns into slabs of cw: each
+// solves its own slab through every level (the columns of a triangular solve
+// are independent, so no workgroup waits on another).
 constexpr int kTailBlock = 1024;
 // With one row per wavefront (G = 64) and a single column chunk (p <= 64
 // CPL), a level with fewer rows than wavefronts splits each row's non-zeros
@@ -310,7 +312,13 @@ constexpr int kTailBlock = 1024;
 template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, int l1,
                                                               const TV *x, int ldx, TV *y,
-                                                              int ldy, int p, int glog) {
+                                                              int ldy, int p, int glog, int cw) {
+  {
+    const int c0 = blockIdx.x * cw;
+    x += c0;
+    y += c0;
+    p = min(cw, p - c0);
+  }
   constexpr int NW = kTailBlock / 64;
   __shared__ TV part[NW][64 * CPL];
   const int G = 1 << glog;
@@ -826,6 +834,7 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
 }
 
 std::atomic<bool> g_syncfree{false};
+std::atomic<int> g_tail_slab{64};
 
 void triop_check(TriOp &op, hipStream_t s) {
   // only a sync-free launch since the last check can have set the word
@@ -912,21 +921,31 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
       }
     } else {
       const prof::ExtPair ep = prof::ext_begin();
+      // More columns than a slab: one workgroup per slab of 64 columns (one
+      // column per lane, one row per wavefront, the same per-column
+      // arithmetic as one workgroup over all columns).
+      const int slab = g_tail_slab;
+      const bool split = slab > 0 && p > slab;
+      const int nslab = split ? (p + slab - 1) / slab : 1;
+      const int tglog = split ? 6 : glog;
+      const int cw = split ? slab : p;
 #define BPF_TAIL1(U, C, VV)                                                                                 \
   do {                                                                                                       \
     if (ep.a)                                                                                                \
-      hipExtLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(1), dim3(kTailBlock), 0, s, ep.a, ep.b, \
-                            0, a, seg.l0, seg.l1, x, ldx, y, ldy, p, glog);                                  \
+      hipExtLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(nslab), dim3(kTailBlock), 0, s, ep.a,   \
+                            ep.b, 0, a, seg.l0, seg.l1, x, ldx, y, ldy, p, tglog, cw);                       \
     else                                                                                                     \
-      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(1), dim3(kTailBlock), 0, s, a, seg.l0,     \
-                         seg.l1, x, ldx, y, ldy, p, glog);                                                   \
+      hipLaunchKernelGGL((tri_tail_kernel<TV, TV, U, C, VV>), dim3(nslab), dim3(kTailBlock), 0, s, a,         \
+                         seg.l0, seg.l1, x, ldx, y, ldy, p, tglog, cw);                                      \
   } while (0)
 #define BPF_TAIL(U, C)                                                                                      \
   do {                                                                                                       \
     if (vec) BPF_TAIL1(U, C, true);                                                                          \
     else BPF_TAIL1(U, C, false);                                                                             \
   } while (0)
-      if (op.unit) {
+      if (split) {
+        if (op.unit) BPF_TAIL1(true, 1, false); else BPF_TAIL1(false, 1, false);
+      } else if (op.unit) {
         if (cpl == 1) BPF_TAIL(true, 1); else if (cpl == 2) BPF_TAIL(true, 2); else BPF_TAIL(true, 4);
       } else {
         if (cpl == 1) BPF_TAIL(false, 1); else if (cpl == 2) BPF_TAIL(false, 2); else BPF_TAIL(false, 4);

@@ -96,9 +96,23 @@ const char *bpf_version(void);
  * handles): A's relabelled copy for the products and relabelled views of the
  * preconditioner's triangular operators (the same factor, levels and
  * arithmetic per row), the blocks permuted at the layout conversion, so that
- * gathered rows lie near the rows gathering them. */
+ * gathered rows lie near the rows gathering them.
+ * BPF_OPT_TAIL_SLAB: column width (0 or 64; default 64) of the slabs that
+ * the narrow-level part of a level-scheduled triangular solve over more than
+ * that many right-hand sides is split into, one workgroup per slab walking
+ * the narrow levels on its own (columns are independent); 0 = one workgroup
+ * for all columns.  Results are identical.
+ * BPF_OPT_DENSE_INVERSE: the dense tail of a refactorization handle (the last
+ * rows of a large factor, solved as a dense block) is solved by explicit
+ * inverses of its two triangular factors, built by recursive doubling on
+ * matrix cores at each refactorization (two block-triangular products per
+ * solve instead of a chain of diagonal-block steps; built by the first solve
+ * that uses them after a refactorization): 0 = never (blocked substitution),
+ * 1 (default) = in the FP32 solves that refinement follows, 2 = in every
+ * solve. */
 enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4,
-       BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7, BPF_OPT_RELABEL_MIN = 8 };
+       BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7, BPF_OPT_RELABEL_MIN = 8,
+       BPF_OPT_TAIL_SLAB = 9, BPF_OPT_DENSE_INVERSE = 10 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 

@@ -230,3 +230,72 @@ def test_kernel_timers_cover_factor_and_dense_tail(bpf, gen):
     n_dlu, ms_dlu, by_dlu, fl_dlu = fam[5]
     assert n_dlu == 1 + 3 and ms_dlu > 0 and by_dlu > 0 and fl_dlu > 0   # factor + three solves
     assert fam[1][0] > 0   # level-scheduled triangular solves
+
+
+@pytest.mark.parametrize("p", [100, 256])
+def test_tail_slabs_are_exact(bpf, gen, oracle, p):
+    # The narrow-level part of a triangular solve over more than 64
+    # right-hand sides runs as one workgroup per column slab: every column's
+    # arithmetic is the same as with one workgroup for all of them, so FP64
+    # and FP32 (refined) LU solves and ILU(0) applications are bitwise equal
+    # whatever the slab width, and the FP64 solve is the direct solution.
+    grid, S = gen.make_system(10000, p, seed=52)
+    assert S.n >= 16384
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, nrhs=p)
+    Pi = bpf.ilu0(A, p)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    old = bpf.get_option("tail_slab")
+    out = {}
+    try:
+        for slab in (0, 64):
+            bpf.set_option("tail_slab", slab)
+            X = bpf.HipMatrix(S.B.shape)
+            bpf.ldiv_(X, rf, B)
+            Xr, berr = rf.solve_refined(A, B, refine=1)
+            Y = bpf.HipMatrix(S.B.shape)
+            bpf.ldiv_(Y, Pi, B)
+            out[slab] = (X.to_numpy(), Xr.to_numpy(), berr, Y.to_numpy())
+    finally:
+        bpf.set_option("tail_slab", old)
+    for a, b in zip(out[0], out[64]):
+        np.testing.assert_array_equal(a, b)
+    Xd = oracle.direct_solve(S.indptr, S.indices, S.data, S.B)
+    np.testing.assert_allclose(out[64][0], Xd, rtol=1e-8, atol=1e-10 * np.abs(Xd).max())
+
+
+def test_dense_tail_inverse_factors(bpf, gen, oracle):
+    # The dense tail solved by explicit inverses of its triangular factors
+    # (recursive doubling on matrix cores, by the first solve that uses them
+    # after a refactorization): with the option at 2 every solve uses them and the FP64 solve is the direct
+    # solution at a backward error of 1e-10 or better; at 1 only the FP32
+    # solves do (the FP64 solve is bitwise the blocked substitution's); the
+    # refined solutions meet the tolerance in every mode.
+    grid, S = gen.make_system(10000, 48, seed=53)
+    assert S.n >= 16384
+    A = bpf.HipSparseMatrixCSR(S.indptr, S.indices, S.data)
+    rf = bpf.CUSOLVERRF.CusolverRfLU(A, nrhs=48)
+    B = bpf.HipMatrix.from_numpy(S.B)
+    vals = gen.newton_value_sets(grid, 1, seed=54)[0]
+    A.set_values(vals)
+    Xd = oracle.direct_solve(S.indptr, S.indices, vals, S.B)
+    old = bpf.get_option("dense_inverse")
+    out = {}
+    try:
+        for mode in (0, 1, 2):
+            bpf.set_option("dense_inverse", mode)
+            bpf.CUSOLVERRF.update_(rf, A)
+            X = bpf.HipMatrix(S.B.shape)
+            bpf.ldiv_(X, rf, B)
+            Xr, berr = rf.solve_refined(A, B, refine=2, berr_tol=1e-10)
+            out[mode] = (X.to_numpy(), Xr.to_numpy(), berr)
+    finally:
+        bpf.set_option("dense_inverse", old)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for mode in (0, 1, 2):
+        X, Xr, berr = out[mode]
+        assert berr.max() <= 1e-10, (mode, berr.max())
+        np.testing.assert_allclose(Xr, Xd, rtol=1e-7, atol=1e-9 * np.abs(Xd).max())
+        np.testing.assert_allclose(X, Xd, rtol=1e-8, atol=1e-10 * np.abs(Xd).max())
+        be = oracle.backward_error(S.indptr, S.indices, vals, X, S.B)
+        assert be.max() <= 1e-10, (mode, be.max())
