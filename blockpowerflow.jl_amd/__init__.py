@@ -83,7 +83,7 @@ def default_context() -> Context:
 
 _OPTIONS = {"syncfree": 1, "autotune": 2, "overlap": 3, "batch_threads": 4, "wide_min": 5, "graph": 6,
             "graph_replays": 7, "relabel_min": 8, "tail_slab": 9,
-            "dense_inverse": 10}
+            "dense_inverse": 10, "dense_blocks": 11, "tail_rounds": 12}
 
 
 def set_option(name: str, value) -> None:
@@ -114,7 +114,11 @@ def set_option(name: str, value) -> None:
     refactorization handle by explicit inverses of its triangular factors,
     built by the first solve that uses them after each refactorization: 0 =
     never (blocked substitution), 1 = in the FP32 solves that refinement
-    follows, 2 = in every solve."""
+    follows, 2 = in every solve; "dense_blocks" (default 12, 0 = none) is the
+    most 128-row blocks of the dense tail of a refactorization handle created
+    afterwards (at most about n / 16 rows); "tail_rounds" (default 2) admits a
+    triangular-solve level to the single narrow-level launch when its rows fit
+    in that many rounds of the launch's wavefronts (results are identical)."""
     check(L.bpf_set_option(_OPTIONS[name], int(value)))
 
 

@@ -158,6 +158,14 @@ int bpf_set_option(int option, long long value) {
       BPF_REQUIRE(value >= 0 && value <= 2, "dense_inverse must be 0, 1 or 2");
       g_dense_inverse = (int)value;
     }
+    else if (option == BPF_OPT_DENSE_BLOCKS) {
+      BPF_REQUIRE(value >= 0 && value <= 256, "dense_blocks must be in 0..256");
+      g_dense_blocks = (int)value;
+    }
+    else if (option == BPF_OPT_TAIL_ROUNDS) {
+      BPF_REQUIRE(value >= 0 && value <= 64, "tail_rounds must be in 0..64");
+      g_tail_rounds = (int)value;
+    }
     else if (option == BPF_OPT_TAIL_SLAB) {
       BPF_REQUIRE(value == 0 || value == 64, "tail_slab must be 0 or 64");
       g_tail_slab = (int)value;
@@ -181,6 +189,8 @@ int bpf_get_option(int option, long long *value) {
     else if (option == BPF_OPT_RELABEL_MIN) *value = g_relabel_min;
     else if (option == BPF_OPT_TAIL_SLAB) *value = g_tail_slab;
     else if (option == BPF_OPT_DENSE_INVERSE) *value = g_dense_inverse;
+    else if (option == BPF_OPT_DENSE_BLOCKS) *value = g_dense_blocks;
+    else if (option == BPF_OPT_TAIL_ROUNDS) *value = g_tail_rounds;
     else if (option == BPF_OPT_GRAPH_REPLAYS) *value = g_graph_replays;
     else BPF_THROW(BPF_E_INVALID, "unknown option %d", option);
   });

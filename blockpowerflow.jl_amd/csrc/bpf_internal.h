@@ -178,6 +178,11 @@ extern std::atomic<int> g_tail_slab;
 // Process-wide: solves of a refactorization handle's dense tail by explicit
 // inverse factors (0 never, 1 FP32 solves, 2 all solves).
 extern std::atomic<int> g_dense_inverse;
+// Process-wide: most 128-row blocks of a refactorization handle's dense tail.
+extern std::atomic<int> g_dense_blocks;
+// Process-wide: a triangular-solve level joins the narrow-level launch when
+// its rows fit in this many rounds of that launch's wavefronts.
+extern std::atomic<int> g_tail_rounds;
 // Process-wide number of host threads a batched solve runs its bins on.
 extern std::atomic<int> g_batch_threads;
 // Process-wide: block GMRES (FP64) with at least this many right-hand sides

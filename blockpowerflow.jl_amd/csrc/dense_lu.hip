@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 namespace bpf {
@@ -39,13 +40,147 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 acc) {
 
 constexpr int kLd = kDluNb + 1;
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane L of each row of 16 lanes, broadcast to that row (DPP row_newbcast).
+template <int L>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + L, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + L, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double pick4(const double (&x)[4], int q) {
+  return q == 0 ? x[0] : q == 1 ? x[1] : q == 2 ? x[2] : x[3];
+}
+
+// Register layout of a 16 x 16 tile in one wavefront: lane l holds column
+// c = l % 16, rows 4 (l / 16) + q in x[q].  The steps below are templates on
+// the pivot index so that every lane index and register index is a constant.
+struct TileLane {
+  int c, g, lane, pw;
+};
+
+// Elimination step KK: pivot by one lane read, its column spread along the
+// rows of 16 lanes by DPP, its row to the other lane groups by a permute;
+// multipliers are products with the pivot's reciprocal.
+template <int KK>
+__device__ __forceinline__ void lu_step(double (&x)[4], const TileLane &t, int *status, int zbase) {
+  if (KK >= t.pw) return;
+  constexpr int SRC = (KK >> 2) * 16 + KK;   // lane holding G[KK][KK] in x[KK & 3]
+  double d = readlane_d(x[KK & 3], SRC);
+  if (d == 0.0) {
+    if (t.lane == 0) atomicMin(status, zbase + KK);
+    d = 1.0;
+    if (t.lane == SRC) x[KK & 3] = 1.0;
+  }
+  const double dinv = 1.0 / d;
+  const double u = __shfl(x[KK & 3], (KK >> 2) * 16 + t.c);   // G[KK][c]
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = 4 * t.g + q;
+    const double m = row_bcast<KK>(x[q]) * dinv;   // G[r][KK] / d
+    if (r > KK && r < t.pw) {
+      if (t.c == KK) x[q] = m;
+      else if (t.c > KK && t.c < t.pw) x[q] = fma(-m, u, x[q]);
+    }
+  }
+}
+
+// Step I of the two substitutions on the identity: U X = I at row 15 - I
+// (rdiag: 1 / U[c][c] on the lanes of group c / 4) and L X = I at row I.
+template <int I>
+__device__ __forceinline__ void inv_step(const double (&x)[4], double (&bu)[4], double (&bl)[4], double rdiag,
+                                         const TileLane &t) {
+  constexpr int KU = 15 - I, KL = I;
+  if (KU < t.pw) {
+    const double rk = readlane_d(rdiag, (KU >> 2) * 16 + KU);
+    if (t.g == (KU >> 2)) bu[KU & 3] *= rk;
+    const double xk = __shfl(bu[KU & 3], (KU >> 2) * 16 + t.c);   // X[KU][c]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double urk = row_bcast<KU>(x[q]);   // U[r][KU]
+      if (4 * t.g + q < KU) bu[q] = fma(-urk, xk, bu[q]);
+    }
+  }
+  if (KL < t.pw) {
+    const double xk = __shfl(bl[KL & 3], (KL >> 2) * 16 + t.c);   // X[KL][c]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 4 * t.g + q;
+      const double lrk = row_bcast<KL>(x[q]);   // L[r][KL]
+      if (r > KL && r < t.pw) bl[q] = fma(-lrk, xk, bl[q]);
+    }
+  }
+}
+
+template <int... K>
+__device__ __forceinline__ void lu_steps(std::integer_sequence<int, K...>, double (&x)[4], const TileLane &t,
+                                         int *status, int zbase) {
+  (lu_step<K>(x, t, status, zbase), ...);
+}
+template <int... K>
+__device__ __forceinline__ void inv_steps(std::integer_sequence<int, K...>, const double (&x)[4], double (&bu)[4],
+                                          double (&bl)[4], double rdiag, const TileLane &t) {
+  (inv_step<K>(x, bu, bl, rdiag, t), ...);
+}
+
+// One wavefront: LU without pivoting of the pw x pw diagonal tile at
+// G[p0][p0] (pw <= 16) and the inverses of both its triangles.  Lane l holds
+// column c = l % 16, rows 4 (l / 16) + q in x[q] (the identity outside the
+// pw x pw block).  Each pivot is read by one lane read, its column spread
+// along the rows of 16 lanes by DPP, its row to the other lane groups by a
+// permute; multipliers are products with the pivot's reciprocal.  U_PP^{-1}
+// and L_PP^{-1} then follow by column-oriented substitution on the identity
+// (interleaved), with the reciprocals of U's diagonal formed once.  The
+// tile goes back to G, the inverses to sU / sL and to the diagonal blocks of
+// Uinv / Linv.  A zero pivot at row r sets *status = min(*status, base + p0 + r).
+__device__ __forceinline__ void tile_lu(double *G, int p0, int pw, double (*sU)[17], double (*sL)[17],
+                                        double *Linv, double *Uinv, int *status, int base, int lane) {
+  const TileLane t{lane & 15, lane >> 4, lane, pw};
+  const int c = t.c, g = t.g;
+  double x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = 4 * g + q;
+    x[q] = (r < pw && c < pw) ? G[(p0 + r) * kLd + p0 + c] : (r == c ? 1.0 : 0.0);
+  }
+  lu_steps(std::make_integer_sequence<int, 16>{}, x, t, status, base + p0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = 4 * g + q;
+    if (r < pw && c < pw) G[(p0 + r) * kLd + p0 + c] = x[q];
+  }
+  // 1 / U[c][c], held by the lanes of group c / 4 (register c % 4)
+  const double rdiag = g == (c >> 2) ? 1.0 / pick4(x, c & 3) : 1.0;
+  double bu[4], bl[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bu[q] = bl[q] = (4 * g + q == c) ? 1.0 : 0.0;
+  inv_steps(std::make_integer_sequence<int, 16>{}, x, bu, bl, rdiag, t);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = 4 * g + q;
+    sU[r][c] = bu[q];
+    sL[r][c] = bl[q];
+    if (r < pw && c < pw) {
+      Uinv[(size_t)(p0 + r) * kDluNb + p0 + c] = bu[q];
+      Linv[(size_t)(p0 + r) * kDluNb + p0 + c] = bl[q];
+    }
+  }
+}
+
 // LU without pivoting of the w x w block at S (ld lds), w <= kDluNb, in LDS:
 // 16-column sub-panels.  One wavefront factors the sub-panel's 16 x 16
-// diagonal block and inverts both of its triangles (lanes 0-15 the columns
-// of U_PP^{-1}, lanes 16-31 those of L_PP^{-1}); the L column block
-// (G_rP U_PP^{-1}), the U row block (L_PP^{-1} G_Pc) and the trailing update
-// then run on matrix cores.  The 16 x 16 inverses are also the diagonal
-// blocks of U^{-1} and L^{-1}, which are completed blockwise at the end.
+// diagonal block and inverts both of its triangles (tile_lu); the L column
+// block (G_rP U_PP^{-1}), the U row block (L_PP^{-1} G_Pc) and the trailing
+// update then run on matrix cores, the next diagonal tile updated and
+// factored by wave 0 while the others update the rest.  The 16 x 16 inverses
+// are also the diagonal blocks of U^{-1} and L^{-1}, which are completed
+// blockwise at the end.
 // A zero pivot at block row r sets *status = min(*status, base + r).
 __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int w, double *Linv, double *Uinv,
                                                         int *status, int base) {
@@ -58,134 +193,54 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
     G[e] = (r < w && c < w) ? S[(size_t)r * lds + c] : 0.0;
   }
   __syncthreads();
+  // Sub-panel 0's diagonal tile, then per sub-panel: its L column and U row
+  // blocks (all wavefronts), and the trailing update with look-ahead: wave 0
+  // updates the next diagonal tile first and factors it while the other
+  // wavefronts update the rest of the trailing block.
+  if (wave == 0) tile_lu(G, 0, min(16, w), sU, sL, Linv, Uinv, status, base, lane);
+  __syncthreads();
   for (int p0 = 0; p0 < w; p0 += 16) {
     const int pw = min(16, w - p0);
-    if (wave == 0) {
-      // The 16 x 16 diagonal block in registers: lane l holds column
-      // c = l % 16, rows 4 (l / 16) + q in x[q]; pivot rows, multipliers and
-      // solved rows move between lanes by shuffles.  LU without pivoting
-      // (the same division and fused update per entry as an elimination in
-      // place), then U_PP^{-1} and L_PP^{-1} by column-oriented substitution
-      // on the identity; outside the pw x pw block both are the identity.
-      const int c = lane & 15, g = lane >> 4;
-      double x[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = 4 * g + q;
-        x[q] = (r < pw && c < pw) ? G[(p0 + r) * kLd + p0 + c] : (r == c ? 1.0 : 0.0);
-      }
-#pragma unroll
-      for (int kk = 0; kk < 16; ++kk) {
-        if (kk >= pw) break;
-        const int rowl = (kk >> 2) * 16;   // lanes holding row kk (register kk & 3)
-        double d = __shfl(x[kk & 3], rowl + kk);
-        if (d == 0.0) {
-          if (lane == 0) atomicMin(status, base + p0 + kk);
-          d = 1.0;
-          if (lane == rowl + kk) x[kk & 3] = 1.0;
-        }
-        const double u = __shfl(x[kk & 3], rowl + c);   // G[kk][c]
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = 4 * g + q;
-          const double m = __shfl(x[q], g * 16 + kk) / d;   // G[r][kk] / d
-          if (r > kk && r < pw) {
-            if (c == kk) x[q] = m;
-            else if (c > kk && c < pw) x[q] = fma(-m, u, x[q]);
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = 4 * g + q;
-        if (r < pw && c < pw) G[(p0 + r) * kLd + p0 + c] = x[q];
-      }
-      double bu[4], bl[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) bu[q] = bl[q] = (4 * g + q == c) ? 1.0 : 0.0;
-      // U X = I, rows kk = pw-1 .. 0
-#pragma unroll
-      for (int kk = 15; kk >= 0; --kk) {
-        if (kk >= pw) continue;
-        const int rowl = (kk >> 2) * 16;
-        const double ukk = __shfl(x[kk & 3], rowl + kk);
-        if (g == (kk >> 2)) bu[kk & 3] = bu[kk & 3] / ukk;
-        const double xk = __shfl(bu[kk & 3], rowl + c);   // X[kk][c]
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double urk = __shfl(x[q], g * 16 + kk);   // U[r][kk]
-          if (4 * g + q < kk) bu[q] = fma(-urk, xk, bu[q]);
-        }
-      }
-      // L X = I (unit diagonal), rows kk = 0 .. pw-1
-#pragma unroll
-      for (int kk = 0; kk < 16; ++kk) {
-        if (kk >= pw) break;
-        const int rowl = (kk >> 2) * 16;
-        const double xk = __shfl(bl[kk & 3], rowl + c);   // X[kk][c]
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = 4 * g + q;
-          const double lrk = __shfl(x[q], g * 16 + kk);   // L[r][kk]
-          if (r > kk && r < pw) bl[q] = fma(-lrk, xk, bl[q]);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        sU[4 * g + q][c] = bu[q];
-        sL[4 * g + q][c] = bl[q];
-      }
-    }
-    __syncthreads();
+    const int t0 = p0 + 16;
+    const int nt = (w - t0 + 15) / 16;
+    if (nt <= 0) break;
     // L column block G[r][P] <- G[r][P] U_PP^{-1} and U row block
     // G[P][c] <- L_PP^{-1} G[P][c], one 16 x 16 tile per wavefront turn; each
     // wavefront reads its whole tile before overwriting it.
-    {
-      const int t0 = p0 + 16;
-      const int nt = (w - t0 + 15) / 16;
-      for (int tt = wave; tt < 2 * nt; tt += 16) {
-        const int t = tt >> 1;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-        if ((tt & 1) == 0) {
-          const int rb = t0 + t * 16;
+    for (int tt = wave; tt < 2 * nt; tt += 16) {
+      const int t = tt >> 1;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      if ((tt & 1) == 0) {
+        const int rb = t0 + t * 16;
 #pragma unroll
-          for (int k0 = 0; k0 < 16; k0 += 4) {
-            const int k = k0 + lr;
-            acc = mfma4(rb + lc < w ? G[(rb + lc) * kLd + p0 + k] : 0.0, sU[k][lc], acc);
-          }
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const int k = k0 + lr;
+          acc = mfma4(rb + lc < w ? G[(rb + lc) * kLd + p0 + k] : 0.0, sU[k][lc], acc);
+        }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = rb + lr + 4 * q;
-            if (r < w && lc < pw) G[r * kLd + p0 + lc] = acc[q];
-          }
-        } else {
-          const int cb = t0 + t * 16;
+        for (int q = 0; q < 4; ++q) {
+          const int r = rb + lr + 4 * q;
+          if (r < w && lc < pw) G[r * kLd + p0 + lc] = acc[q];
+        }
+      } else {
+        const int cb = t0 + t * 16;
 #pragma unroll
-          for (int k0 = 0; k0 < 16; k0 += 4) {
-            const int k = k0 + lr;
-            acc = mfma4(sL[lc][k], cb + lc < w ? G[(p0 + k) * kLd + cb + lc] : 0.0, acc);
-          }
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const int k = k0 + lr;
+          acc = mfma4(sL[lc][k], cb + lc < w ? G[(p0 + k) * kLd + cb + lc] : 0.0, acc);
+        }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = lr + 4 * q, c = cb + lc;
-            if (r < pw && c < w) G[(p0 + r) * kLd + c] = acc[q];
-          }
+        for (int q = 0; q < 4; ++q) {
+          const int r = lr + 4 * q, c = cb + lc;
+          if (r < pw && c < w) G[(p0 + r) * kLd + c] = acc[q];
         }
       }
     }
-    // The diagonal blocks of U^{-1} and L^{-1} (completed below).
-    for (int e = tid; e < 256; e += 1024) {
-      const int i = e >> 4, j = e & 15;
-      if (i < pw && j < pw) {
-        Uinv[(size_t)(p0 + i) * kDluNb + p0 + j] = sU[i][j];
-        Linv[(size_t)(p0 + i) * kDluNb + p0 + j] = sL[i][j];
-      }
-    }
     __syncthreads();
-    // trailing update G[r][c] -= sum_k L[r][p0+k] U[p0+k][c], r, c >= p0 + 16
-    const int t0 = p0 + 16;
-    const int nt = (w - t0 + 15) / 16;
-    for (int tt = wave; tt < nt * nt; tt += 16) {
+    // trailing update G[r][c] -= sum_k L[r][p0+k] U[p0+k][c], r, c >= t0:
+    // tile 0 (the next diagonal tile) by wave 0, which then factors it; the
+    // other tiles by wavefronts 1-15
+    auto trail = [&](int tt) {
       const int ti = tt / nt, tj = tt % nt;
       dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -198,6 +253,15 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
         const int i = t0 + ti * 16 + lr + 4 * q, j = t0 + tj * 16 + lc;
         if (i < w && j < w) G[i * kLd + j] -= acc[q];
       }
+    };
+    if (wave == 0) {
+      trail(0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      tile_lu(G, t0, min(16, w - t0), sU, sL, Linv, Uinv, status, base, lane);
+    } else {
+      for (int tt = wave; tt < nt * nt; tt += 15) trail(tt);
     }
     __syncthreads();
   }

@@ -129,6 +129,7 @@ constexpr int kDenseMinN = 16384;
 }  // namespace
 
 std::atomic<int> bpf::g_dense_inverse{1};
+std::atomic<int> bpf::g_dense_blocks{12};
 
 namespace {
 
@@ -194,7 +195,7 @@ void setup(bpf_ctx ctx, bpf_csr A, int ordering, double tol, bpf_rf rf, bool all
   // blocks measured slower).
   if (allow_dense && n >= kDenseMinN) {
     const int nb = kDluNb;
-    rf->m = std::min(12 * nb, (n / 16) / nb * nb);
+    rf->m = std::min(g_dense_blocks * nb, (n / 16) / nb * nb);
   }
   if (!rf->dense()) {
     build_triop(rf->lo, n, lu.frow, lu.fcol, 0, s);

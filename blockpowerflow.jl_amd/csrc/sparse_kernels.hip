@@ -835,6 +835,7 @@ void launch_spmm_colmajor(hipStream_t s, int n, const int *rowptr, const int *co
 
 std::atomic<bool> g_syncfree{false};
 std::atomic<int> g_tail_slab{64};
+std::atomic<int> g_tail_rounds{2};
 
 void triop_check(TriOp &op, hipStream_t s) {
   // only a sync-free launch since the last check can have set the word
@@ -883,7 +884,7 @@ void trisolve(hipStream_t s, TriOp &op, int p, const TV *x, int ldx, TV *y, int 
   int glog, cpl;
   lane_shape(p, glog, cpl);
   const int rpb = kBlock >> glog;
-  const auto plan = make_plan(op, 2 * (kTailBlock >> glog));
+  const auto plan = make_plan(op, g_tail_rounds * (kTailBlock >> glog));
   // Vector loads when the lanes of a row cover its columns exactly and the
   // rows are aligned (each lane then owns cpl consecutive columns).
   const bool vec = cpl > 1 && ((1 << glog) * cpl == p) && vec_ok<TV>(x, ldx, cpl) && vec_ok<TV>(y, ldy, cpl);

@@ -109,10 +109,16 @@ const char *bpf_version(void);
  * solve instead of a chain of diagonal-block steps; built by the first solve
  * that uses them after a refactorization): 0 = never (blocked substitution),
  * 1 (default) = in the FP32 solves that refinement follows, 2 = in every
- * solve. */
+ * solve.
+ * BPF_OPT_DENSE_BLOCKS: the dense tail of a refactorization handle created
+ * from now on spans at most this many 128-row blocks (default 12; 0 = no
+ * dense tail), and at most about n / 16 rows.
+ * BPF_OPT_TAIL_ROUNDS: a dependency level of a triangular solve joins the
+ * single narrow-level launch when its rows fit in this many rounds of that
+ * launch's wavefronts (default 2).  Results are identical. */
 enum { BPF_OPT_SYNCFREE = 1, BPF_OPT_AUTOTUNE = 2, BPF_OPT_OVERLAP = 3, BPF_OPT_BATCH_THREADS = 4,
        BPF_OPT_WIDE_MIN = 5, BPF_OPT_GRAPH = 6, BPF_OPT_GRAPH_REPLAYS = 7, BPF_OPT_RELABEL_MIN = 8,
-       BPF_OPT_TAIL_SLAB = 9, BPF_OPT_DENSE_INVERSE = 10 };
+       BPF_OPT_TAIL_SLAB = 9, BPF_OPT_DENSE_INVERSE = 10, BPF_OPT_DENSE_BLOCKS = 11, BPF_OPT_TAIL_ROUNDS = 12 };
 int bpf_set_option(int option, long long value);
 int bpf_get_option(int option, long long *value);
 
