@@ -67,10 +67,13 @@ struct TileLane {
 
 // Elimination step KK: pivot by one lane read, its column spread along the
 // rows of 16 lanes by DPP, its row to the other lane groups by a permute;
-// multipliers are products with the pivot's reciprocal.
+// multipliers are products with the pivot's reciprocal.  Every lane runs the
+// same instructions: rows at or above the pivot get a zero multiplier,
+// columns at or left of it a zero pivot-row entry, and the pivot column
+// takes the multipliers.  Outside the pw x pw block the tile is the
+// identity, which the steps leave unchanged.
 template <int KK>
 __device__ __forceinline__ void lu_step(double (&x)[4], const TileLane &t, int *status, int zbase) {
-  if (KK >= t.pw) return;
   constexpr int SRC = (KK >> 2) * 16 + KK;   // lane holding G[KK][KK] in x[KK & 3]
   double d = readlane_d(x[KK & 3], SRC);
   if (d == 0.0) {
@@ -80,14 +83,13 @@ __device__ __forceinline__ void lu_step(double (&x)[4], const TileLane &t, int *
   }
   const double dinv = 1.0 / d;
   const double u = __shfl(x[KK & 3], (KK >> 2) * 16 + t.c);   // G[KK][c]
+  const double up = t.c > KK ? u : 0.0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int r = 4 * t.g + q;
-    const double m = row_bcast<KK>(x[q]) * dinv;   // G[r][KK] / d
-    if (r > KK && r < t.pw) {
-      if (t.c == KK) x[q] = m;
-      else if (t.c > KK && t.c < t.pw) x[q] = fma(-m, u, x[q]);
-    }
+    const bool below = 4 * t.g + q > KK;
+    const double m = below ? row_bcast<KK>(x[q]) * dinv : 0.0;   // G[r][KK] / d
+    const double xn = fma(-m, up, x[q]);
+    x[q] = (below && t.c == KK) ? m : xn;
   }
 }
 
@@ -97,24 +99,17 @@ template <int I>
 __device__ __forceinline__ void inv_step(const double (&x)[4], double (&bu)[4], double (&bl)[4], double rdiag,
                                          const TileLane &t) {
   constexpr int KU = 15 - I, KL = I;
-  if (KU < t.pw) {
-    const double rk = readlane_d(rdiag, (KU >> 2) * 16 + KU);
-    if (t.g == (KU >> 2)) bu[KU & 3] *= rk;
-    const double xk = __shfl(bu[KU & 3], (KU >> 2) * 16 + t.c);   // X[KU][c]
+  const double rk = readlane_d(rdiag, (KU >> 2) * 16 + KU);
+  if (t.g == (KU >> 2)) bu[KU & 3] *= rk;
+  const double xu = __shfl(bu[KU & 3], (KU >> 2) * 16 + t.c);   // X[KU][c]
+  const double xl = __shfl(bl[KL & 3], (KL >> 2) * 16 + t.c);   // X[KL][c]
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double urk = row_bcast<KU>(x[q]);   // U[r][KU]
-      if (4 * t.g + q < KU) bu[q] = fma(-urk, xk, bu[q]);
-    }
-  }
-  if (KL < t.pw) {
-    const double xk = __shfl(bl[KL & 3], (KL >> 2) * 16 + t.c);   // X[KL][c]
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = 4 * t.g + q;
-      const double lrk = row_bcast<KL>(x[q]);   // L[r][KL]
-      if (r > KL && r < t.pw) bl[q] = fma(-lrk, xk, bl[q]);
-    }
+  for (int q = 0; q < 4; ++q) {
+    const int r = 4 * t.g + q;
+    const double urk = r < KU ? row_bcast<KU>(x[q]) : 0.0;   // U[r][KU]
+    const double lrk = r > KL ? row_bcast<KL>(x[q]) : 0.0;   // L[r][KL]
+    bu[q] = fma(-urk, xu, bu[q]);
+    bl[q] = fma(-lrk, xl, bl[q]);
   }
 }
 
@@ -173,6 +168,94 @@ __device__ __forceinline__ void tile_lu(double *G, int p0, int pw, double (*sU)[
   }
 }
 
+// Diagonal tile B of a kDluNb x kDluNb inverse in the MFMA output layout
+// (lane l: rows 16 B + l / 16 + 4 q, column 16 B + l % 16); the identity
+// outside the w x w block.
+__device__ __forceinline__ dbl4 diag_tile(const double *Inv, int B, int w, int lane) {
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 t;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = 16 * B + lr + 4 * q, c = 16 * B + lc;
+    t[q] = (r < w && c < w) ? Inv[(size_t)r * kDluNb + c] : (r == c ? 1.0 : 0.0);
+  }
+  return t;
+}
+
+// D = A_II B for the diagonal inverse tile A_II (Inv, block I) and B in the
+// MFMA output layout (which is the B-operand layout of step q = 0..3).
+__device__ __forceinline__ dbl4 diag_times(const double *Inv, int I, int w, dbl4 b, int lane) {
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int r = 16 * I + lc, c = 16 * I + 4 * s + lr;
+    const double a = (r < w && c < w) ? Inv[(size_t)r * kDluNb + c] : (r == c ? 1.0 : 0.0);
+    d = mfma4(a, b[s], d);
+  }
+  return d;
+}
+
+// Block column J of X = U^{-1} (U: upper part of G): X_JJ from the diagonal
+// inverse, then for I = J-1 .. 0: X_IJ = -X_II sum_{I<M<=J} U_IM X_MJ.
+// Writes the column's tiles (zeros below the diagonal block) to Uinv.
+template <int J>
+__device__ __forceinline__ void inv_ucol(const double *G, double *Uinv, int w, int nbk, int lane) {
+  if (J >= nbk) return;
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 X[J + 1];
+  X[J] = diag_tile(Uinv, J, w, lane);
+#pragma unroll
+  for (int I = J - 1; I >= 0; --I) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int M = I + 1; M <= J; ++M)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = mfma4(G[(16 * I + lc) * kLd + 16 * M + 4 * s + lr], X[M][s], acc);
+    X[I] = -diag_times(Uinv, I, w, acc, lane);
+  }
+#pragma unroll
+  for (int I = 0; I < 8; ++I) {
+    if (I >= nbk) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 16 * I + lr + 4 * q, c = 16 * J + lc;
+      if (I != J) Uinv[(size_t)r * kDluNb + c] = I < J ? X[I < J ? I : 0][q] : 0.0;
+    }
+  }
+}
+
+// Block column J of Z = L^{-1} (L: strictly lower part of G, unit
+// diagonal): Z_JJ from the diagonal inverse, then for I = J+1 .. nbk-1:
+// Z_IJ = -Z_II sum_{J<=M<I} L_IM Z_MJ.  Writes the column's tiles (zeros
+// above the diagonal block) to Linv.
+template <int J>
+__device__ __forceinline__ void inv_lcol(const double *G, double *Linv, int w, int nbk, int lane) {
+  if (J >= nbk) return;
+  const int lr = lane >> 4, lc = lane & 15;
+  dbl4 Z[8];
+  Z[J] = diag_tile(Linv, J, w, lane);
+#pragma unroll
+  for (int I = J + 1; I < 8; ++I) {
+    if (I >= nbk) break;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int M = J; M < I; ++M)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = mfma4(G[(16 * I + lc) * kLd + 16 * M + 4 * s + lr], Z[M][s], acc);
+    Z[I] = -diag_times(Linv, I, w, acc, lane);
+  }
+#pragma unroll
+  for (int I = 0; I < 8; ++I) {
+    if (I >= nbk) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 16 * I + lr + 4 * q, c = 16 * J + lc;
+      if (I != J) Linv[(size_t)r * kDluNb + c] = I > J ? Z[I][q] : 0.0;
+    }
+  }
+}
+
 // LU without pivoting of the w x w block at S (ld lds), w <= kDluNb, in LDS:
 // 16-column sub-panels.  One wavefront factors the sub-panel's 16 x 16
 // diagonal block and inverts both of its triangles (tile_lu); the L column
@@ -188,9 +271,22 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
   __shared__ double sU[16][17], sL[16][17];   // current sub-panel's U_PP^{-1}, L_PP^{-1}
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane >> 4, lc = lane & 15;
-  for (int e = tid; e < kDluNb * kLd; e += 1024) {
-    const int r = e / kLd, c = e % kLd;
-    G[e] = (r < w && c < w) ? S[(size_t)r * lds + c] : 0.0;
+  {
+    // 128 x 128 as 8,192 pairs of columns, eight per thread, loads first
+    double2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + i * 1024, r = e >> 6, c = (e & 63) * 2;
+      const double *src = S + (size_t)r * lds + c;
+      if (r < w && c + 1 < w) v[i] = *reinterpret_cast<const double2 *>(src);
+      else v[i] = double2{r < w && c < w ? src[0] : 0.0, 0.0};
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + i * 1024, r = e >> 6, c = (e & 63) * 2;
+      G[r * kLd + c] = v[i].x;
+      G[r * kLd + c + 1] = v[i].y;
+    }
   }
   __syncthreads();
   // Sub-panel 0's diagonal tile, then per sub-panel: its L column and U row
@@ -265,120 +361,40 @@ __global__ __launch_bounds__(1024) void dlu_diag_kernel(double *S, int lds, int 
     }
     __syncthreads();
   }
-  for (int e = tid; e < w * w; e += 1024) {
-    const int r = e / w, c = e % w;
-    S[(size_t)r * lds + c] = G[r * kLd + c];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int e = tid + i * 1024, r = e >> 6, c = (e & 63) * 2;
+    double *dst = S + (size_t)r * lds + c;
+    if (r < w && c + 1 < w) *reinterpret_cast<double2 *>(dst) = double2{G[r * kLd + c], G[r * kLd + c + 1]};
+    else if (r < w && c < w) dst[0] = G[r * kLd + c];
   }
-  __syncthreads();
-  // Inverses, blockwise on 16 x 16 blocks (I, J = block indices).
-  //  U^{-1} = X upper: X_II = U_II^{-1} (one thread per column), then
-  //    X_IJ = -X_II sum_{I<M<=J} U_IM X_MJ, block column J by wavefront J;
-  //    X is kept transposed in the strict lower part (L is saved in S) with
-  //    its diagonal in dg[].
-  //  L^{-1} = Z unit lower: L moves transposed into the strict upper part,
-  //    Z_II by substitution, then Z_IJ = -Z_II sum_{J<=M<I} L_IM Z_MJ, block
-  //    column J by wavefront J; Z kept in the strict lower part.
-  __shared__ double dg[kDluNb];
-  __shared__ double scr[kDluNb / 16][16][17];
-  const int nb = (w + 15) / 16;
-  // X_II: the sub-panel inverses stored during the factorization
-  for (int e = tid; e < w * 16; e += 1024) {
-    const int c = e >> 4, r = (c / 16) * 16 + (e & 15);
-    if (r == c) dg[c] = Uinv[(size_t)c * kDluNb + c];
-    else if (r < c) G[c * kLd + r] = Uinv[(size_t)r * kDluNb + c];
-  }
-  __syncthreads();
-  auto xu = [&](int I, int J, int k, int n) -> double {   // X_IJ[k][n], I <= J
-    const int r = I * 16 + k, c = J * 16 + n;
-    if (r >= w || c >= w) return 0.0;
-    return r < c ? G[c * kLd + r] : (r == c ? dg[r] : 0.0);
-  };
-  if (wave < nb) {
-    const int J = wave;
-    double (*sw)[17] = scr[wave];
-    for (int I = J - 1; I >= 0; --I) {
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-      for (int M = I + 1; M <= J; ++M)
-#pragma unroll
-        for (int k0 = 0; k0 < 16; k0 += 4) {
-          const int k = k0 + lr;
-          const int rr = I * 16 + lc, cc = M * 16 + k;
-          acc = mfma4((rr < w && cc < w) ? G[rr * kLd + cc] : 0.0, xu(M, J, k, lc), acc);
-        }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = acc[q];
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      dbl4 x4 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int k0 = 0; k0 < 16; k0 += 4) x4 = mfma4(xu(I, I, lc, k0 + lr), sw[k0 + lr][lc], x4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = I * 16 + lr + 4 * q, c = J * 16 + lc;
-        if (r < w && c < w) G[c * kLd + r] = -x4[q];
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // The inverses from the 16 x 16 diagonal-block inverses (written to Uinv /
+  // Linv by tile_lu): block column J of U^{-1} by wavefront J, of L^{-1} by
+  // wavefront 8 + J, each column's tiles kept in registers (an MFMA output
+  // tile is laid out as the B operand of the next product).
+  const int nbk = (w + 15) / 16;
+  if (wave < 8) {
+    switch (wave) {
+      case 0: inv_ucol<0>(G, Uinv, w, nbk, lane); break;
+      case 1: inv_ucol<1>(G, Uinv, w, nbk, lane); break;
+      case 2: inv_ucol<2>(G, Uinv, w, nbk, lane); break;
+      case 3: inv_ucol<3>(G, Uinv, w, nbk, lane); break;
+      case 4: inv_ucol<4>(G, Uinv, w, nbk, lane); break;
+      case 5: inv_ucol<5>(G, Uinv, w, nbk, lane); break;
+      case 6: inv_ucol<6>(G, Uinv, w, nbk, lane); break;
+      default: inv_ucol<7>(G, Uinv, w, nbk, lane); break;
     }
-  }
-  __syncthreads();
-  for (int e = tid; e < kDluNb * kDluNb; e += 1024) {
-    const int r = e / kDluNb, c = e % kDluNb;
-    double v = 0.0;
-    if (r < w && c < w) v = r < c ? G[c * kLd + r] : (r == c ? dg[r] : 0.0);
-    Uinv[e] = v;
-  }
-  __syncthreads();
-  for (int e = tid; e < w * w; e += 1024) {
-    const int r = e / w, mm = e % w;
-    if (r > mm) G[mm * kLd + r] = S[(size_t)r * lds + mm];
-  }
-  __syncthreads();
-  // Z_II: the sub-panel inverses stored during the factorization
-  for (int e = tid; e < w * 16; e += 1024) {
-    const int c = e >> 4, r = (c / 16) * 16 + (e & 15);
-    if (r > c && r < w) G[r * kLd + c] = Linv[(size_t)r * kDluNb + c];
-  }
-  __syncthreads();
-  auto zl = [&](int I, int J, int k, int n) -> double {   // Z_IJ[k][n], I >= J
-    const int r = I * 16 + k, c = J * 16 + n;
-    if (r >= w || c >= w) return 0.0;
-    return r > c ? G[r * kLd + c] : (r == c ? 1.0 : 0.0);
-  };
-  auto lt = [&](int I, int M, int k, int n) -> double {   // L_IM[k][n], I > M (stored transposed)
-    const int r = I * 16 + k, c = M * 16 + n;
-    return (r < w && c < w && r > c) ? G[c * kLd + r] : 0.0;
-  };
-  if (wave < nb) {
-    const int J = wave;
-    double (*sw)[17] = scr[wave];
-    for (int I = J + 1; I < nb; ++I) {
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-      for (int M = J; M < I; ++M)
-#pragma unroll
-        for (int k0 = 0; k0 < 16; k0 += 4) acc = mfma4(lt(I, M, lc, k0 + lr), zl(M, J, k0 + lr, lc), acc);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sw[lr + 4 * q][lc] = acc[q];
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int k0 = 0; k0 < 16; k0 += 4) z4 = mfma4(zl(I, I, lc, k0 + lr), sw[k0 + lr][lc], z4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = I * 16 + lr + 4 * q, c = J * 16 + lc;
-        if (r < w && c < w) G[r * kLd + c] = -z4[q];
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  } else {
+    switch (wave - 8) {
+      case 0: inv_lcol<0>(G, Linv, w, nbk, lane); break;
+      case 1: inv_lcol<1>(G, Linv, w, nbk, lane); break;
+      case 2: inv_lcol<2>(G, Linv, w, nbk, lane); break;
+      case 3: inv_lcol<3>(G, Linv, w, nbk, lane); break;
+      case 4: inv_lcol<4>(G, Linv, w, nbk, lane); break;
+      case 5: inv_lcol<5>(G, Linv, w, nbk, lane); break;
+      case 6: inv_lcol<6>(G, Linv, w, nbk, lane); break;
+      default: inv_lcol<7>(G, Linv, w, nbk, lane); break;
     }
-  }
-  __syncthreads();
-  for (int e = tid; e < kDluNb * kDluNb; e += 1024) {
-    const int r = e / kDluNb, c = e % kDluNb;
-    double v = 0.0;
-    if (r < w && c < w) v = r > c ? G[r * kLd + c] : (r == c ? 1.0 : 0.0);
-    Linv[e] = v;
   }
 }
 
