@@ -14,6 +14,7 @@ void build_elim_plan(int n, const std::vector<int> &rowptr, const std::vector<in
   elim_plan_host(n, rowptr, colidx, diagpos, pinfo, map);
   upload(out.pinfo, pinfo, s);
   upload(out.map, map, s);
+  out.rdiag.ensure(std::max(1, n) * sizeof(double));
 }
 
 std::vector<Seg> make_plan(const TriOp &op, int tail_rows) {

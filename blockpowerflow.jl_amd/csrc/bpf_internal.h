@@ -159,6 +159,7 @@ struct TriOp : TriLevels {
 // Device copy of an elimination plan (elim_plan_host).
 struct ElimPlan {
   DevBuf pinfo, map;
+  DevBuf rdiag;   // 1 / U[k][k] of every row k eliminated so far (written by the factorization)
   bool ready() const { return pinfo.ptr && map.ptr; }
 };
 // Process-wide switch between the sync-free single-launch triangular

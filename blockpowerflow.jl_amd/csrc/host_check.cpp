@@ -279,7 +279,7 @@ static void check_elim_plan(const Csr &A) {
     for (int e = r0; e < A.diagpos[i]; ++e) {
       const int k = A.colidx[e];
       const PivInfo &pi = pinfo[e];
-      CHECK(pi.diag == A.diagpos[k] && pi.end == A.rowptr[k + 1], "pivot (%d,%d) extent", i, k);
+      CHECK(pi.diag == A.diagpos[k] && pi.end == A.rowptr[k + 1] && pi.row == k, "pivot (%d,%d) extent", i, k);
       for (int j = A.diagpos[k] + 1; j < A.rowptr[k + 1]; ++j) {
         int want = -1;
         for (int f = r0; f < r1; ++f)

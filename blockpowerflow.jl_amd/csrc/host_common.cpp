@@ -39,7 +39,7 @@ void elim_plan_host(int n, const std::vector<int> &rowptr, const std::vector<int
     const int r0 = rowptr[i], r1 = rowptr[i + 1];
     for (int e = r0; e < diagpos[i]; ++e) {
       const int k = colidx[e];
-      pinfo[e] = PivInfo{diagpos[k], rowptr[k + 1], (int)map.size(), 0};
+      pinfo[e] = PivInfo{diagpos[k], rowptr[k + 1], (int)map.size(), k};
       // row i's columns after position e are the ones above column k
       const int *b = colidx.data() + e + 1, *end = colidx.data() + r1;
       for (int j = diagpos[k] + 1; j < rowptr[k + 1]; ++j) {

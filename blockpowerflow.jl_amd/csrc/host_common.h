@@ -80,15 +80,15 @@ void analyze_triop(TriLevels &op, int n, const std::vector<int> &rowptr,
 // ------------------------------------------------- elimination plan
 // Elimination plan of an IKJ factorization on a fixed pattern (ILU(0) on
 // A's pattern, LU on the fill pattern), built once per pattern: for every
-// strictly lower entry e = (i, k), PivInfo at e holds pivot row k's diagonal
-// position and end and where its update positions start in `map`: for each
+// strictly lower entry e = (i, k), PivInfo at e holds pivot row k, its
+// diagonal position and end and where its update positions start in `map`: for each
 // upper entry (k, j) of row k, in order, the position of (i, j) relative to
 // row i's start, or -1 where row i has no entry (a fill ILU(0) drops).  So
 // no pivot step searches row i's pattern and the loads of a pivot depend on
 // nothing computed during the elimination.  `map` ends with one -1 (a pivot
 // row without upper entries still reads one entry).
 struct PivInfo {
-  int diag, end, moff, pad_;
+  int diag, end, moff, row;
 };
 void elim_plan_host(int n, const std::vector<int> &rowptr, const std::vector<int> &colidx,
                     const std::vector<int> &diagpos, std::vector<PivInfo> &pinfo,

@@ -421,6 +421,7 @@ struct FacDev {
   const int *rowptr, *colidx, *diagpos, *order, *lvl;
   const PivInfo *pinfo;   // ILU / LU elimination plan: pivot rows per lower
   const int *map;         // entry, and the update positions of each pivot
+  double *rdiag;          // 1 / U[k][k] per row k, written once row k is eliminated
   double *fac;
   int *status;
   int cap;  // LDS entries per wave
@@ -453,14 +454,14 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
   if (dloc > 0) {
     struct Piv {
       int ulen, moff, d, pos0, pos1;
-      double ukk, val0, val1;
+      double ukk, rukk, val0, val1;
     };
     const PivInfo *pinfo = f.pinfo + base;
     int wbase = 0;
     PivInfo wpi = pinfo[min(lane, dloc - 1)];
     auto fetch = [&](int kk) {
       kk = min(kk, dloc - 1);
-      int d, end, moff;
+      int d, end, moff, k;
       if (kk >= wbase + 64) {
         // next 64 pivots (reached exactly at the window's end: kk == wbase)
         wbase += 64;
@@ -468,16 +469,19 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
         d = __builtin_amdgcn_readlane(wpi.diag, 0);
         end = __builtin_amdgcn_readlane(wpi.end, 0);
         moff = __builtin_amdgcn_readlane(wpi.moff, 0);
+        k = __builtin_amdgcn_readlane(wpi.row, 0);
       } else {
         d = __builtin_amdgcn_readlane(wpi.diag, kk - wbase);
         end = __builtin_amdgcn_readlane(wpi.end, kk - wbase);
         moff = __builtin_amdgcn_readlane(wpi.moff, kk - wbase);
+        k = __builtin_amdgcn_readlane(wpi.row, kk - wbase);
       }
       Piv q;
       q.d = d;
       q.ulen = end - d - 1;
       q.moff = moff;
       q.ukk = f.fac[d];
+      q.rukk = f.rdiag[k];
       // (an empty upper part reads entry 0: row k + 1's first value and the
       // next pivot's first position, or the map's final padding entry)
       const int last = max(q.ulen - 1, 0);
@@ -493,14 +497,14 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
       const int k = s_col[kk];
       if (k >= kmax) return false;
       const double ukk = cur.ukk;
-      const double l = s_val[kk] / ukk;
+      const double l = s_val[kk] * cur.rukk;   // the multiplier: a product with the pivot's reciprocal
       if (lane < cur.ulen && cur.pos0 >= 0) s_val[cur.pos0] -= l * cur.val0;
       if (lane + 64 < cur.ulen && cur.pos1 >= 0) s_val[cur.pos1] -= l * cur.val1;
       for (int u = 128 + lane; u < cur.ulen; u += 64) {
         const int pos = f.map[cur.moff + u];
         if (pos >= 0) s_val[pos] -= l * f.fac[cur.d + 1 + u];
       }
-      wave_sync();
+      // (the updates only reach positions after kk)
       if (lane == 0) {
         s_val[kk] = l;
         if (ukk == 0.0) atomicMin(f.status, k);
@@ -521,7 +525,10 @@ __device__ void ilu_row(const FacDev &f, int i, int lane, int *s_col, double *s_
     }
   }
   for (int q = lane; q < len; q += 64) f.fac[base + q] = s_val[q];
-  if (lane == 0 && kmax > i && s_val[dloc] == 0.0) atomicMin(f.status, i);
+  if (lane == 0 && kmax > i) {
+    if (s_val[dloc] == 0.0) atomicMin(f.status, i);
+    f.rdiag[i] = 1.0 / s_val[dloc];
+  }
   wave_sync();
 }
 
@@ -975,7 +982,9 @@ static void launch_factor(bool ic, hipStream_t s, TriOp &lower, const int *rowpt
   if (per_wave > 64 * 1024)
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
-  FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), pinfo, map, fac, d_status, cap};
+  double *rdiag = eplan ? eplan->rdiag.as<double>() : nullptr;
+  FacDev f{rowptr, colidx, diagpos, lower.order.as<int>(), lower.lvl.as<int>(), pinfo, map, rdiag, fac, d_status,
+           cap};
   const int tok = prof::begin(s);
   if (g_syncfree && lower.n > 0) {
     const size_t shm = per_wave * wpb;
@@ -1028,8 +1037,8 @@ void launch_partial_factor(hipStream_t s, int r0, int r1, int kmax, const int *r
   if (per_wave > 64 * 1024)
     BPF_THROW(BPF_E_UNSUPPORTED, "factor row with %d entries exceeds the LDS capacity", max_row_nnz);
   const int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, (64 * 1024) / per_wave));
-  FacDev f{rowptr, colidx, diagpos, nullptr, nullptr, plan.pinfo.as<PivInfo>(), plan.map.as<int>(), fac, d_status,
-           cap};
+  FacDev f{rowptr, colidx, diagpos, nullptr, nullptr, plan.pinfo.as<PivInfo>(), plan.map.as<int>(),
+           plan.rdiag.as<double>(), fac, d_status, cap};
   const int tok = prof::begin(s);
   hipLaunchKernelGGL(fac_partial_kernel, dim3((r1 - r0 + wpb - 1) / wpb), dim3(64 * wpb), per_wave * wpb, s, f,
                      r0, r1, kmax);
