@@ -644,12 +644,13 @@ void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double 
 }
 
 void backward_error_rm(bpf_ctx ctx, bpf_csr A, int p, const double *R, const double *X,
-                       const double *B, double *h_berr, DevBuf &stat) {
+                       const double *B, double *h_berr, DevBuf &stat, bool b_norms_known) {
+  BPF_REQUIRE(!b_norms_known || stat.bytes >= 3 * (size_t)p * sizeof(double), "B's norms are not in stat");
   stat.ensure(3 * (size_t)p * sizeof(double));
   double *d = stat.as<double>();
   colstat_rowmajor<double>(ctx->stream, A->n, p, R, p, 0, d);
   colstat_rowmajor<double>(ctx->stream, A->n, p, X, p, 0, d + p);
-  colstat_rowmajor<double>(ctx->stream, A->n, p, B, p, 0, d + 2 * p);
+  if (!b_norms_known) colstat_rowmajor<double>(ctx->stream, A->n, p, B, p, 0, d + 2 * p);
   std::vector<double> h(3 * (size_t)p);
   BPF_CHECK_HIP(hipMemcpyAsync(h.data(), d, 3 * (size_t)p * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));

@@ -226,9 +226,11 @@ struct WideDense {
 
 // Row-major FP64 residual R = B - A X.
 void residual_rm(hipStream_t s, bpf_csr A, int p, const double *B, const double *X, double *R);
-// Per-column backward error from row-major FP64 R, X, B; host output.
+// Per-column backward error from row-major FP64 R, X, B; host output.  With
+// b_norms_known, B's column norms are the ones an earlier call on the same B
+// left in stat.
 void backward_error_rm(bpf_ctx ctx, bpf_csr A, int p, const double *R, const double *X,
-                       const double *B, double *h_berr, DevBuf &stat);
+                       const double *B, double *h_berr, DevBuf &stat, bool b_norms_known = false);
 void scale_columns(hipStream_t s, int n, int p, double *X, int ld, const double *d_scale);
 
 }  // namespace bpf

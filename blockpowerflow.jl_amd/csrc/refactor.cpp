@@ -28,9 +28,11 @@ void launch_perm_cm_to_rm(hipStream_t s, int n, int p, const double *src, int ld
 template <typename TS>
 void launch_perm_rm_to_cm(hipStream_t s, int n, int p, const TS *src, int lds, const int *rowmap,
                           double *dst, int ldd);
+// Row-major rows by a row map: dst[k] = src[map[k]] (gather), dst[map[k]] =
+// src[k] (scatter) or, with add, dst[map[k]] += src[k].
 template <typename TS, typename TD>
 void launch_perm_rows_rm(hipStream_t s, int n, int p, const TS *src, const int *rowmap, TD *dst,
-                         bool scatter);
+                         bool scatter, bool add = false);
 }  // namespace bpf
 
 struct bpf_rf_s {
@@ -365,6 +367,7 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
     launch_perm_rows_rm<float, double>(s, n, p, z, rf->dQ.as<int>(), X, true);
     const bool adaptive = berr_tol > 0.0;
     std::vector<double> be(p, 0.0);
+    bool bnorms = false;
     int steps = 0;
     for (int it = 0; it <= refine; ++it) {
       const bool last = it == refine;
@@ -372,15 +375,14 @@ int bpf_rf_solve_refined(bpf_rf rf, bpf_csr A, int nrhs, const double *d_B, int 
       if (last && !want_berr) break;
       residual_rm(s, A, p, B, X, R);
       if (want_berr) {
-        backward_error_rm(rf->ctx, A, p, R, X, B, be.data(), rf->stat);
+        backward_error_rm(rf->ctx, A, p, R, X, B, be.data(), rf->stat, bnorms);
+        bnorms = true;   // B's column norms stay in rf->stat for the later residuals
         if (adaptive && std::all_of(be.begin(), be.end(), [&](double e) { return e <= berr_tol; })) break;
       }
       if (last) break;
       launch_perm_rows_rm<double, float>(s, n, p, R, rf->dP.as<int>(), z, false);
       tri_solve<float>(rf, p, z);
-      // X[Q[k]] += z[k]: scatter into R (free now), then add.
-      launch_perm_rows_rm<float, double>(s, n, p, z, rf->dQ.as<int>(), R, true);
-      axpby(s, m, 1.0, R, 1.0, X);
+      launch_perm_rows_rm<float, double>(s, n, p, z, rf->dQ.as<int>(), X, true, true);   // X[Q[k]] += z[k]
       ++steps;
     }
     if (h_berr) std::copy(be.begin(), be.end(), h_berr);

@@ -1132,13 +1132,20 @@ __global__ __launch_bounds__(kBlock) void perm_rm_to_cm_kernel(int n, int p, con
 
 // row-major n x p: gather dst[k] = src[map[k]] or scatter dst[map[k]] = src[k]
 template <typename TS, typename TD>
+// Row-major n x p rows moved by a row map, one wavefront per row: mode 0
+// dst[k] = src[map[k]], 1 dst[map[k]] = src[k], 2 dst[map[k]] += src[k].
 __global__ __launch_bounds__(kBlock) void perm_rows_kernel(int n, int p, const TS *src, const int *map,
-                                                          TD *dst, int scatter) {
-  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= (size_t)n * p) return;
-  const int k = (int)(e / p), c = (int)(e % p);
-  if (scatter) dst[(size_t)map[k] * p + c] = static_cast<TD>(src[(size_t)k * p + c]);
-  else dst[(size_t)k * p + c] = static_cast<TD>(src[(size_t)map[k] * p + c]);
+                                                          TD *dst, int mode) {
+  const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int mk = map[k];
+  const TS *sr = src + (size_t)(mode == 0 ? mk : k) * p;
+  TD *dr = dst + (size_t)(mode == 0 ? k : mk) * p;
+  for (int c = lane; c < p; c += 64) {
+    const TD v = static_cast<TD>(sr[c]);
+    dr[c] = mode == 2 ? dr[c] + v : v;
+  }
 }
 
 }  // namespace
@@ -1176,15 +1183,18 @@ template void launch_perm_rm_to_cm<float>(hipStream_t, int, int, const float *, 
 
 template <typename TS, typename TD>
 void launch_perm_rows_rm(hipStream_t s, int n, int p, const TS *src, const int *rowmap, TD *dst,
-                         bool scatter) {
-  const size_t m = (size_t)n * p;
-  if (m == 0) return;
-  hipLaunchKernelGGL((perm_rows_kernel<TS, TD>), dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock),
-                     0, s, n, p, src, rowmap, dst, scatter ? 1 : 0);
+                         bool scatter, bool add) {
+  if (n <= 0 || p <= 0) return;
+  const int rpb = kBlock / 64;
+  hipLaunchKernelGGL((perm_rows_kernel<TS, TD>), dim3((unsigned)((n + rpb - 1) / rpb)), dim3(kBlock), 0, s, n, p,
+                     src, rowmap, dst, add ? 2 : (scatter ? 1 : 0));
   BPF_CHECK_HIP(hipGetLastError());
 }
-template void launch_perm_rows_rm<double, float>(hipStream_t, int, int, const double *, const int *, float *, bool);
-template void launch_perm_rows_rm<float, double>(hipStream_t, int, int, const float *, const int *, double *, bool);
-template void launch_perm_rows_rm<double, double>(hipStream_t, int, int, const double *, const int *, double *, bool);
+template void launch_perm_rows_rm<double, float>(hipStream_t, int, int, const double *, const int *, float *, bool,
+                                                  bool);
+template void launch_perm_rows_rm<float, double>(hipStream_t, int, int, const float *, const int *, double *, bool,
+                                                  bool);
+template void launch_perm_rows_rm<double, double>(hipStream_t, int, int, const double *, const int *, double *, bool,
+                                                   bool);
 
 }  // namespace bpf

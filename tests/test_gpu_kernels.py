@@ -184,3 +184,36 @@ def test_spmm_autotuned_shape_is_exact(bpf, gen, oracle, p):
     np.testing.assert_array_equal(out[0], out[1])
     ref = oracle.spmm(S.indptr, S.indices, S.data, S.B)
     np.testing.assert_allclose(out[1].T, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("p", [1, 16, 128, 200])
+def test_fp32_triangular_solve_backward_error(bpf, gen, p):
+    # The FP32 level-scheduled triangular solve (the mixed form's inner
+    # preconditioner and the refactorization's FP32 solves) has a
+    # componentwise backward error of a few FP32 roundings per row.  On a
+    # lower-triangular A, ILU(0) is exact (L = A D^-1, U = D), so its FP32
+    # application y satisfies |x - A y| <= (k + 4) eps32 |A| |y| (k the
+    # longest row, x the FP32 input; residual in FP64).  A lost or reordered
+    # term, a wrong level order or a wrong diagonal breaks it by orders of
+    # magnitude.
+    _, S = _system(gen, 2000, p, 24)
+    T = sp.tril(S.scipy()).tocsr()
+    T.sort_indices()
+    A = bpf.HipSparseMatrixCSR(T.indptr.astype(np.int32), T.indices.astype(np.int32), T.data)
+    P = bpf.ilu0(A, p)
+    x32 = np.asfortranarray(S.B.astype(np.float32))
+    X = bpf.HipMatrix.from_numpy(x32, dtype=np.float32)
+    Y = bpf.HipMatrix(S.B.shape, np.float32)
+    bpf.ldiv_(Y, P, X)
+    y = Y.to_numpy().astype(np.float64)
+    r = x32.astype(np.float64) - T @ y
+    kmax = int(np.diff(T.indptr).max())
+    bound = (kmax + 4) * np.finfo(np.float32).eps * (abs(T) @ np.abs(y))
+    assert np.all(np.abs(r) <= bound + 1e-300), np.max(np.abs(r) / (bound + 1e-300))
+    # and the FP64 application of the same factor solves T y = x to FP64 rounding
+    X64 = bpf.HipMatrix.from_numpy(x32.astype(np.float64))
+    Y64 = bpf.HipMatrix(S.B.shape)
+    bpf.ldiv_(Y64, P, X64)
+    y64 = Y64.to_numpy()
+    r64 = x32.astype(np.float64) - T @ y64
+    assert np.all(np.abs(r64) <= (kmax + 4) * np.finfo(np.float64).eps * (abs(T) @ np.abs(y64)) + 1e-300)
