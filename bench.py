@@ -62,7 +62,8 @@ KERNELS = {
     "spmm": ("spmm_rm_kernel", "hbm", "spmm_rm_kernel"),
     "trsv": ("tri_level_kernel, tri_tail_kernel", "hbm", "tri_level_kernel"),
     "factor": ("fac_level_kernel, fac_tail_kernel, fac_partial_kernel", "hbm", "fac_level_kernel"),
-    "dense_lu": ("dlu_diag_kernel, dgemm_kernel, copy_block_kernel", "mfma", "dense_lu"),
+    "dense_lu": ("dlu_diag_kernel, dgemm_batch_kernel, copy_block_kernel, diag_inv_copy_kernel", "mfma",
+                 "dense_lu"),
     "proj": ("proj_partial_kernel or proj16_kernel, then proj_reduce_kernel", "mfma", "projection"),
     "gemm": ("gemm128d_kernel (FP64 128-wide) or block_gemm_kernel", "mfma", "block_gemm_kernel"),
     "proj16": ("proj16_kernel or proj_partial_kernel (widths <= 32), then proj_reduce_kernel", "hbm",
