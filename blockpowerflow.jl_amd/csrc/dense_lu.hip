@@ -665,6 +665,8 @@ void dense_tail_move(hipStream_t s, int cnt, const int *pos, const int *didx, do
 
 void dense_lu_factor(hipStream_t s, int m, double *S, double *Linv, double *Uinv, double *P, double *Q,
                      int *status, int base) {
+  // (16-row GEMM tiles; the diagonal-block kernel moves column pairs of S)
+  BPF_REQUIRE(m % 16 == 0 && reinterpret_cast<uintptr_t>(S) % 16 == 0, "dense tail of %d rows", m);
   const int nb = kDluNb;
   const int tok = prof::begin(s);
   double flops = 0.0;
