@@ -309,6 +309,11 @@ constexpr int kTailBlock = 1024;
 // over several wavefronts; the partial sums meet in LDS.  Deep elimination
 // tails (one long row per level, as in fill-reducing LU orderings) then run
 // at the speed of the whole workgroup instead of one wavefront.
+// The segment's level bounds, row pointers, row numbers and diagonals are
+// staged in LDS first when they fit (at most kTailRows rows and kTailLevels
+// levels), and each row's right-hand side and diagonal are read before the
+// barrier that precedes the row's final combination.
+constexpr int kTailRows = 2048, kTailLevels = 1024;
 template <typename TF, typename TV, bool UNIT, int CPL, bool VEC>
 __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, int l1,
                                                               const TV *x, int ldx, TV *y,
@@ -321,14 +326,33 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
   }
   constexpr int NW = kTailBlock / 64;
   __shared__ TV part[NW][64 * CPL];
+  __shared__ int s_lvl[kTailLevels + 1];
+  __shared__ int s_ptr[kTailRows + 1], s_ord[kTailRows];
+  __shared__ TF s_dg[kTailRows];
+  const TF *val = static_cast<const TF *>(a.val);
+  const TF *dg = static_cast<const TF *>(a.diag);
+  const int T0 = a.lvl[l0], T1 = a.lvl[l1];
+  const bool staged = T1 - T0 <= kTailRows && l1 - l0 <= kTailLevels;
+  if (staged) {
+    for (int e = threadIdx.x; e <= l1 - l0; e += kTailBlock) s_lvl[e] = a.lvl[l0 + e] - T0;
+    for (int e = threadIdx.x; e <= T1 - T0; e += kTailBlock) s_ptr[e] = a.ptr[T0 + e];
+    for (int e = threadIdx.x; e < T1 - T0; e += kTailBlock) {
+      s_ord[e] = a.order[T0 + e];
+      if (!UNIT) s_dg[e] = dg[T0 + e];
+    }
+    __syncthreads();
+  }
+  auto lvl_at = [&](int l) { return staged ? T0 + s_lvl[l - l0] : a.lvl[l]; };
+  auto ptr_at = [&](int t) { return staged ? s_ptr[t - T0] : a.ptr[t]; };
+  auto ord_at = [&](int t) { return staged ? s_ord[t - T0] : a.order[t]; };
+  auto dg_at = [&](int t) { return staged ? s_dg[t - T0] : dg[t]; };
   const int G = 1 << glog;
   const int ngroups = kTailBlock >> glog;
   const int gid = threadIdx.x >> glog;
   const int lane = threadIdx.x & (G - 1);
   const bool can_split = G == 64 && p <= 64 * CPL;
-  const TF *val = static_cast<const TF *>(a.val);
   for (int l = l0; l < l1; ++l) {
-    const int t0 = a.lvl[l], t1 = a.lvl[l + 1];
+    const int t0 = lvl_at(l), t1 = lvl_at(l + 1);
     const int rows = t1 - t0;
     if (!can_split || rows >= NW) {
       for (int t = t0 + gid; t < t1; t += ngroups)
@@ -336,44 +360,53 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
     } else {
       const int wpr = NW / rows;          // wavefronts per row
       const int r = gid / wpr, sub = gid % wpr;
+      int i = 0;
+      TV xr[CPL];
+      TV dinv = TV(1);
       if (r < rows) {
         const int t = t0 + r;
-        const int k0 = a.ptr[t], k1 = a.ptr[t + 1];
+        const int k0 = ptr_at(t), k1 = ptr_at(t + 1);
+        if (sub == 0) {
+          i = ord_at(t);
+          if (!UNIT) dinv = TV(1) / static_cast<TV>(dg_at(t));
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const int c = lane + m * 64;
+            xr[m] = c < p ? x[(size_t)i * ldx + c] : TV(0);
+          }
+        }
         TV acc[CPL];
 #pragma unroll
         for (int m = 0; m < CPL; ++m) acc[m] = TV(0);
         TV acc2[CPL];
 #pragma unroll
         for (int m = 0; m < CPL; ++m) acc2[m] = TV(0);
-        int k = k0 + sub;
-        // four non-zeros per step: their column indices, values and y rows
-        // are loaded together, two accumulator sets
-        for (; k + 3 * wpr < k1; k += 4 * wpr) {
-          const int j0 = a.col[k], j1 = a.col[k + wpr], j2 = a.col[k + 2 * wpr], j3 = a.col[k + 3 * wpr];
-          const TV v0 = static_cast<TV>(val[k]), v1 = static_cast<TV>(val[k + wpr]);
-          const TV v2 = static_cast<TV>(val[k + 2 * wpr]), v3 = static_cast<TV>(val[k + 3 * wpr]);
-          const TV *y0 = y + (size_t)j0 * ldy, *y1 = y + (size_t)j1 * ldy;
-          const TV *y2 = y + (size_t)j2 * ldy, *y3 = y + (size_t)j3 * ldy;
+        // four of the wavefront's non-zeros per step (every wpr-th of the
+        // row): their column indices and values, then their y rows, loaded
+        // together; the last step's missing entries read the row's last
+        // entry and are skipped; two accumulator sets
+        for (int k = k0 + sub; k < k1; k += 4 * wpr) {
+          int j[4];
+          TV v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int kk = min(k + u * wpr, k1 - 1);
+            j[u] = a.col[kk];
+            v[u] = static_cast<TV>(val[kk]);
+          }
+          const int nv = (k1 - k + wpr - 1) / wpr;   // entries of this step that exist
 #pragma unroll
           for (int m = 0; m < CPL; ++m) {
             const int c = lane + m * 64;
             if (c < p) {
-              const TV a0 = y0[c], a1 = y1[c], a2 = y2[c], a3 = y3[c];
-              acc[m] = fma(v0, a0, acc[m]);
-              acc2[m] = fma(v1, a1, acc2[m]);
-              acc[m] = fma(v2, a2, acc[m]);
-              acc2[m] = fma(v3, a3, acc2[m]);
-            }
-          }
-        }
-        for (; k < k1; k += wpr) {
-          const int j = a.col[k];
-          const TV v = static_cast<TV>(val[k]);
-          const TV *yr = y + (size_t)j * ldy;
+              TV av[4];
 #pragma unroll
-          for (int m = 0; m < CPL; ++m) {
-            const int c = lane + m * 64;
-            if (c < p) acc[m] = fma(v, yr[c], acc[m]);
+              for (int u = 0; u < 4; ++u) av[u] = y[(size_t)j[u] * ldy + c];
+              acc[m] = fma(v[0], av[0], acc[m]);
+              if (nv > 1) acc2[m] = fma(v[1], av[1], acc2[m]);
+              if (nv > 2) acc[m] = fma(v[2], av[2], acc[m]);
+              if (nv > 3) acc2[m] = fma(v[3], av[3], acc2[m]);
+            }
           }
         }
 #pragma unroll
@@ -381,15 +414,11 @@ __global__ __launch_bounds__(kTailBlock) void tri_tail_kernel(TriDev a, int l0, 
       }
       __syncthreads();
       if (r < rows && sub == 0) {
-        const int t = t0 + r;
-        const int i = a.order[t];
-        TV dinv = TV(1);
-        if (!UNIT) dinv = TV(1) / static_cast<TV>(static_cast<const TF *>(a.diag)[t]);
 #pragma unroll
         for (int m = 0; m < CPL; ++m) {
           const int c = lane + m * 64;
           if (c < p) {
-            TV s = x[(size_t)i * ldx + c];
+            TV s = xr[m];
             for (int q = 0; q < wpr; ++q) s -= part[gid + q][c];
             y[(size_t)i * ldy + c] = UNIT ? s : s * dinv;
           }
