@@ -444,7 +444,11 @@ struct GemmBatch {
 // One workgroup per (16 TM) x (16 TN) output tile of one GEMM of the batch;
 // its four wavefronts split the k range (wavefront w takes the 16-wide
 // chunks w, w + 4, ...) and their partial tiles are summed in LDS in a fixed
-// order (bitwise reproducible).  Operands come straight from global memory:
+// order (bitwise reproducible).  Workgroups go to the 8 XCDs in turn, so
+// when the tile count is a multiple of 8 workgroup b takes tile
+// (b % 8) * (ntiles / 8) + b / 8: each XCD gets a contiguous run of tiles
+// (the column tiles of the same row tiles) and fetches its A rows into its
+// own L2 once.  Operands come straight from global memory:
 // per chunk, lane group g = lane / 16 owns k = k0 + 4g .. k0 + 4g + 3 (a
 // 32-byte run of each of its A rows and four B rows), so MFMA step s pairs A
 // and B on k = k0 + 4g + s.  The next chunk's operands are loaded before the
@@ -452,7 +456,7 @@ struct GemmBatch {
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void dgemm_batch_kernel(GemmBatch b) {
   __shared__ dbl4 red[3][TM * TN][64];
-  const int t = blockIdx.x;
+  const int t = b.ntiles % 8 == 0 ? (int)(blockIdx.x % 8) * (b.ntiles / 8) + (int)(blockIdx.x / 8) : blockIdx.x;
   int id = 0;
   while (id + 1 < b.nd && t >= b.d[id + 1].tile0) ++id;
   const GemmDesc g = b.d[id];
