@@ -1771,16 +1771,7 @@ __global__ __launch_bounds__(kB) void colstat_rm_kernel(int n, int p, const T *X
   const int rstride = (kB / 64) * gridDim.x;
   double m = 0.0;
   if (c < p) {
-    // four rows' loads in flight, combined in row order
-    int r = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
-    for (; r + 3 * rstride < n; r += 4 * rstride) {
-      double v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = to_d(X[(size_t)(r + u * rstride) * ld + c]);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) m = mode == 0 ? fmax(m, fabs(v[u])) : m + v[u] * v[u];
-    }
-    for (; r < n; r += rstride) {
+    for (int r = blockIdx.x * (kB / 64) + (threadIdx.x >> 6); r < n; r += rstride) {
       const double v = to_d(X[(size_t)r * ld + c]);
       m = mode == 0 ? fmax(m, fabs(v)) : m + v * v;
     }
